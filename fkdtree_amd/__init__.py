@@ -1,0 +1,309 @@
+"""MI355X-native FKDTree: k-d tree build and box search in gfx950 HIP kernels.
+
+Python mirror of the reference interface ``FKDTree<float, dim>``
+(``FKDTree.h:6-52``) over the C ABI of ``include/fkdtree/fkd_capi.h``:
+
+* ``FKDTree(points)``            - constructor (copies the points to the GPU)
+* ``build()``                    - ``FKDTree_cpu.h:221-277``
+* ``getDimensionVector(d)``      - heap-ordered coordinates (``:209-211``)
+* ``getIdVector()``              - heap-ordered ids (``:212-213``)
+* ``search_in_the_box(lo, hi)``  - ids in the closed box (``:123-169``)
+* ``search_in_the_box_multiple`` - per-box counts, the OpenCL backend's result
+                                   (``FKDTree_opencl.h:123-164``)
+* ``search_in_the_box_multiple_csr`` - per-box ids as (offsets, ids)
+
+``FKDEvents`` holds many independent trees (events) built and searched
+together.  Device buffers (``DeviceArray``), streams and timers are thin
+wrappers of the same library; nothing here falls back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import gen  # noqa: F401
+from ._lib import FKDError, check, lib
+
+__all__ = ["FKDTree", "FKDEvents", "DeviceArray", "Stream", "Timer", "FKDError", "device_info"]
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(256)
+    cus = C.c_int()
+    fr = C.c_size_t()
+    tot = C.c_size_t()
+    check(lib().fkd_device_info(name, 256, C.byref(cus), C.byref(fr), C.byref(tot)))
+    return {"name": name.value.decode(), "compute_units": cus.value, "free_bytes": fr.value,
+            "total_bytes": tot.value}
+
+
+class Stream:
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().fkd_stream_create(C.byref(h)))
+        self.handle = h.value
+
+    def sync(self):
+        check(lib().fkd_stream_sync(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().fkd_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP events on a stream: elapsed milliseconds between two records."""
+
+    def __init__(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        check(lib().fkd_event_create(C.byref(a)))
+        check(lib().fkd_event_create(C.byref(b)))
+        self.a, self.b = a.value, b.value
+
+    def start(self, stream=None):
+        check(lib().fkd_event_record(self.a, _sh(stream)))
+
+    def stop(self, stream=None):
+        check(lib().fkd_event_record(self.b, _sh(stream)))
+
+    def ms(self) -> float:
+        v = C.c_float()
+        check(lib().fkd_event_elapsed_ms(self.a, self.b, C.byref(v)))
+        return float(v.value)
+
+    def __del__(self):
+        try:
+            lib().fkd_event_destroy(self.a)
+            lib().fkd_event_destroy(self.b)
+        except Exception:
+            pass
+
+
+def _sh(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+class DeviceArray:
+    """A device allocation holding a numpy-typed array."""
+
+    def __init__(self, shape, dtype, ptr: Optional[int] = None):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self._own = ptr is None
+        if ptr is None:
+            p = C.c_void_p()
+            check(lib().fkd_malloc(C.byref(p), max(self.nbytes, 1)))
+            ptr = p.value
+        self.ptr = ptr
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, stream=None) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        check(lib().fkd_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, _sh(stream)))
+        check(lib().fkd_stream_sync(_sh(stream)))
+
+    def download(self, stream=None, count: Optional[int] = None) -> np.ndarray:
+        if count is None:
+            out = np.empty(self.shape, dtype=self.dtype)
+        else:
+            out = np.empty(int(count), dtype=self.dtype)
+        check(lib().fkd_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes, _sh(stream)))
+        return out
+
+    def free(self):
+        if self._own and self.ptr:
+            lib().fkd_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _as_soa(coords, dim=None) -> np.ndarray:
+    c = np.ascontiguousarray(coords, dtype=np.float32)
+    if c.ndim != 2:
+        raise ValueError("coords must be 2-D [dim, n]")
+    return c
+
+
+def _points_aos(points=None, coords=None, ids=None) -> Tuple[int, int, np.ndarray]:
+    """(dim, n, AoS uint32 records) from FKDPoint-like records or SoA coords+ids."""
+    if points is not None:
+        rec = np.ascontiguousarray(points)
+        if rec.dtype != np.uint32:
+            raise ValueError("points must be uint32 records [n, dim+1] (coordinate bits, id)")
+        n, w = rec.shape
+        return w - 1, n, rec
+    c = _as_soa(coords)
+    dim, n = c.shape
+    i = np.arange(n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+    rec = np.empty((n, dim + 1), dtype=np.uint32)
+    rec[:, :dim] = c.T.view(np.uint32)
+    rec[:, dim] = i
+    return dim, n, rec
+
+
+class _TreeBase:
+    _h = None
+
+    def _init_handle(self, h: C.c_void_p, stream):
+        self._h = h.value
+        self.stream = stream
+        d = C.c_int()
+        e = C.c_uint32()
+        n = C.c_uint64()
+        check(lib().fkd_tree_info(self._h, C.byref(d), C.byref(e), C.byref(n)))
+        self.dim, self.n_events, self.n = d.value, e.value, n.value
+        self._mirror = None
+
+    def build(self) -> None:
+        check(lib().fkd_build(self._h, _sh(self.stream)))
+        check(lib().fkd_stream_sync(_sh(self.stream)))
+        self._mirror = None
+
+    def build_async(self) -> None:
+        """Enqueue the build on the tree's stream and return."""
+        check(lib().fkd_build(self._h, _sh(self.stream)))
+        self._mirror = None
+
+    def tree_arrays(self) -> Tuple[np.ndarray, np.ndarray]:
+        """Heap-ordered SoA coordinates [dim, n] and ids [n] (host copies)."""
+        if self._mirror is None:
+            c = np.empty((self.dim, self.n), dtype=np.float32)
+            i = np.empty(self.n, dtype=np.uint32)
+            check(lib().fkd_get_tree(self._h, c.ctypes.data, i.ctypes.data, _sh(self.stream)))
+            self._mirror = (c, i)
+        return self._mirror
+
+    def device_tree(self) -> Tuple[int, int]:
+        c, i = C.c_void_p(), C.c_void_p()
+        check(lib().fkd_tree_device(self._h, C.byref(c), C.byref(i)))
+        return c.value, i.value
+
+    def _search_host(self, mins, maxs, qev=None) -> Tuple[np.ndarray, np.ndarray]:
+        mins = _as_soa(mins)
+        maxs = _as_soa(maxs)
+        if mins.shape != maxs.shape or mins.shape[0] != self.dim:
+            raise ValueError("mins/maxs must be [dim, nq]")
+        nq = mins.shape[1]
+        off = np.empty(nq + 1, dtype=np.uint64)
+        qp = None
+        if qev is not None:
+            qev = np.ascontiguousarray(qev, dtype=np.uint32)
+            qp = qev.ctypes.data
+        check(lib().fkd_search_host(self._h, nq, mins.ctypes.data, maxs.ctypes.data, qp, off.ctypes.data,
+                                    _sh(self.stream)))
+        ids = np.empty(int(off[-1]), dtype=np.uint32)
+        check(lib().fkd_fetch_results(self._h, ids.ctypes.data, _sh(self.stream)))
+        return off, ids
+
+    def search_device(self, nq: int, d_mins: int, d_maxs: int, d_offsets: int, d_out: Optional[int],
+                      capacity: int, d_qev: Optional[int] = None, stream=None, count: bool = True,
+                      fill: bool = True) -> None:
+        """Enqueue count+scan and/or fill on device buffers (raw pointers)."""
+        s = _sh(stream if stream is not None else self.stream)
+        if count:
+            check(lib().fkd_search_count(self._h, nq, d_mins, d_maxs, d_qev, d_offsets, s))
+        if fill:
+            check(lib().fkd_search_fill(self._h, nq, d_mins, d_maxs, d_qev, d_offsets, d_out, capacity, s))
+
+    def close(self):
+        if self._h:
+            lib().fkd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FKDTree(_TreeBase):
+    """One k-d tree, ``FKDTree<float, dim>`` on the GPU."""
+
+    def __init__(self, points=None, *, coords=None, ids=None, stream=None):
+        dim, n, rec = _points_aos(points, coords, ids)
+        h = C.c_void_p()
+        check(lib().fkd_create(dim, n, rec.ctypes.data if n else None, _sh(stream), C.byref(h)))
+        self._init_handle(h, stream)
+
+    @classmethod
+    def from_device(cls, dim: int, n: int, d_coords: int, d_ids: int, stream=None) -> "FKDTree":
+        self = cls.__new__(cls)
+        off = np.array([0, n], dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().fkd_create_device(dim, 1, off.ctypes.data, d_coords, d_ids, _sh(stream), C.byref(h)))
+        self._init_handle(h, stream)
+        return self
+
+    # Reference-named accessors (FKDTree.h:20-22)
+    def getDimensionVector(self, dimension: int) -> np.ndarray:
+        c, _ = self.tree_arrays()
+        return c[dimension] if 0 <= dimension < self.dim else c[0]
+
+    def getIdVector(self) -> np.ndarray:
+        return self.tree_arrays()[1]
+
+    def search_in_the_box(self, min_point: Sequence[float], max_point: Sequence[float]) -> np.ndarray:
+        lo = np.asarray(min_point, dtype=np.float32).reshape(self.dim, 1)
+        hi = np.asarray(max_point, dtype=np.float32).reshape(self.dim, 1)
+        _, ids = self._search_host(lo, hi)
+        return ids
+
+    def search_in_the_box_multiple(self, mins, maxs) -> np.ndarray:
+        off, _ = self._search_host(mins, maxs)
+        return np.diff(off).astype(np.uint32)
+
+    def search_in_the_box_multiple_csr(self, mins, maxs) -> Tuple[np.ndarray, np.ndarray]:
+        return self._search_host(mins, maxs)
+
+
+class FKDEvents(_TreeBase):
+    """Independent trees of many events, built and searched in shared launches."""
+
+    def __init__(self, coords, ids, offsets, stream=None):
+        dim, n, rec = _points_aos(None, coords, ids)
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().fkd_create_events(dim, off.shape[0] - 1, off.ctypes.data, rec.ctypes.data if n else None,
+                                      _sh(stream), C.byref(h)))
+        self.offsets = off
+        self._init_handle(h, stream)
+
+    @classmethod
+    def from_device(cls, dim: int, offsets, d_coords: int, d_ids: int, stream=None) -> "FKDEvents":
+        self = cls.__new__(cls)
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().fkd_create_device(dim, off.shape[0] - 1, off.ctypes.data, d_coords, d_ids, _sh(stream),
+                                      C.byref(h)))
+        self.offsets = off
+        self._init_handle(h, stream)
+        return self
+
+    def search(self, mins, maxs, query_events) -> Tuple[np.ndarray, np.ndarray]:
+        """Query q searches the tree of event query_events[q]: (offsets, ids)."""
+        return self._search_host(mins, maxs, query_events)

@@ -1,0 +1,289 @@
+// Build kernels: key preparation and the per-depth stable partition.
+//
+// Data layout (P = points of the batch, all events end to end):
+//   X[dim][P] float   input coordinates (SoA),  ids[P] uint32
+//   A[m][P]  uint32   point indices; for every dimension m one array kept in
+//                     in-order order of the tree under construction, each
+//                     interval sorted by (coordinate m, id)
+//   R[m][P]  uint32   rank of every point in the (event, coordinate m, id)
+//                     order: comparing ranks equals the reference comparator
+//                     (FKDTree_cpu.h:246-252)
+//   tree outputs: Tc[dim][P] float, Tid[P] uint32 (heap order per event,
+//                 the arrays getDimensionVector/getIdVector return,
+//                 FKDTree_cpu.h:209-213) and packed node records for search.
+#pragma once
+#include "common.hpp"
+#include "radix.hpp"
+
+namespace fkd {
+
+// Record stride (floats) of a packed node: coordinates then the id bits.
+template <int DIM>
+struct NodeRec {
+  static constexpr int kFloats = DIM <= 3 ? 4 : 8;
+};
+
+// AoS FKDPoint<float, DIM> records (DIM floats then a uint32 id) -> SoA.
+template <int DIM>
+__global__ void aos_to_soa_kernel(const uint32_t* __restrict__ aos, float* __restrict__ X,
+                                  uint32_t* __restrict__ ids, uint32_t P) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) X[(uint64_t)m * P + i] = __uint_as_float(aos[(uint64_t)i * (DIM + 1) + m]);
+    ids[i] = aos[(uint64_t)i * (DIM + 1) + DIM];
+  }
+}
+
+// Keys and values for the coordinate sorts.  order == nullptr means point
+// order already is (event, id) order.  Sets bit 0 of *err on a NaN.
+template <int DIM>
+__global__ void keys_kernel(const float* __restrict__ X, const uint32_t* __restrict__ order,
+                            uint32_t P, uint32_t* __restrict__ K, uint32_t* __restrict__ V,
+                            uint64_t stride, uint32_t* err) {
+  bool nan = false;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+    const uint32_t p = order ? order[i] : i;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      const float x = X[(uint64_t)m * P + p];
+      nan |= (x != x);
+      K[m * stride + i] = fkey(x);
+      V[m * stride + i] = p;
+    }
+  }
+  if (nan) atomicOr(err, 1u);
+}
+
+// Keys and values for the id sort (tie order inside every event).
+__global__ void id_keys_kernel(const uint32_t* __restrict__ ids, uint32_t P, uint32_t* __restrict__ K,
+                               uint32_t* __restrict__ V) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+    K[i] = ids[i];
+    V[i] = i;
+  }
+}
+
+constexpr int kLevelItems = 8;
+constexpr int kLevelTile = kThreads * kLevelItems;
+
+template <int DIM>
+struct LevelArgs {
+  uint32_t L;                 // depth whose intervals are split
+  uint32_t d;                 // split dimension L % DIM
+  const uint32_t* Ad;         // array d (not moved at this depth)
+  const uint32_t* Rd;         // ranks in dimension d
+  const uint32_t* src[3];     // the DIM-1 other arrays ...
+  uint32_t* dst[3];           // ... and where their partitioned copies go
+  TileMap tm;                 // tiles of kLevelTile points, event aligned
+  const float* X;             // input points
+  const uint32_t* ids;
+  uint32_t P;
+  float* Tc;                  // tree outputs
+  uint32_t* Tid;
+  float* nodes;
+  uint64_t* desc;             // look-back descriptors, zero at launch
+  uint32_t* ctr;              // dynamic block counter, zero at launch
+};
+
+template <int DIM>
+__device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t node, uint32_t p) {
+  constexpr int RF = NodeRec<DIM>::kFloats;
+  float c[DIM];
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+    c[m] = a.X[(uint64_t)m * a.P + p];
+    a.Tc[(uint64_t)m * a.P + o + node] = c[m];
+  }
+  const uint32_t id = a.ids[p];
+  a.Tid[o + node] = id;
+  float* rec = a.nodes + (uint64_t)(o + node) * RF;
+  if (RF == 4) {
+    float4 v;
+    v.x = c[0];
+    v.y = DIM > 1 ? c[DIM > 1 ? 1 : 0] : 0.f;
+    v.z = DIM > 2 ? c[DIM > 2 ? 2 : 0] : 0.f;
+    v.w = __uint_as_float(id);
+    *reinterpret_cast<float4*>(rec) = v;
+  } else {
+    float4 v0, v1;
+    v0.x = c[0];
+    v0.y = c[DIM > 1 ? 1 : 0];
+    v0.z = c[DIM > 2 ? 2 : 0];
+    v0.w = c[DIM > 3 ? 3 : 0];
+    v1.x = __uint_as_float(id);
+    v1.y = v1.z = v1.w = 0.f;
+    reinterpret_cast<float4*>(rec)[0] = v0;
+    reinterpret_cast<float4*>(rec)[1] = v1;
+  }
+}
+
+// Segmented (reset-at-interval-start) counts of left / right elements.
+struct SegCount {
+  uint32_t f, l, r;
+};
+__device__ inline SegCount seg_op(SegCount a, SegCount b) {
+  SegCount c;
+  c.f = a.f | b.f;
+  c.l = b.f ? b.l : a.l + b.l;
+  c.r = b.f ? b.r : a.r + b.r;
+  return c;
+}
+// Descriptor word: status (2) | f (1) | l (30) | r (30).
+__device__ inline uint64_t seg_pack(SegCount s) {
+  return ((uint64_t)s.f << 60) | ((uint64_t)s.l << 30) | (uint64_t)s.r;
+}
+__device__ inline SegCount seg_unpack(uint64_t w) {
+  SegCount s;
+  s.f = (uint32_t)(w >> 60) & 1u;
+  s.l = (uint32_t)(w >> 30) & 0x3fffffffu;
+  s.r = (uint32_t)w & 0x3fffffffu;
+  return s;
+}
+
+enum : uint32_t { kNone = 0, kLeft = 1, kMed = 2, kRight = 3 };
+
+// One depth of the build for every interval of every event at once.
+// Block b handles moving array b % (DIM-1) over tile b / (DIM-1) (b is the
+// dynamic block index, so look-back waits only on blocks already running).
+// For each interval [start, start+size) of depth L with median rank
+// k = left_size(size) (FKDTree_cpu.h:239): the median is the point at
+// start+k of array d; every other array is stably partitioned into
+// [start, start+k) (rank below the median's), start+k (the median) and
+// (start+k, start+size) (rank above).  Blocks of array 0 also store the
+// depth-L nodes (add_at_position, FKDTree_cpu.h:253-255) and, on the last
+// split depth, the leaves (FKDTree_cpu.h:271-276).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
+  constexpr uint32_t M = DIM - 1;
+  __shared__ uint32_t s_blk;
+  __shared__ SegCount s_wave[kThreads / kWave];
+  __shared__ SegCount s_carry;
+  const uint32_t bid = dynamic_block(a.ctr, &s_blk);
+  const uint32_t arr = M ? bid % M : 0u;
+  const uint32_t t = M ? bid / M : bid;
+  const uint32_t e = a.tm.tile_ev[t];
+  const uint32_t lt = t - a.tm.tb[e];
+  const uint32_t o = a.tm.off[e];
+  const uint32_t n = a.tm.off[e + 1] - o;
+  const Shape sh = shape_of(n);
+  const bool split = a.L < sh.D;
+  const bool root_only = (n == 1 && a.L == 0);
+  if (!split) {
+    if (M && threadIdx.x == 0) publish(&a.desc[bid], kStPrefix | seg_pack(SegCount{1u, 0u, 0u}));
+    if (root_only && arr == 0 && threadIdx.x == 0) write_node(a, o, 0u, a.Ad[o]);
+    return;
+  }
+  const uint32_t first = lt * (uint32_t)kLevelTile + threadIdx.x * kLevelItems;
+  const uint32_t lvl0 = (1u << a.L) - 1u;
+  const bool last = (a.L + 1 == sh.D);
+
+  uint32_t cls[kLevelItems], pt[kLevelItems], dest0[kLevelItems];
+  SegCount agg{0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < kLevelItems; ++i) {
+    const uint32_t jl = first + i;
+    cls[i] = kNone;
+    pt[i] = 0;
+    dest0[i] = 0;
+    if (jl < n) {
+      const uint32_t ts = subtree_at(sh, a.L, jl);
+      uint32_t start, size;
+      subtree_range(sh, a.L, ts, start, size);
+      if (jl - start < size) {
+        const uint32_t mpos = start + left_size(size);
+        const uint32_t pm = a.Ad[o + mpos];
+        if (arr == 0) {
+          if (jl == mpos) {
+            write_node(a, o, lvl0 + ts, pm);
+          } else if (last) {
+            write_node(a, o, 2u * (lvl0 + ts) + 1u + (jl > mpos ? 1u : 0u), a.Ad[o + jl]);
+          }
+        }
+        if (M) {
+          const uint32_t rm = a.Rd[pm];
+          const uint32_t p = a.src[arr][o + jl];
+          const uint32_t r = a.Rd[p];
+          pt[i] = p;
+          if (jl == start) agg = SegCount{1u, 0u, 0u};
+          if (r < rm) {
+            cls[i] = kLeft;
+            dest0[i] = o + start;
+            agg.l++;
+          } else if (r > rm) {
+            cls[i] = kRight;
+            dest0[i] = o + mpos + 1u;
+            agg.r++;
+          } else {
+            cls[i] = kMed;
+            dest0[i] = o + mpos;
+          }
+          if (jl == start) cls[i] |= 4u;  // interval starts here
+        }
+      }
+    }
+  }
+  if (!M) return;
+
+  // Block-wide segmented exclusive scan of the per-thread aggregates.
+  SegCount incl = agg;
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int ofs = 1; ofs < kWave; ofs <<= 1) {
+    SegCount u;
+    u.f = __shfl_up(incl.f, ofs);
+    u.l = __shfl_up(incl.l, ofs);
+    u.r = __shfl_up(incl.r, ofs);
+    if ((int)lane >= ofs) incl = seg_op(u, incl);
+  }
+  SegCount excl;
+  excl.f = __shfl_up(incl.f, 1);
+  excl.l = __shfl_up(incl.l, 1);
+  excl.r = __shfl_up(incl.r, 1);
+  if (lane == 0) excl = SegCount{0u, 0u, 0u};
+  const int wave = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_wave[wave] = incl;
+  __syncthreads();
+  SegCount wcarry{0u, 0u, 0u}, tile_agg{0u, 0u, 0u};
+#pragma unroll
+  for (int w = 0; w < kThreads / kWave; ++w) {
+    if (w < wave) wcarry = seg_op(wcarry, s_wave[w]);
+    tile_agg = seg_op(tile_agg, s_wave[w]);
+  }
+  excl = seg_op(wcarry, excl);
+
+  if (threadIdx.x == 0) {
+    SegCount carry{0u, 0u, 0u};
+    if (lt == 0) {
+      publish(&a.desc[bid], kStPrefix | seg_pack(tile_agg));
+    } else {
+      publish(&a.desc[bid], kStAggregate | seg_pack(tile_agg));
+      for (uint32_t j = bid - M;; j -= M) {
+        const uint64_t w = poll(&a.desc[j]);
+        carry = seg_op(seg_unpack(w), carry);
+        if ((w & kStMask) == kStPrefix || carry.f) break;
+      }
+      publish(&a.desc[bid], kStPrefix | seg_pack(seg_op(carry, tile_agg)));
+    }
+    s_carry = carry;
+  }
+  __syncthreads();
+  const SegCount run0 = seg_op(s_carry, excl);
+  uint32_t nl = run0.l, nr = run0.r;
+  uint32_t* D = a.dst[arr];
+#pragma unroll
+  for (int i = 0; i < kLevelItems; ++i) {
+    const uint32_t c = cls[i];
+    if (c & 4u) {
+      nl = 0;
+      nr = 0;
+    }
+    switch (c & 3u) {
+      case kLeft: D[dest0[i] + nl] = pt[i]; ++nl; break;
+      case kRight: D[dest0[i] + nr] = pt[i]; ++nr; break;
+      case kMed: D[dest0[i]] = pt[i]; break;
+      default: break;
+    }
+  }
+}
+
+}  // namespace fkd

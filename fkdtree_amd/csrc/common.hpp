@@ -1,0 +1,120 @@
+// Device helpers shared by the build and search kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fkd {
+
+constexpr int kThreads = 256;  // every kernel runs 256-thread (4-wave) blocks
+constexpr int kWave = 64;
+
+__host__ __device__ inline uint32_t floor_log2(uint32_t x) { return 31u - (uint32_t)__builtin_clz(x | 1u); }
+
+// Number of nodes in the left subtree of a left-complete binary tree with
+// `len` nodes: the rank of the median the reference selects for an interval
+// of that length (partition_complete_kdtree, FKDTree_cpu.h:287-296).
+__host__ __device__ inline uint32_t left_size(uint32_t len) {
+  if (len <= 1) return 0;
+  const uint32_t p = 1u << floor_log2(len);
+  return (p / 2 - 1 <= len - p) ? p - 1 : len - p / 2;
+}
+
+// Shape of the heap-ordered left-complete tree of n nodes: deepest depth D and
+// the number R of nodes on depth D.
+struct Shape {
+  uint32_t n, D, R;
+};
+__host__ __device__ inline Shape shape_of(uint32_t n) {
+  Shape s;
+  s.n = n;
+  s.D = floor_log2(n);
+  s.R = n - ((1u << s.D) - 1u);
+  return s;
+}
+
+// The build keeps every point array in in-order (symmetric) order of the final
+// tree: the interval of the reference's node t on depth L (its theIntervalMin /
+// theIntervalLength, FKDTree_cpu.h:224-227, 257-267) is the in-order range of
+// that subtree.  Closed forms, with h = D - L and H = 2^h:
+//   start(t) = t*H + min(R, t*H)
+//   size(t)  = H - 1 + clamp(R - t*H, 0, H)
+// and consecutive subtrees are separated by one position holding an ancestor.
+__host__ __device__ inline void subtree_range(const Shape& s, uint32_t L, uint32_t t,
+                                              uint32_t& start, uint32_t& size) {
+  const uint32_t h = s.D - L;
+  const uint32_t tH = t << h;
+  const uint32_t H = 1u << h;
+  start = tH + (s.R < tH ? s.R : tH);
+  uint32_t m = s.R > tH ? s.R - tH : 0u;
+  m = m < H ? m : H;
+  size = H - 1u + m;
+}
+
+// Index t (0 <= t < 2^L) of the depth-L subtree whose in-order range holds
+// position j, or of the subtree just before the separator j.
+__host__ __device__ inline uint32_t subtree_at(const Shape& s, uint32_t L, uint32_t j) {
+  const uint32_t h = s.D - L;
+  const uint32_t B = (((s.R >> h) + 1u) << h) + s.R;
+  return j < B ? (j >> (h + 1u)) : ((j - s.R) >> h);
+}
+
+// Total order of float keys under IEEE '<' with -0.0 == +0.0 (the reference's
+// comparator, FKDTree_cpu.h:246-252, compares coordinates with '<' and '==').
+// NaN is rejected before keys are formed.
+__device__ inline uint32_t fkey(float f) {
+  uint32_t u = __float_as_uint(f);
+  u = (u << 1) == 0u ? 0u : u;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+template <class T>
+__device__ inline T wave_incl_sum(T v) {
+  const int lane = (int)lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    T t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Publication of decoupled look-back descriptors: status lives in the top two
+// bits of a 64-bit word written with one release store.
+constexpr uint64_t kStInvalid = 0, kStAggregate = 1ull << 62, kStPrefix = 2ull << 62;
+constexpr uint64_t kStMask = 3ull << 62;
+
+__device__ inline void publish(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t poll(uint64_t* p) {
+  uint64_t v;
+  while (((v = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) & kStMask) ==
+         kStInvalid)
+    __builtin_amdgcn_s_sleep(1);
+  return v;
+}
+
+// Dynamic block index (in launch order of acquisition), broadcast to the block.
+__device__ inline uint32_t dynamic_block(uint32_t* ctr, uint32_t* lds) {
+  if (threadIdx.x == 0) *lds = atomicAdd(ctr, 1u);
+  __syncthreads();
+  const uint32_t b = *lds;
+  __syncthreads();
+  return b;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace fkd

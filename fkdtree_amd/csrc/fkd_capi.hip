@@ -1,0 +1,685 @@
+// Host runtime of the MI355X FKDTree library: tree objects, device memory,
+// launch orchestration and the C ABI declared in include/fkdtree/fkd_capi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "build.hpp"
+#include "common.hpp"
+#include "fkdtree/fkd_capi.h"
+#include "radix.hpp"
+#include "scan.hpp"
+#include "search.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define FKD_HIP(x)                                                                         \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(FKD_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define FKD_TRY(x)         \
+  do {                     \
+    int r_ = (x);          \
+    if (r_ != FKD_OK) return r_; \
+  } while (0)
+
+constexpr uint32_t kMaxPoints = 1u << 30;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int ensure(size_t want) {
+    if (want <= bytes) return FKD_OK;
+    if (p) {
+      FKD_HIP(hipFree(p));
+      p = nullptr;
+      bytes = 0;
+    }
+    want = std::max<size_t>(want, 256);
+    FKD_HIP(hipMalloc(&p, want));
+    bytes = want;
+    return FKD_OK;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+int launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(FKD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return FKD_OK;
+}
+
+int fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (n == 0) return FKD_OK;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+  fkd::fill_u32_kernel<<<blocks, 256, 0, s>>>(p, v, n);
+  return launch_check("fill_u32");
+}
+
+int fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
+  if (n == 0) return FKD_OK;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+  fkd::fill_u64_kernel<<<blocks, 256, 0, s>>>(p, v, n);
+  return launch_check("fill_u64");
+}
+
+uint32_t grid_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 8192)); }
+
+// Event-aligned tiling: event e owns tiles [tb[e], tb[e+1]).
+struct Tiling {
+  uint32_t T = 0;
+  DevBuf tile_ev, tb;
+  int init(const std::vector<uint64_t>& off, uint32_t tile, hipStream_t s) {
+    const size_t E = off.size() - 1;
+    std::vector<uint32_t> htb(E + 1), hev;
+    uint32_t t = 0;
+    for (size_t e = 0; e < E; ++e) {
+      htb[e] = t;
+      const uint64_t n = off[e + 1] - off[e];
+      const uint32_t k = (uint32_t)((n + tile - 1) / tile);
+      for (uint32_t i = 0; i < k; ++i) hev.push_back((uint32_t)e);
+      t += k;
+    }
+    htb[E] = t;
+    T = t;
+    FKD_TRY(tb.ensure(htb.size() * 4));
+    FKD_TRY(tile_ev.ensure(std::max<size_t>(hev.size(), 1) * 4));
+    FKD_HIP(hipMemcpyAsync(tb.p, htb.data(), htb.size() * 4, hipMemcpyHostToDevice, s));
+    if (!hev.empty())
+      FKD_HIP(hipMemcpyAsync(tile_ev.p, hev.data(), hev.size() * 4, hipMemcpyHostToDevice, s));
+    FKD_HIP(hipStreamSynchronize(s));
+    return FKD_OK;
+  }
+  fkd::TileMap map(const uint32_t* off) const {
+    fkd::TileMap m;
+    m.tile_ev = tile_ev.as<uint32_t>();
+    m.tb = tb.as<uint32_t>();
+    m.off = off;
+    m.T = T;
+    return m;
+  }
+};
+
+}  // namespace
+
+struct fkd_tree_s {
+  int dim = 3;
+  uint32_t n_events = 0;
+  uint32_t P = 0;
+  uint32_t Dmax = 0;
+  bool ids_in_order = true;
+  bool built = false;
+  std::vector<uint64_t> h_off;
+  DevBuf off;                  // uint32 event offsets (n_events + 1)
+  Tiling radix_tiles, level_tiles;
+  DevBuf X, ids;               // input copy
+  DevBuf Tc, Tid, nodes;       // outputs
+  DevBuf A, K, R;              // build scratch
+  DevBuf counts, offsets, desc, ctr, err;
+  DevBuf q_counts, q_off, q_hits, q_boxes, q_ev;  // search scratch
+  uint64_t last_nq = 0, last_total = 0;
+};
+
+namespace {
+
+template <int DIM>
+int run_radix_pass(fkd_tree_s* t, uint32_t M, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
+                   uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift, bool write_keys,
+                   bool write_ranks, hipStream_t s) {
+  const fkd::TileMap tm = t->radix_tiles.map(t->off.as<uint32_t>());
+  const uint64_t nc = (uint64_t)M * 256u * tm.T;
+  dim3 grid(tm.T, M);
+  fkd::radix_count_kernel<<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, t->counts.as<uint32_t>());
+  FKD_TRY(launch_check("radix_count"));
+  const uint64_t st = fkd::scan_tiles(nc);
+  FKD_TRY(fill_u64(t->desc.as<uint64_t>(), 0, st, s));
+  FKD_TRY(fill_u32(t->ctr.as<uint32_t>(), 0, 1, s));
+  fkd::scan_exclusive_kernel<uint32_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
+      t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), nc, t->desc.as<uint64_t>(), t->ctr.as<uint32_t>());
+  FKD_TRY(launch_check("radix_scan"));
+  if (write_keys && !write_ranks)
+    fkd::radix_scatter_kernel<true, false><<<grid, fkd::kThreads, 0, s>>>(
+        kin, kout, vin, vout, stride, ranks, t->P, shift, tm, t->offsets.as<uint32_t>());
+  else if (!write_keys && write_ranks)
+    fkd::radix_scatter_kernel<false, true><<<grid, fkd::kThreads, 0, s>>>(
+        kin, kout, vin, vout, stride, ranks, t->P, shift, tm, t->offsets.as<uint32_t>());
+  else if (write_keys && write_ranks)
+    fkd::radix_scatter_kernel<true, true><<<grid, fkd::kThreads, 0, s>>>(
+        kin, kout, vin, vout, stride, ranks, t->P, shift, tm, t->offsets.as<uint32_t>());
+  else
+    fkd::radix_scatter_kernel<false, false><<<grid, fkd::kThreads, 0, s>>>(
+        kin, kout, vin, vout, stride, ranks, t->P, shift, tm, t->offsets.as<uint32_t>());
+  return launch_check("radix_scatter");
+}
+
+template <int DIM>
+int build_impl(fkd_tree_s* t, hipStream_t s) {
+  const uint32_t P = t->P;
+  if (P == 0) {
+    t->built = true;
+    return FKD_OK;
+  }
+  const uint64_t S = P;  // stride of one array
+  uint32_t* A = t->A.as<uint32_t>();  // A[m][2][P]
+  uint32_t* K = t->K.as<uint32_t>();  // K[m][2][P]
+  uint32_t* R = t->R.as<uint32_t>();  // R[m][P]
+  auto Abuf = [&](int m, int c) { return A + ((uint64_t)m * 2 + c) * S; };
+  auto Kbuf = [&](int m, int c) { return K + ((uint64_t)m * 2 + c) * S; };
+  const float* X = t->X.as<float>();
+  const uint32_t grid = grid_for(P);
+
+  // 1. Tie order: if ids are not increasing inside every event, sort point
+  //    indices by (event, id) first.
+  const uint32_t* order = nullptr;
+  if (!t->ids_in_order) {
+    fkd::id_keys_kernel<<<grid, 256, 0, s>>>(t->ids.as<uint32_t>(), P, Kbuf(0, 0), Abuf(0, 0));
+    FKD_TRY(launch_check("id_keys"));
+    int c = 0;
+    for (int pass = 0; pass < 4; ++pass) {
+      const bool lastp = pass == 3;
+      FKD_TRY(run_radix_pass<DIM>(t, 1, Kbuf(0, c), Kbuf(0, 1 - c), Abuf(0, c), lastp ? R : Abuf(0, 1 - c),
+                                  S, nullptr, 8u * pass, !lastp, false, s));
+      c = 1 - c;
+    }
+    order = R;  // R[0] holds the (event, id) order until the key pass rewrites it
+  }
+  // 2. Keys of every coordinate, in tie order.
+  fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>());
+  FKD_TRY(launch_check("keys"));
+  // 3. Stable LSD sort of every coordinate inside every event; the last pass
+  //    also stores each point's rank.
+  {
+    int c = 0;
+    for (int pass = 0; pass < 4; ++pass) {
+      const bool lastp = pass == 3;
+      FKD_TRY(run_radix_pass<DIM>(t, DIM, Kbuf(0, c), Kbuf(0, 1 - c), Abuf(0, c), Abuf(0, 1 - c), 2 * S,
+                                  lastp ? R : nullptr, 8u * pass, !lastp, lastp, s));
+      c = 1 - c;
+    }
+  }
+  // The rank scatter writes R[m][p] for m < DIM with stride P: radix passes
+  // use stride 2*S for keys/values, so ranks were addressed m * P.  Values are
+  // now in buffer 0 of every array.
+  // 4. One partition launch per depth.
+  int cur[4] = {0, 0, 0, 0};
+  const uint32_t levels = std::max<uint32_t>(1, t->Dmax);
+  const fkd::TileMap tm = t->level_tiles.map(t->off.as<uint32_t>());
+  constexpr uint32_t M = DIM - 1;
+  for (uint32_t L = 0; L < levels; ++L) {
+    fkd::LevelArgs<DIM> a;
+    a.L = L;
+    a.d = L % DIM;
+    a.Ad = Abuf(a.d, cur[a.d]);
+    a.Rd = R + (uint64_t)a.d * S;
+    int k = 0;
+    int moved[3] = {0, 0, 0};
+    for (int m = 0; m < DIM; ++m) {
+      if ((uint32_t)m == a.d) continue;
+      a.src[k] = Abuf(m, cur[m]);
+      a.dst[k] = Abuf(m, 1 - cur[m]);
+      moved[k] = m;
+      ++k;
+    }
+    for (; k < 3; ++k) {
+      a.src[k] = nullptr;
+      a.dst[k] = nullptr;
+    }
+    a.tm = tm;
+    a.X = X;
+    a.ids = t->ids.as<uint32_t>();
+    a.P = P;
+    a.Tc = t->Tc.as<float>();
+    a.Tid = t->Tid.as<uint32_t>();
+    a.nodes = t->nodes.as<float>();
+    a.desc = t->desc.as<uint64_t>();
+    a.ctr = t->ctr.as<uint32_t>();
+    const uint32_t blocks = (M ? M : 1u) * tm.T;
+    if (M) FKD_TRY(fill_u64(a.desc, 0, blocks, s));
+    FKD_TRY(fill_u32(a.ctr, 0, 1, s));
+    fkd::level_kernel<DIM><<<blocks, fkd::kThreads, 0, s>>>(a);
+    FKD_TRY(launch_check("level"));
+    for (uint32_t i = 0; i < M; ++i) cur[moved[i]] ^= 1;
+  }
+  t->built = true;
+  return FKD_OK;
+}
+
+template <int DIM>
+int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                      uint64_t* qoff, hipStream_t s) {
+  FKD_TRY(t->q_counts.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
+  if (nq > 0) {
+    fkd::SearchArgs a{};
+    a.nodes = t->nodes.as<float>();
+    a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
+    a.n_single = t->P;
+    a.qev = qev;
+    a.nq = nq;
+    a.mins = mins;
+    a.maxs = maxs;
+    a.counts = t->q_counts.as<uint32_t>();
+    fkd::search_kernel<DIM, false><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
+    FKD_TRY(launch_check("search_count"));
+  }
+  const uint64_t st = fkd::scan_tiles(nq);
+  FKD_TRY(t->desc.ensure(st * 8));
+  FKD_TRY(fill_u64(t->desc.as<uint64_t>(), 0, st, s));
+  FKD_TRY(fill_u32(t->ctr.as<uint32_t>(), 0, 1, s));
+  fkd::scan_exclusive_kernel<uint64_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
+      t->q_counts.as<uint32_t>(), qoff, nq, t->desc.as<uint64_t>(), t->ctr.as<uint32_t>());
+  return launch_check("search_scan");
+}
+
+template <int DIM>
+int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                     const uint64_t* qoff, uint32_t* out, uint64_t cap, hipStream_t s) {
+  if (nq == 0) return FKD_OK;
+  fkd::SearchArgs a{};
+  a.nodes = t->nodes.as<float>();
+  a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
+  a.n_single = t->P;
+  a.qev = qev;
+  a.nq = nq;
+  a.mins = mins;
+  a.maxs = maxs;
+  a.qoff = qoff;
+  a.out = out;
+  a.cap = cap;
+  fkd::search_kernel<DIM, true><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
+  return launch_check("search_fill");
+}
+
+__global__ void check_input_kernel(const float* X, const uint32_t* ids, const uint32_t* off,
+                                   uint32_t n_events, uint32_t P, int dim, uint32_t* err) {
+  bool nan = false, unordered = false;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+    for (int m = 0; m < dim; ++m) {
+      const float x = X[(uint64_t)m * P + i];
+      nan |= (x != x);
+    }
+    if (i + 1 < P && ids[i] >= ids[i + 1]) {
+      // Only a problem when i and i+1 are in the same event.
+      uint32_t lo = 0, hi = n_events;  // find e with off[e] <= i < off[e+1]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (off[mid] <= i) lo = mid; else hi = mid;
+      }
+      if (i + 1 < off[lo + 1]) unordered = true;
+    }
+  }
+  if (nan) atomicOr(err, 1u);
+  if (unordered) atomicOr(err, 2u);
+}
+
+int validate_offsets(uint32_t n_events, const uint64_t* event_offsets, uint64_t* P) {
+  if (n_events == 0) return fail(FKD_ERR_INVALID, "n_events must be >= 1");
+  if (!event_offsets) return fail(FKD_ERR_INVALID, "event_offsets is NULL");
+  if (event_offsets[0] != 0) return fail(FKD_ERR_INVALID, "event_offsets[0] must be 0");
+  for (uint32_t e = 0; e < n_events; ++e) {
+    if (event_offsets[e + 1] < event_offsets[e]) return fail(FKD_ERR_INVALID, "event_offsets not monotone");
+    if (event_offsets[e + 1] - event_offsets[e] > kMaxPoints)
+      return fail(FKD_ERR_INVALID, "more than 2^30 points in one event");
+  }
+  *P = event_offsets[n_events];
+  if (*P > kMaxPoints) return fail(FKD_ERR_INVALID, "more than 2^30 points in one handle");
+  return FKD_OK;
+}
+
+// Allocates everything, given the input already in t->X / t->ids; validates.
+int finish_create(fkd_tree_s* t, hipStream_t s) {
+  const uint32_t P = t->P, E = t->n_events;
+  const int dim = t->dim;
+  std::vector<uint32_t> off32(E + 1);
+  for (uint32_t e = 0; e <= E; ++e) off32[e] = (uint32_t)t->h_off[e];
+  t->Dmax = 0;
+  for (uint32_t e = 0; e < E; ++e) {
+    const uint64_t n = t->h_off[e + 1] - t->h_off[e];
+    if (n) t->Dmax = std::max(t->Dmax, fkd::floor_log2((uint32_t)n));
+  }
+  FKD_TRY(t->off.ensure((E + 1) * 4));
+  FKD_HIP(hipMemcpyAsync(t->off.p, off32.data(), (E + 1) * 4, hipMemcpyHostToDevice, s));
+  FKD_TRY(t->radix_tiles.init(t->h_off, fkd::kRadixTile, s));
+  FKD_TRY(t->level_tiles.init(t->h_off, fkd::kLevelTile, s));
+  const size_t S = std::max<uint32_t>(P, 1);
+  const int rf = dim <= 3 ? 4 : 8;
+  FKD_TRY(t->Tc.ensure(S * dim * 4));
+  FKD_TRY(t->Tid.ensure(S * 4));
+  FKD_TRY(t->nodes.ensure(S * rf * 4));
+  FKD_TRY(t->A.ensure(S * dim * 2 * 4));
+  FKD_TRY(t->K.ensure(S * dim * 2 * 4));
+  FKD_TRY(t->R.ensure(S * dim * 4));
+  const uint64_t nc = (uint64_t)dim * 256u * std::max<uint32_t>(t->radix_tiles.T, 1);
+  FKD_TRY(t->counts.ensure(nc * 4));
+  FKD_TRY(t->offsets.ensure((nc + 1) * 4));
+  const uint64_t ndesc = std::max<uint64_t>(fkd::scan_tiles(nc), (uint64_t)dim * t->level_tiles.T + 1);
+  FKD_TRY(t->desc.ensure(ndesc * 8));
+  FKD_TRY(t->ctr.ensure(64));
+  FKD_TRY(t->err.ensure(64));
+  FKD_TRY(fill_u32(t->err.as<uint32_t>(), 0, 1, s));
+  if (P) {
+    check_input_kernel<<<grid_for(P), 256, 0, s>>>(t->X.as<float>(), t->ids.as<uint32_t>(), t->off.as<uint32_t>(),
+                                                    E, P, dim, t->err.as<uint32_t>());
+    FKD_TRY(launch_check("check_input"));
+  }
+  uint32_t err = 0;
+  FKD_HIP(hipMemcpyAsync(&err, t->err.p, 4, hipMemcpyDeviceToHost, s));
+  FKD_HIP(hipStreamSynchronize(s));
+  if (err & 1u) return fail(FKD_ERR_NAN, "NaN coordinate in the input points");
+  t->ids_in_order = (err & 2u) == 0;
+  return FKD_OK;
+}
+
+template <int DIM>
+int aos_upload(fkd_tree_s* t, const void* host_points, hipStream_t s) {
+  const uint32_t P = t->P;
+  if (P == 0) return FKD_OK;
+  DevBuf staging;
+  FKD_TRY(staging.ensure((size_t)P * (DIM + 1) * 4));
+  FKD_HIP(hipMemcpyAsync(staging.p, host_points, (size_t)P * (DIM + 1) * 4, hipMemcpyHostToDevice, s));
+  fkd::aos_to_soa_kernel<DIM><<<grid_for(P), 256, 0, s>>>(staging.as<uint32_t>(), t->X.as<float>(),
+                                                           t->ids.as<uint32_t>(), P);
+  FKD_TRY(launch_check("aos_to_soa"));
+  FKD_HIP(hipStreamSynchronize(s));
+  return FKD_OK;
+}
+
+template <class F>
+int dispatch_dim(int dim, F&& f) {
+  switch (dim) {
+    case 1: return f(std::integral_constant<int, 1>());
+    case 2: return f(std::integral_constant<int, 2>());
+    case 3: return f(std::integral_constant<int, 3>());
+    case 4: return f(std::integral_constant<int, 4>());
+    default: return fail(FKD_ERR_INVALID, "dim must be 1..4");
+  }
+}
+
+int create_common(int dim, uint32_t n_events, const uint64_t* event_offsets, fkd_tree_s** out) {
+  if (!out) return fail(FKD_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (dim < 1 || dim > 4) return fail(FKD_ERR_INVALID, "dim must be 1..4");
+  uint64_t P = 0;
+  FKD_TRY(validate_offsets(n_events, event_offsets, &P));
+  fkd_tree_s* t = new fkd_tree_s();
+  t->dim = dim;
+  t->n_events = n_events;
+  t->P = (uint32_t)P;
+  t->h_off.assign(event_offsets, event_offsets + n_events + 1);
+  const size_t S = std::max<uint64_t>(P, 1);
+  int r = t->X.ensure(S * dim * 4);
+  if (r == FKD_OK) r = t->ids.ensure(S * 4);
+  if (r != FKD_OK) {
+    delete t;
+    return r;
+  }
+  *out = t;
+  return FKD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fkd_last_error(void) { return g_err.c_str(); }
+
+int fkd_create_events(int dim, uint32_t n_events, const uint64_t* event_offsets, const void* host_points,
+                      fkd_stream_t stream, fkd_tree_t* out) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  fkd_tree_s* t = nullptr;
+  FKD_TRY(create_common(dim, n_events, event_offsets, &t));
+  if (t->P && !host_points) {
+    delete t;
+    return fail(FKD_ERR_INVALID, "host_points is NULL");
+  }
+  int r = dispatch_dim(dim, [&](auto D) { return aos_upload<decltype(D)::value>(t, host_points, s); });
+  if (r == FKD_OK) r = finish_create(t, s);
+  if (r != FKD_OK) {
+    delete t;
+    return r;
+  }
+  *out = t;
+  return FKD_OK;
+}
+
+int fkd_create(int dim, uint32_t n, const void* host_points, fkd_stream_t stream, fkd_tree_t* out) {
+  const uint64_t off[2] = {0, n};
+  return fkd_create_events(dim, 1, off, host_points, stream, out);
+}
+
+int fkd_create_device(int dim, uint32_t n_events, const uint64_t* event_offsets, const float* dev_coords,
+                      const uint32_t* dev_ids, fkd_stream_t stream, fkd_tree_t* out) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  fkd_tree_s* t = nullptr;
+  FKD_TRY(create_common(dim, n_events, event_offsets, &t));
+  int r = FKD_OK;
+  if (t->P) {
+    if (!dev_coords || !dev_ids) {
+      delete t;
+      return fail(FKD_ERR_INVALID, "dev_coords / dev_ids is NULL");
+    }
+    hipError_t e = hipMemcpyAsync(t->X.p, dev_coords, (size_t)t->P * dim * 4, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(t->ids.p, dev_ids, (size_t)t->P * 4, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) r = fail(FKD_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+  }
+  if (r == FKD_OK) r = finish_create(t, s);
+  if (r != FKD_OK) {
+    delete t;
+    return r;
+  }
+  *out = t;
+  return FKD_OK;
+}
+
+void fkd_destroy(fkd_tree_t tree) { delete tree; }
+
+int fkd_build(fkd_tree_t t, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dispatch_dim(t->dim, [&](auto D) { return build_impl<decltype(D)::value>(t, s); });
+}
+
+int fkd_get_tree(fkd_tree_t t, float* coords_out, uint32_t* ids_out, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (t->P) {
+    if (coords_out)
+      FKD_HIP(hipMemcpyAsync(coords_out, t->Tc.p, (size_t)t->P * t->dim * 4, hipMemcpyDeviceToHost, s));
+    if (ids_out) FKD_HIP(hipMemcpyAsync(ids_out, t->Tid.p, (size_t)t->P * 4, hipMemcpyDeviceToHost, s));
+  }
+  FKD_HIP(hipStreamSynchronize(s));
+  return FKD_OK;
+}
+
+int fkd_tree_device(fkd_tree_t t, const float** dev_coords, const uint32_t** dev_ids) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (dev_coords) *dev_coords = t->Tc.as<float>();
+  if (dev_ids) *dev_ids = t->Tid.as<uint32_t>();
+  return FKD_OK;
+}
+
+int fkd_tree_info(fkd_tree_t t, int* dim, uint32_t* n_events, uint64_t* n_points) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (dim) *dim = t->dim;
+  if (n_events) *n_events = t->n_events;
+  if (n_points) *n_points = t->P;
+  return FKD_OK;
+}
+
+int fkd_search_count(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                     uint64_t* dev_offsets, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  if (!dev_offsets || (nq && (!mins || !maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
+  if (!qev && t->n_events > 1) return fail(FKD_ERR_INVALID, "query events required for a batch of trees");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dispatch_dim(t->dim, [&](auto D) {
+    return search_count_impl<decltype(D)::value>(t, nq, mins, maxs, qev, dev_offsets, s);
+  });
+}
+
+int fkd_search_fill(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                    const uint64_t* dev_offsets, uint32_t* out, uint64_t capacity, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  if (!dev_offsets || (nq && (!mins || !maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
+  if (!qev && t->n_events > 1) return fail(FKD_ERR_INVALID, "query events required for a batch of trees");
+  if (capacity && !out) return fail(FKD_ERR_INVALID, "out is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dispatch_dim(t->dim, [&](auto D) {
+    return search_fill_impl<decltype(D)::value>(t, nq, mins, maxs, qev, dev_offsets, out, capacity, s);
+  });
+}
+
+int fkd_search_host(fkd_tree_t t, uint32_t nq, const float* host_mins, const float* host_maxs,
+                    const uint32_t* host_qev, uint64_t* host_offsets, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!host_offsets || (nq && (!host_mins || !host_maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bb = (size_t)nq * t->dim * 4;
+  const size_t bb_al = (bb + 255) & ~size_t(255);
+  FKD_TRY(t->q_boxes.ensure(bb_al + bb + 256));
+  float* dmins = t->q_boxes.as<float>();
+  float* dmaxs = reinterpret_cast<float*>(reinterpret_cast<char*>(t->q_boxes.p) + bb_al);
+  const uint32_t* dqev = nullptr;
+  if (nq) {
+    FKD_HIP(hipMemcpyAsync(dmins, host_mins, bb, hipMemcpyHostToDevice, s));
+    FKD_HIP(hipMemcpyAsync(dmaxs, host_maxs, bb, hipMemcpyHostToDevice, s));
+    if (host_qev) {
+      FKD_TRY(t->q_ev.ensure((size_t)nq * 4));
+      FKD_HIP(hipMemcpyAsync(t->q_ev.p, host_qev, (size_t)nq * 4, hipMemcpyHostToDevice, s));
+      dqev = t->q_ev.as<uint32_t>();
+    }
+  }
+  FKD_TRY(t->q_off.ensure(((size_t)nq + 1) * 8));
+  FKD_TRY(fkd_search_count(t, nq, dmins, dmaxs, dqev, t->q_off.as<uint64_t>(), stream));
+  FKD_HIP(hipMemcpyAsync(host_offsets, t->q_off.p, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));
+  FKD_HIP(hipStreamSynchronize(s));
+  const uint64_t total = host_offsets[nq];
+  FKD_TRY(t->q_hits.ensure(std::max<uint64_t>(total, 1) * 4));
+  FKD_TRY(fkd_search_fill(t, nq, dmins, dmaxs, dqev, t->q_off.as<uint64_t>(), t->q_hits.as<uint32_t>(), total,
+                          stream));
+  FKD_HIP(hipStreamSynchronize(s));
+  t->last_nq = nq;
+  t->last_total = total;
+  return FKD_OK;
+}
+
+int fkd_fetch_results(fkd_tree_t t, uint32_t* host_ids, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (t->last_total) {
+    if (!host_ids) return fail(FKD_ERR_INVALID, "host_ids is NULL");
+    FKD_HIP(hipMemcpyAsync(host_ids, t->q_hits.p, t->last_total * 4, hipMemcpyDeviceToHost, s));
+  }
+  FKD_HIP(hipStreamSynchronize(s));
+  return FKD_OK;
+}
+
+int fkd_device_info(char* name, size_t name_len, int* compute_units, size_t* free_bytes, size_t* total_bytes) {
+  int dev = 0;
+  FKD_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  FKD_HIP(hipGetDeviceProperties(&prop, dev));
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (compute_units) *compute_units = prop.multiProcessorCount;
+  size_t fr = 0, tot = 0;
+  FKD_HIP(hipMemGetInfo(&fr, &tot));
+  if (free_bytes) *free_bytes = fr;
+  if (total_bytes) *total_bytes = tot;
+  return FKD_OK;
+}
+
+int fkd_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(FKD_ERR_INVALID, "ptr is NULL");
+  FKD_HIP(hipMalloc(ptr, std::max<size_t>(bytes, 256)));
+  return FKD_OK;
+}
+int fkd_free(void* ptr) {
+  if (ptr) FKD_HIP(hipFree(ptr));
+  return FKD_OK;
+}
+int fkd_memcpy_htod(void* dst, const void* src, size_t bytes, fkd_stream_t stream) {
+  if (bytes) FKD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
+  return FKD_OK;
+}
+int fkd_memcpy_dtoh(void* dst, const void* src, size_t bytes, fkd_stream_t stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (bytes) FKD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+  FKD_HIP(hipStreamSynchronize(s));
+  return FKD_OK;
+}
+int fkd_stream_create(fkd_stream_t* stream) {
+  hipStream_t s;
+  FKD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return FKD_OK;
+}
+int fkd_stream_destroy(fkd_stream_t stream) {
+  FKD_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return FKD_OK;
+}
+int fkd_stream_sync(fkd_stream_t stream) {
+  FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return FKD_OK;
+}
+int fkd_event_create(fkd_event_t* ev) {
+  hipEvent_t e;
+  FKD_HIP(hipEventCreate(&e));
+  *ev = e;
+  return FKD_OK;
+}
+int fkd_event_destroy(fkd_event_t ev) {
+  FKD_HIP(hipEventDestroy(static_cast<hipEvent_t>(ev)));
+  return FKD_OK;
+}
+int fkd_event_record(fkd_event_t ev, fkd_stream_t stream) {
+  FKD_HIP(hipEventRecord(static_cast<hipEvent_t>(ev), static_cast<hipStream_t>(stream)));
+  return FKD_OK;
+}
+int fkd_event_elapsed_ms(fkd_event_t a, fkd_event_t b, float* ms) {
+  FKD_HIP(hipEventSynchronize(static_cast<hipEvent_t>(b)));
+  FKD_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(a), static_cast<hipEvent_t>(b)));
+  return FKD_OK;
+}
+
+int fkd_shape_subtree(uint32_t n, uint32_t L, uint32_t t, uint32_t* start, uint32_t* size) {
+  if (n == 0) return fail(FKD_ERR_INVALID, "n == 0");
+  const fkd::Shape sh = fkd::shape_of(n);
+  if (L > sh.D || t >= (1u << L)) return fail(FKD_ERR_INVALID, "L / t out of range");
+  fkd::subtree_range(sh, L, t, *start, *size);
+  return FKD_OK;
+}
+
+int fkd_shape_locate(uint32_t n, uint32_t L, uint32_t j, uint32_t* t) {
+  if (n == 0) return fail(FKD_ERR_INVALID, "n == 0");
+  const fkd::Shape sh = fkd::shape_of(n);
+  if (L > sh.D || j >= n) return fail(FKD_ERR_INVALID, "L / j out of range");
+  *t = fkd::subtree_at(sh, L, j);
+  return FKD_OK;
+}
+
+}  // extern "C"

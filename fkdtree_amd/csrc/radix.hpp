@@ -1,0 +1,138 @@
+// Per-event segmented LSD radix sort passes (8-bit digits), several
+// independent key arrays at once (one per blockIdx.y).
+//
+// Tiles never straddle events: event e owns tiles [tb[e], tb[e+1]).  Digit
+// counts are laid out per event as [digit][tile], so one global exclusive
+// scan over all of them yields, for every (event, digit, tile), the first
+// output slot inside that event's own range: the sort never mixes events.
+#pragma once
+#include "common.hpp"
+
+namespace fkd {
+
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kThreads * kRadixItems;
+constexpr int kRadixPerWave = kRadixTile / (kThreads / kWave);
+
+struct TileMap {
+  const uint32_t* tile_ev;  // tile -> event
+  const uint32_t* tb;       // event -> first tile (n_events + 1 entries)
+  const uint32_t* off;      // event -> first point (n_events + 1 entries)
+  uint32_t T;               // number of tiles
+};
+
+struct TileInfo {
+  uint32_t e, lt, nt, base, end;
+};
+
+template <int TILE>
+__device__ inline TileInfo tile_info(const TileMap& tm, uint32_t t) {
+  TileInfo ti;
+  ti.e = tm.tile_ev[t];
+  const uint32_t tb0 = tm.tb[ti.e];
+  ti.lt = t - tb0;
+  ti.nt = tm.tb[ti.e + 1] - tb0;
+  const uint32_t o = tm.off[ti.e], o1 = tm.off[ti.e + 1];
+  ti.base = o + ti.lt * (uint32_t)TILE;
+  ti.end = min(ti.base + (uint32_t)TILE, o1);
+  return ti;
+}
+
+// counts[m][256*tb[e] + digit*nt + lt] = number of keys of tile t in array m
+// whose digit (key >> shift) & 255 equals digit.
+__global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* __restrict__ keys,
+                                                               uint64_t kstride, uint32_t shift,
+                                                               TileMap tm, uint32_t* counts) {
+  __shared__ uint32_t hist[4][256];
+  const uint32_t t = blockIdx.x, m = blockIdx.y;
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) hist[w][threadIdx.x] = 0;
+  __syncthreads();
+  const TileInfo ti = tile_info<kRadixTile>(tm, t);
+  const uint32_t* K = keys + m * kstride;
+  for (uint32_t i = ti.base + threadIdx.x; i < ti.end; i += kThreads)
+    atomicAdd(&hist[wave][(K[i] >> shift) & 255u], 1u);
+  __syncthreads();
+  const uint32_t c = hist[0][threadIdx.x] + hist[1][threadIdx.x] + hist[2][threadIdx.x] +
+                     hist[3][threadIdx.x];
+  counts[(uint64_t)m * 256u * tm.T + 256ull * (t - ti.lt) + (uint64_t)threadIdx.x * ti.nt + ti.lt] = c;
+}
+
+// Stable scatter of one digit pass.  offsets = exclusive scan of the counts of
+// all arrays (array m's entries start at m * P).  Keys are written only when
+// WRITE_KEYS; when WRITE_RANKS, ranks[m][val] = final slot of val.
+template <bool WRITE_KEYS, bool WRITE_RANKS>
+__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
+    const uint32_t* __restrict__ keys_in, uint32_t* __restrict__ keys_out,
+    const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out, uint64_t stride,
+    uint32_t* __restrict__ ranks, uint32_t P, uint32_t shift, TileMap tm,
+    const uint32_t* __restrict__ offsets) {
+  __shared__ uint32_t cnt[4][256];
+  __shared__ uint32_t goff[256];
+  const uint32_t t = blockIdx.x, m = blockIdx.y;
+  const int wave = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int w = 0; w < 4; ++w) cnt[w][threadIdx.x] = 0;
+  const TileInfo ti = tile_info<kRadixTile>(tm, t);
+  goff[threadIdx.x] =
+      offsets[(uint64_t)m * 256u * tm.T + 256ull * (t - ti.lt) + (uint64_t)threadIdx.x * ti.nt + ti.lt] -
+      m * P;
+  __syncthreads();
+  const uint32_t* Ki = keys_in + m * stride;
+  const uint32_t* Vi = vals_in + m * stride;
+  uint32_t key[kRadixItems], val[kRadixItems], rank[kRadixItems];
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  const uint32_t wbase = ti.base + (uint32_t)wave * kRadixPerWave + lane;
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const uint32_t pos = wbase + (uint32_t)r * kWave;
+    const bool valid = pos < ti.end;
+    key[r] = valid ? Ki[pos] : 0u;
+    val[r] = valid ? Vi[pos] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const uint32_t pos = wbase + (uint32_t)r * kWave;
+    const bool valid = pos < ti.end;
+    const uint32_t dig = (key[r] >> shift) & 255u;
+    uint64_t match = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (dig >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      match &= bit ? bal : ~bal;
+    }
+    uint32_t pre = 0;
+    if (valid) pre = cnt[wave][dig];
+    const uint64_t below = match & lt_mask;
+    rank[r] = pre + (uint32_t)__popcll(below);
+    if (valid && below == 0ull) cnt[wave][dig] = pre + (uint32_t)__popcll(match);
+  }
+  __syncthreads();
+  {
+    const uint32_t d = threadIdx.x;
+    const uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d];
+    cnt[0][d] = goff[d];
+    cnt[1][d] = goff[d] + c0;
+    cnt[2][d] = goff[d] + c0 + c1;
+    cnt[3][d] = goff[d] + c0 + c1 + c2;
+  }
+  __syncthreads();
+  uint32_t* Ko = keys_out + m * stride;
+  uint32_t* Vo = vals_out + m * stride;
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const uint32_t pos = wbase + (uint32_t)r * kWave;
+    if (pos < ti.end) {
+      const uint32_t dig = (key[r] >> shift) & 255u;
+      const uint32_t dest = cnt[wave][dig] + rank[r];
+      Vo[dest] = val[r];
+      if (WRITE_KEYS) Ko[dest] = key[r];
+      if (WRITE_RANKS) ranks[m * (uint64_t)P + val[r]] = dest;
+    }
+  }
+}
+
+}  // namespace fkd

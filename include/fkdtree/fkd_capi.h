@@ -1,0 +1,142 @@
+/*
+ * C ABI of the MI355X (gfx950) FKDTree hot paths: the level-by-level k-d tree
+ * build and the closed-box search, for float points of dimension 1..4.
+ *
+ * It replaces the work behind the reference's tree interface
+ * (FKDTree.h:6-52) and its GPU backends (FKDTree_opencl.h:17-290,
+ * searchInTheBoxCuda.cu:174-185).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every call returns FKD_OK (0) or a negative FKD_ERR_* code; the message
+ *    of the last failure on the calling thread is fkd_last_error().
+ *  - `stream` is a hipStream_t (NULL = the default stream).  Calls that only
+ *    enqueue GPU work return before it finishes; calls documented as
+ *    synchronous wait for `stream`.
+ *  - Calls on one tree need external synchronisation; different trees are
+ *    independent.
+ *  - Device coordinates are SoA: coords[d * P + i] is coordinate d of point i
+ *    (P = number of points).  Query boxes are SoA too: mins[d * nq + q].
+ *  - Limits: at most 2^30 points per tree / batch and per event.
+ *  - Preconditions (as for the reference comparator, FKDTree_cpu.h:246-252):
+ *    ids unique within a tree; coordinates not NaN (rejected at creation).
+ */
+#ifndef FKDTREE_FKD_CAPI_H_
+#define FKDTREE_FKD_CAPI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FKD_OK 0
+#define FKD_ERR_INVALID (-1)  /* bad argument */
+#define FKD_ERR_HIP (-2)      /* HIP runtime error */
+#define FKD_ERR_NAN (-3)      /* NaN coordinate */
+#define FKD_ERR_STATE (-4)    /* e.g. search before build */
+#define FKD_ERR_CAPACITY (-5) /* output buffer too small */
+
+typedef struct fkd_tree_s* fkd_tree_t;
+typedef void* fkd_stream_t; /* hipStream_t */
+
+const char* fkd_last_error(void);
+
+/* ---- tree life cycle ------------------------------------------------- */
+
+/* Tree over n FKDPoint<float, dim> records in host memory (dim floats then a
+ * uint32 id, 4*(dim+1) bytes each; FKDPoint.h:13-104).  Copies the points to
+ * the device (the reference constructors copy them too: FKDTree_cpu.h:19-23,
+ * FKDTree_opencl.h:20-56).  Synchronous.
+ * Replaces: FKDTree_CPU / FKDTree_OpenCL constructors (FKDTree.h:9). */
+int fkd_create(int dim, uint32_t n, const void* host_points, fkd_stream_t stream,
+               fkd_tree_t* out);
+
+/* Batch of independent trees ("events"): event e owns records
+ * [event_offsets[e], event_offsets[e+1]) of host_points.  Ids need only be
+ * unique within an event.  Synchronous.  No reference counterpart: the
+ * reference builds one tree per object. */
+int fkd_create_events(int dim, uint32_t n_events, const uint64_t* event_offsets,
+                      const void* host_points, fkd_stream_t stream, fkd_tree_t* out);
+
+/* Same from device-resident SoA coordinates and ids (copied on the device).
+ * event_offsets is a host array; n_events = 1 with {0, P} is a single tree. */
+int fkd_create_device(int dim, uint32_t n_events, const uint64_t* event_offsets,
+                      const float* dev_coords, const uint32_t* dev_ids, fkd_stream_t stream,
+                      fkd_tree_t* out);
+
+void fkd_destroy(fkd_tree_t tree);
+
+/* Build every tree of the handle.  Enqueues only.
+ * Replaces: FKDTree_CPU::build (FKDTree_cpu.h:221-277),
+ *           FKDTree_OpenCL::build (FKDTree_opencl.h:208-246). */
+int fkd_build(fkd_tree_t tree, fkd_stream_t stream);
+
+/* Heap-ordered tree arrays of all events end to end, copied to the host:
+ * coords_out[d * P + i], ids_out[i].  Synchronous.
+ * Replaces: getDimensionVector / getIdVector (FKDTree_cpu.h:209-213,
+ *           FKDTree_opencl.h:251-262). */
+int fkd_get_tree(fkd_tree_t tree, float* coords_out, uint32_t* ids_out, fkd_stream_t stream);
+
+/* Device pointers of the same arrays (valid until fkd_destroy). */
+int fkd_tree_device(fkd_tree_t tree, const float** dev_coords, const uint32_t** dev_ids);
+
+/* Shape queries. */
+int fkd_tree_info(fkd_tree_t tree, int* dim, uint32_t* n_events, uint64_t* n_points);
+
+/* ---- search ---------------------------------------------------------- */
+
+/* Count pass + scan: dev_offsets[q] = first output slot of query q,
+ * dev_offsets[nq] = total hits (uint64).  dev_query_events (may be NULL:
+ * event 0) names the event whose tree query q searches.  Enqueues only.
+ * Replaces, with search_fill: search_in_the_box (FKDTree_cpu.h:123-169) for
+ * nq boxes at once; the counts alone are the result of
+ * FKDTree_OpenCL::search_in_the_box_multiple (FKDTree_opencl.h:123-164). */
+int fkd_search_count(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
+                     const uint32_t* dev_query_events, uint64_t* dev_offsets,
+                     fkd_stream_t stream);
+
+/* Fill pass: ids of the points inside box q go to
+ * dev_ids_out[dev_offsets[q] .. dev_offsets[q+1]) (slots >= capacity are
+ * dropped; check dev_offsets[nq] <= capacity).  Within a query the order is
+ * unspecified.  Enqueues only. */
+int fkd_search_fill(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
+                    const uint32_t* dev_query_events, const uint64_t* dev_offsets,
+                    uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream);
+
+/* Host convenience: boxes in host memory (SoA), host_offsets[nq+1] filled,
+ * result ids kept on the device until fkd_fetch_results.  Synchronous. */
+int fkd_search_host(fkd_tree_t tree, uint32_t nq, const float* host_mins, const float* host_maxs,
+                    const uint32_t* host_query_events, uint64_t* host_offsets,
+                    fkd_stream_t stream);
+int fkd_fetch_results(fkd_tree_t tree, uint32_t* host_ids, fkd_stream_t stream);
+
+/* ---- device plumbing (memory, streams, timing) ------------------------- */
+
+int fkd_device_info(char* name, size_t name_len, int* compute_units, size_t* free_bytes,
+                    size_t* total_bytes);
+int fkd_malloc(void** ptr, size_t bytes);
+int fkd_free(void* ptr);
+int fkd_memcpy_htod(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
+int fkd_memcpy_dtoh(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
+int fkd_stream_create(fkd_stream_t* stream);
+int fkd_stream_destroy(fkd_stream_t stream);
+int fkd_stream_sync(fkd_stream_t stream);
+typedef void* fkd_event_t;
+int fkd_event_create(fkd_event_t* ev);
+int fkd_event_destroy(fkd_event_t ev);
+int fkd_event_record(fkd_event_t ev, fkd_stream_t stream);
+int fkd_event_elapsed_ms(fkd_event_t start, fkd_event_t stop, float* ms);
+
+/* ---- host-side helpers (no GPU needed) --------------------------------- */
+
+/* Closed-form shape of the heap-ordered tree of n nodes, for tests:
+ * start/size of subtree t on depth L, and the subtree holding position j. */
+int fkd_shape_subtree(uint32_t n, uint32_t L, uint32_t t, uint32_t* start, uint32_t* size);
+int fkd_shape_locate(uint32_t n, uint32_t L, uint32_t j, uint32_t* t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FKDTREE_FKD_CAPI_H_ */
