@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the FKDTree hot paths on one MI355X.
+
+Workload (default ``mixed_1m``): N = 1,048,576 points in 3-D, half uniform in
+[0,1)^3 and half in 1,000 Gaussian blobs (sigma 0.005), quantised to a 2^-16
+grid; 1,048,576 closed boxes of half-width 0.01 centred on random points.
+
+A timed step is one box search of all queries against the resident tree:
+count pass + exclusive scan + fill pass, writing every hit id (CSR output).
+``value`` is box queries per second over the timed steps.  Builds are timed
+in their own loop (``builds_per_s``).  Inputs and outputs stay in HBM; timing
+uses HIP events on the stream the kernels run on.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 is refused: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mixed_1m", choices=["mixed_1m", "uniform_100k", "uniform_4m"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def workload(name):
+    from fkdtree_amd import gen
+    if name == "mixed_1m":
+        return gen.workload_mixed_1m()
+    if name == "uniform_100k":
+        return gen.workload_uniform_100k()
+    return gen.workload_uniform_4m()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the sharded layout is described in DESIGN.md", file=sys.stderr)
+        return 2
+    import fkdtree_amd as F
+    from fkdtree_amd import DeviceArray, Stream, Timer
+
+    info = F.device_info()
+    pts, boxes = workload(args.workload)
+    n, q, dim = pts.n, boxes.q, pts.dim
+    stream = Stream()
+    d_coords = DeviceArray.from_host(pts.coords, stream)
+    d_ids = DeviceArray.from_host(pts.ids, stream)
+    d_mins = DeviceArray.from_host(boxes.mins, stream)
+    d_maxs = DeviceArray.from_host(boxes.maxs, stream)
+    tree = F.FKDTree.from_device(dim, n, d_coords.ptr, d_ids.ptr, stream)
+
+    # --- build -----------------------------------------------------------
+    for _ in range(args.warmup):
+        tree.build_async()
+    stream.sync()
+    tm = Timer()
+    tm.start(stream)
+    for _ in range(args.steps):
+        tree.build_async()
+    tm.stop(stream)
+    build_ms = tm.ms() / args.steps
+
+    # --- search: size the output once ------------------------------------
+    d_off = DeviceArray(q + 1, np.uint64)
+    tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, None, 0, stream=stream, fill=False)
+    total = int(d_off.download(stream)[-1])
+    d_out = DeviceArray(max(total, 1), np.uint32)
+    d_cnt = DeviceArray(q, np.uint32)
+    d_vis = DeviceArray(q, np.uint32)
+    from fkdtree_amd._lib import check, lib
+    check(lib().fkd_search_stats(tree._h, q, d_mins.ptr, d_maxs.ptr, None, d_cnt.ptr, d_vis.ptr, stream.handle))
+    visited = d_vis.download(stream).astype(np.int64)
+    hits = d_cnt.download(stream).astype(np.int64)
+    assert int(hits.sum()) == total
+
+    def step():
+        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.sync()
+    t_wall = time.perf_counter()
+    tm.start(stream)
+    for _ in range(args.steps):
+        step()
+    tm.stop(stream)
+    search_ms = tm.ms() / args.steps
+    wall_ms = (time.perf_counter() - t_wall) * 1e3 / args.steps
+
+    # --- per-kernel timing of the two traversal kernels ------------------
+    tm.start(stream)
+    for _ in range(args.steps):
+        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, None, 0, stream=stream, fill=False)
+    tm.stop(stream)
+    count_ms = tm.ms() / args.steps  # count kernel + scan
+    tm.start(stream)
+    for _ in range(args.steps):
+        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream, count=False)
+    tm.stop(stream)
+    fill_ms = tm.ms() / args.steps
+
+    V = int(visited.sum())
+    H = total
+    # Algorithmic bytes per query of the fill kernel: 24 B box + 8 B offset +
+    # 12 B per visited node (3 coordinates) + 8 B per hit (id read + write).
+    fill_bytes = 32 * q + 12 * V + 8 * H
+    # Whole search (count + fill), the per-query model 28 + 12 V + 8 H.
+    search_bytes = 28 * q + 12 * V + 8 * H
+    # Build: per depth a 4 B key read plus a 16 B read and write per point.
+    D = int(np.floor(np.log2(max(n, 1))))
+    build_bytes = 36 * n * D
+    achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
+
+    res = {
+        "metric": "box queries/s + builds/s at 1M 3D pts; batched vs plain events/s; p99 ms",
+        "value": q / (search_ms * 1e-3),
+        "unit": "box queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": search_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (deterministic SplitMix64 generator, fkdtree_amd/gen.py)",
+        "config": {"workload": args.workload, "points": n, "queries": q, "dim": dim,
+                   "half_width": 0.01 if args.workload == "mixed_1m" else None},
+        "builds_per_s": 1e3 / build_ms,
+        "build_ms": build_ms,
+        "search_ms": search_ms,
+        "search_wall_ms": wall_ms,
+        "count_scan_ms": count_ms,
+        "fill_ms": fill_ms,
+        "hits": H,
+        "visited_nodes": V,
+        "search_algorithmic_GBs": search_bytes / (search_ms * 1e-3) / 1e9,
+        "build_algorithmic_GBs": build_bytes / (build_ms * 1e-3) / 1e9,
+        "roofline": {"bound": "hbm", "kernel": "search_kernel<3,kFill>", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": None},
+        "device": info["name"],
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads)
+    print(json.dumps(res), flush=True)
+    tree.close()
+    return 0
+
+
+def cpu_baseline(pts, boxes, threads):
+    """Reference CPU tree (oracle/_ref, compiled from the reference sources)
+    when present, else the oracle restatement: build on 1 thread
+    (main.cpp:201-205), search_in_the_box of every box on all host cores
+    (OpenMP in place of tbb::parallel_for, main.cpp:372)."""
+    import oracle
+    cores = threads or os.cpu_count() or 1
+    if oracle.ref_available():
+        kind = "reference"
+        rt = oracle.RefTree(pts.coords, pts.ids)
+        t0 = time.perf_counter()
+        rt.build()
+        build_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        off, _ = rt.search(boxes.mins, boxes.maxs, variant=0, threads=cores)
+        search_s = time.perf_counter() - t0
+        rt.close()
+    else:
+        kind = "port"
+        t0 = time.perf_counter()
+        tc, ti = oracle.build(pts.coords, pts.ids)
+        build_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        off, _ = oracle.search(tc, ti, boxes.mins, boxes.maxs, threads=cores)
+        search_s = time.perf_counter() - t0
+    return {"value": boxes.q / search_s, "unit": "box queries/s", "cores": cores, "kind": kind,
+            "sample": f"all {boxes.q} boxes of the workload, one build of all {pts.n} points on 1 thread",
+            "build_ms_1thread": build_s * 1e3, "hits": int(off[-1])}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -29,6 +29,7 @@ _SIGS = {
     "fkd_tree_info": (I32, [P, C.POINTER(I32), C.POINTER(U32), C.POINTER(U64)]),
     "fkd_search_count": (I32, [P, U32, P, P, P, P, P]),
     "fkd_search_fill": (I32, [P, U32, P, P, P, P, P, U64, P]),
+    "fkd_search_stats": (I32, [P, U32, P, P, P, P, P, P]),
     "fkd_search_host": (I32, [P, U32, P, P, P, P, P]),
     "fkd_fetch_results": (I32, [P, P, P]),
     "fkd_device_info": (I32, [C.c_char_p, SZ, C.POINTER(I32), C.POINTER(SZ), C.POINTER(SZ)]),

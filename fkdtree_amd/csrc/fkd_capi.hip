@@ -278,7 +278,7 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
     a.mins = mins;
     a.maxs = maxs;
     a.counts = t->q_counts.as<uint32_t>();
-    fkd::search_kernel<DIM, false><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
+    fkd::search_kernel<DIM, fkd::kCount><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
     FKD_TRY(launch_check("search_count"));
   }
   const uint64_t st = fkd::scan_tiles(nq);
@@ -288,6 +288,24 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
   fkd::scan_exclusive_kernel<uint64_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
       t->q_counts.as<uint32_t>(), qoff, nq, t->desc.as<uint64_t>(), t->ctr.as<uint32_t>());
   return launch_check("search_scan");
+}
+
+template <int DIM>
+int search_stats_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                      uint32_t* counts, uint32_t* visited, hipStream_t s) {
+  if (nq == 0) return FKD_OK;
+  fkd::SearchArgs a{};
+  a.nodes = t->nodes.as<float>();
+  a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
+  a.n_single = t->P;
+  a.qev = qev;
+  a.nq = nq;
+  a.mins = mins;
+  a.maxs = maxs;
+  a.counts = counts;
+  a.visited = visited;
+  fkd::search_kernel<DIM, fkd::kStats><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
+  return launch_check("search_stats");
 }
 
 template <int DIM>
@@ -305,7 +323,7 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
   a.qoff = qoff;
   a.out = out;
   a.cap = cap;
-  fkd::search_kernel<DIM, true><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
+  fkd::search_kernel<DIM, fkd::kFill><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
   return launch_check("search_fill");
 }
 
@@ -549,6 +567,18 @@ int fkd_search_fill(fkd_tree_t t, uint32_t nq, const float* mins, const float* m
   hipStream_t s = static_cast<hipStream_t>(stream);
   return dispatch_dim(t->dim, [&](auto D) {
     return search_fill_impl<decltype(D)::value>(t, nq, mins, maxs, qev, dev_offsets, out, capacity, s);
+  });
+}
+
+int fkd_search_stats(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                     uint32_t* dev_counts, uint32_t* dev_visited, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  if (nq && (!mins || !maxs || !dev_counts || !dev_visited)) return fail(FKD_ERR_INVALID, "NULL search buffer");
+  if (!qev && t->n_events > 1) return fail(FKD_ERR_INVALID, "query events required for a batch of trees");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dispatch_dim(t->dim, [&](auto D) {
+    return search_stats_impl<decltype(D)::value>(t, nq, mins, maxs, qev, dev_counts, dev_visited, s);
   });
 }
 

@@ -24,6 +24,7 @@ struct SearchArgs {
   const float* mins;      // [DIM][nq]
   const float* maxs;
   uint32_t* counts;       // count pass output
+  uint32_t* visited;      // stats pass: nodes visited per query
   const uint64_t* qoff;   // fill pass: first output slot of every query
   uint32_t* out;          // fill pass output
   uint64_t cap;           // capacity of out
@@ -44,8 +45,11 @@ __device__ inline void load_rec(const float* nodes, uint32_t i, float (&c)[DIM],
   }
 }
 
-template <int DIM, bool FILL>
+enum SearchMode { kCount = 0, kFill = 1, kStats = 2 };
+
+template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
+  constexpr bool FILL = MODE == kFill;
   const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
   if (q >= a.nq) return;
   uint32_t base = 0, n = a.n_single;
@@ -63,8 +67,9 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   const float* nodes = a.nodes + (uint64_t)base * NodeRec<DIM>::kFloats;
   uint64_t w = FILL ? a.qoff[q] : 0;
   uint32_t c = 0;
-  uint32_t node = 0, depth = 0, dd = 0, pending = 0;
+  uint32_t node = 0, depth = 0, dd = 0, pending = 0, vis = 0;
   while (n != 0) {
+    if (MODE == kStats) ++vis;
     float x[DIM];
     uint32_t id;
     load_rec<DIM>(nodes, node, x, id);
@@ -107,6 +112,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     }
   }
   if (!FILL) a.counts[q] = c;
+  if (MODE == kStats) a.visited[q] = vis;
 }
 
 }  // namespace fkd

@@ -104,6 +104,13 @@ int fkd_search_fill(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const f
                     const uint32_t* dev_query_events, const uint64_t* dev_offsets,
                     uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream);
 
+/* Traversal statistics: hits and visited nodes per query (the visited set is
+ * the reference's, FKDTree_cpu.h:150-163), for traffic accounting.
+ * Enqueues only.  No reference counterpart. */
+int fkd_search_stats(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
+                     const uint32_t* dev_query_events, uint32_t* dev_counts, uint32_t* dev_visited,
+                     fkd_stream_t stream);
+
 /* Host convenience: boxes in host memory (SoA), host_offsets[nq+1] filled,
  * result ids kept on the device until fkd_fetch_results.  Synchronous. */
 int fkd_search_host(fkd_tree_t tree, uint32_t nq, const float* host_mins, const float* host_maxs,

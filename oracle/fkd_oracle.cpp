@@ -19,6 +19,12 @@
 // results written by the compiled reference (tests/golden), a brute-force
 // builder (full sort of every interval) and the subtree-bounds invariant for
 // the build, and a brute-force scan for the search.
+// ORC_MUTATE (tests only, never set in a normal build) plants a known mistake
+// so the tests can show that their pins catch it.
+#ifndef ORC_MUTATE
+#define ORC_MUTATE 0
+#endif
+
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -36,8 +42,7 @@ inline unsigned floor_log2(uint64_t x) { return 63u - (unsigned)__builtin_clzll(
 inline uint64_t left_size(uint64_t len) {
   if (len == 1) return 0;
   uint64_t p = 1ull << floor_log2(len);
-  if (p / 2 - 1 <= len - p) return p - 1;
-  return len - p / 2;
+  return (p / 2 - 1 <= len - p) ? p - 1 : len - p / 2;
 }
 
 struct Pt {
@@ -63,14 +68,14 @@ void build_one(int dim, uint64_t n, const float* coords, uint64_t cstride,
     out_ids[pos] = p.id;
   };
   for (unsigned depth = 0; depth < depth_max; ++depth) {
-    const int d = depth % dim;
+    const int d = (depth + (ORC_MUTATE == 2 ? 1 : 0)) % dim;
     const uint64_t first = (1ull << depth) - 1;
     for (uint64_t t = 0; t < (1ull << depth); ++t) {
       const uint64_t node = first + t;
       const uint64_t k = left_size(ilen[node]);
       auto b = pts.begin() + imin[node];
       std::nth_element(b, b + k, b + ilen[node], [d](const Pt& a, const Pt& c) {
-        if (a.x[d] == c.x[d]) return a.id < c.id;
+        if (a.x[d] == c.x[d]) return ORC_MUTATE == 1 ? a.id > c.id : a.id < c.id;
         return a.x[d] < c.x[d];
       });
       store(pts[imin[node] + k], node);
@@ -114,7 +119,8 @@ uint64_t search_one(int dim, uint64_t n, const float* tree, uint64_t tstride,
         bool in = true;
         for (int e = 0; e < dim; ++e) {
           const float w = tree[e * tstride + idx];
-          if (!(w <= b.mx[e] && w >= b.mn[e])) {
+          const bool below_max = ORC_MUTATE == 4 ? w < b.mx[e] : w <= b.mx[e];
+          if (!(below_max && w >= b.mn[e])) {
             in = false;
             break;
           }
@@ -122,7 +128,7 @@ uint64_t search_one(int dim, uint64_t n, const float* tree, uint64_t tstride,
         if (in) emit(tids[idx]);
       }
       const bool lower = v < b.mn[d];
-      const int s0 = lower, s1 = lower || inter;
+      const int s0 = lower, s1 = ORC_MUTATE == 3 ? lower : (lower || inter);
       for (int s = s0; s <= s1; ++s) {
         const uint64_t c = 2 * idx + 1 + s;
         if (c < n) nxt.push_back(c);

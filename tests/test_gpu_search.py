@@ -1,0 +1,80 @@
+"""GPU box search parity: every query's sorted id set bit-exact against the
+oracle's level-BFS search (FKDTree_cpu.h:123-169) and a brute-force scan
+(FKDTree.h:54-86), through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle
+from fkdtree_amd import FKDEvents, FKDTree, gen
+from tests.helpers import KINDS, adversarial_points, self_boxes, sorted_sets, special_boxes
+
+pytestmark = pytest.mark.gpu
+
+
+def check_search(p: gen.PointSet, boxes: gen.Boxes, label: str, brute: bool = True):
+    print(f"[search] {label} n={p.n} q={boxes.q}", flush=True)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    t.build()
+    goff, gids = t.search_in_the_box_multiple_csr(boxes.mins, boxes.maxs)
+    tc, ti = oracle.build(p.coords, p.ids)
+    ooff, oids = oracle.search(tc, ti, boxes.mins, boxes.maxs)
+    assert np.array_equal(goff, ooff), f"{label}: counts differ"
+    assert np.array_equal(sorted_sets(goff, gids), sorted_sets(ooff, oids)), f"{label}: ids differ"
+    if brute:
+        boff, bids = oracle.brute(p.coords, p.ids, boxes.mins, boxes.maxs)
+        assert np.array_equal(goff, boff)
+        assert np.array_equal(sorted_sets(goff, gids), sorted_sets(boff, bids))
+    counts = t.search_in_the_box_multiple(boxes.mins, boxes.maxs)
+    assert np.array_equal(counts, np.diff(ooff).astype(np.uint32))
+    t.close()
+    return goff, gids
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_small_searches(require_gpu, kind):
+    for n in (1, 2, 3, 4, 7, 8, 100, 1000, 4097):
+        p = adversarial_points(kind, n)
+        check_search(p, self_boxes(p, 0.05), f"{kind} hw0.05")
+        check_search(p, special_boxes(p), f"{kind} special")
+
+
+@pytest.mark.parametrize("dim", [1, 2, 4])
+def test_other_dims(require_gpu, dim):
+    for n in (1, 5, 1000, 20000):
+        p = adversarial_points("mixed", n, dim=dim)
+        check_search(p, self_boxes(p, 0.03), f"dim{dim}")
+
+
+def test_single_box_api(require_gpu):
+    p = adversarial_points("mixed", 5000)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    t.build()
+    lo, hi = [0.2, 0.2, 0.2], [0.5, 0.6, 0.7]
+    got = np.sort(t.search_in_the_box(lo, hi))
+    inside = np.all((p.coords >= np.float32(lo)[:, None]) & (p.coords <= np.float32(hi)[:, None]), axis=0)
+    assert np.array_equal(got, np.sort(p.ids[inside]))
+
+
+def test_uniform_100k_all_points(require_gpu):
+    p, b = gen.workload_uniform_100k()
+    check_search(p, b, "uniform_100k", brute=False)
+
+
+def test_mixed_1m_sample(require_gpu):
+    p, b = gen.workload_mixed_1m()
+    sub = gen.Boxes(np.ascontiguousarray(b.mins[:, :20000]), np.ascontiguousarray(b.maxs[:, :20000]))
+    check_search(p, sub, "mixed_1m[:20000]", brute=False)
+
+
+def test_events_search(require_gpu):
+    b = gen.workload_events_2048(n_events=64, scale=0.05)
+    boxes = gen.event_self_boxes(b, 0.01)
+    qev = np.repeat(np.arange(b.n_events, dtype=np.uint32), np.diff(b.offsets))
+    print(f"[search] events n={b.points.n}", flush=True)
+    t = FKDEvents(b.points.coords, b.points.ids, b.offsets)
+    t.build()
+    goff, gids = t.search(boxes.mins, boxes.maxs, qev)
+    tc, ti = oracle.build_events(b.points.coords, b.points.ids, b.offsets)
+    ooff, oids = oracle.search_events(tc, ti, b.offsets, qev, boxes.mins, boxes.maxs)
+    assert np.array_equal(goff, ooff)
+    assert np.array_equal(sorted_sets(goff, gids), sorted_sets(ooff, oids))

@@ -1,0 +1,310 @@
+"""Pins the CPU oracle (oracle/fkd_oracle.cpp) before it is trusted.
+
+* golden fixtures written by the reference's own FKDTree_CPU
+  (tests/golden/manifest.json, made by tests/golden/make_golden.py);
+* the compiled reference itself where oracle/_ref was built;
+* an independent brute-force builder (full sort of every interval) for tiny N;
+* the subtree-bounds invariant of the (coordinate, id) order;
+* brute-force search and an independent, vectorised visited-set computation;
+* planted mistakes in the oracle (compiled mutants) must fail these pins.
+"""
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from fkdtree_amd import gen
+from tests.helpers import adversarial_points, self_boxes, special_boxes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MANIFEST = json.load(open(os.path.join(HERE, "golden", "manifest.json")))
+
+
+def left_size(length):
+    if length == 1:
+        return 0
+    p = 1 << (length.bit_length() - 1)
+    return p - 1 if p // 2 - 1 <= length - p else length - p // 2
+
+
+def brute_force_tree(coords, ids):
+    """Reference build semantics by full sorting of every interval (tiny N)."""
+    dim, n = coords.shape
+    out = np.zeros_like(coords)
+    out_ids = np.zeros_like(ids)
+    if n == 0:
+        return out, out_ids
+    work = list(range(n))
+
+    def key(d):
+        return lambda i: (float(coords[d, i]), int(ids[i]))
+
+    intervals = {0: (0, n)}
+    depth_max = n.bit_length() - 1
+    for depth in range(depth_max + 1):
+        d = depth % dim
+        for t in range(1 << depth):
+            node = (1 << depth) - 1 + t
+            if node >= n:
+                continue
+            a, ln = intervals[node]
+            seg = sorted(work[a:a + ln], key=key(d))
+            work[a:a + ln] = seg
+            k = left_size(ln) if depth < depth_max else 0
+            p = seg[k]
+            out[:, node] = coords[:, p]
+            out_ids[node] = ids[p]
+            if 2 * node + 1 < n:
+                intervals[2 * node + 1] = (a, k)
+            if 2 * node + 2 < n:
+                intervals[2 * node + 2] = (a + k + 1, ln - k - 1)
+    return out, out_ids
+
+
+def check_subtree_bounds(tree, tids):
+    """Every node j lies on the correct side of every ancestor a in the
+    (coordinate of a's split dimension, id) order."""
+    dim, n = tree.shape
+    if n < 2:
+        return
+    j = np.arange(1, n, dtype=np.int64)
+    dj = np.floor(np.log2(j + 1)).astype(np.int64)
+    for L in range(int(dj.max())):
+        sel = dj > L
+        jj, dd = j[sel], dj[sel]
+        anc = ((jj + 1) >> (dd - L)) - 1
+        right = (((jj + 1) >> (dd - L - 1)) & 1).astype(bool)
+        d = L % dim
+        xa, xj = tree[d, anc], tree[d, jj]
+        ia, ij = tids[anc], tids[jj]
+        less = (xj < xa) | ((xj == xa) & (ij < ia))
+        greater = (xj > xa) | ((xj == xa) & (ij > ia))
+        assert np.all(np.where(right, greater, less)), f"subtree order violated below depth {L}"
+
+
+def visited_counts(tree, mins, maxs):
+    """Visited nodes per box by the push rule of FKDTree_cpu.h:150-163,
+    evaluated level by level over the whole heap (no queue)."""
+    dim, n = tree.shape
+    nq = mins.shape[1]
+    out = np.zeros(nq, dtype=np.int64)
+    for q in range(nq):
+        vis = np.zeros(n, dtype=bool)
+        vis[0] = n > 0
+        for i in range(n):
+            if not vis[i]:
+                continue
+            d = (int(np.floor(np.log2(i + 1)))) % dim
+            v = tree[d, i]
+            lower = v < mins[d, q]
+            inter = (v <= maxs[d, q]) and (v >= mins[d, q])
+            if not lower and 2 * i + 1 < n:
+                vis[2 * i + 1] = True
+            if (lower or inter) and 2 * i + 2 < n:
+                vis[2 * i + 2] = True
+        out[q] = vis.sum()
+    return out
+
+
+def result_digest(off, ids):
+    srt = oracle.sort_segments(off, ids)
+    return gen.digest(np.diff(off).astype(np.uint32), srt)
+
+
+# ---------------------------------------------------------------- golden
+
+def test_manifest_inputs_reproduce():
+    """The generator still produces the inputs the fixtures were made from."""
+    for key in ("mixed/3/100", "shuffled_ids/3/4097", "zeros/4/1000", "huge_values/3/17"):
+        kind, dim, n = key.split("/")
+        p = adversarial_points(kind, int(n), dim=int(dim))
+        assert gen.digest(p.coords, p.ids) == MANIFEST["trees"][key]["input_sha"], key
+
+
+@pytest.mark.parametrize("key", sorted(MANIFEST["trees"]))
+def test_oracle_build_matches_reference_fixture(key):
+    kind, dim, n = key.split("/")
+    p = adversarial_points(kind, int(n), dim=int(dim))
+    e = MANIFEST["trees"][key]
+    assert gen.digest(p.coords, p.ids) == e["input_sha"]
+    c, i = oracle.build(p.coords, p.ids)
+    assert gen.digest(c, i) == e["tree_sha"], key
+    if "ids" in e:
+        assert i.tolist() == e["ids"]
+        assert c.view(np.uint32).tolist() == e["coords_bits"]
+
+
+@pytest.mark.parametrize("key", sorted(MANIFEST["searches"]))
+def test_oracle_search_matches_reference_fixture(key):
+    kind, dim, n, bname = key.split("/")
+    p = adversarial_points(kind, int(n), dim=int(dim))
+    boxes = self_boxes(p, 0.05) if bname == "self0.05" else special_boxes(p)
+    c, i = oracle.build(p.coords, p.ids)
+    off, ids = oracle.search(c, i, boxes.mins, boxes.maxs)
+    e = MANIFEST["searches"][key]
+    assert int(off[-1]) == e["total"]
+    assert result_digest(off, ids) == e["result_sha"], key
+
+
+def test_oracle_uniform_100k_fixture():
+    w = MANIFEST["workloads"]["uniform_100k"]
+    p, b = gen.workload_uniform_100k()
+    assert gen.digest(p.coords, p.ids) == w["input_sha"]
+    assert gen.digest(b.mins, b.maxs) == w["boxes_sha"]
+    c, i = oracle.build(p.coords, p.ids)
+    assert gen.digest(c, i) == w["tree_sha"]
+    off, ids = oracle.search(c, i, b.mins, b.maxs)
+    assert int(off[-1]) == w["search"]["total"]
+    assert result_digest(off, ids) == w["search"]["result_sha"]
+
+
+@pytest.mark.slow
+def test_oracle_mixed_1m_fixture():
+    w = MANIFEST["workloads"]["mixed_1m"]
+    p, b = gen.workload_mixed_1m()
+    assert gen.digest(p.coords, p.ids) == w["input_sha"]
+    assert gen.digest(b.mins, b.maxs) == w["boxes_sha"]
+    c, i = oracle.build(p.coords, p.ids)
+    assert gen.digest(c, i) == w["tree_sha"]
+    s = w["search"]["sample"]
+    off, ids = oracle.search(c, i, b.mins[:, :s], b.maxs[:, :s])
+    assert result_digest(off, ids) == w["search"]["result_sha"]
+
+
+def test_oracle_events_fixture():
+    w = MANIFEST["workloads"]["events_256_scale0.1"]
+    b = gen.workload_events_2048(n_events=256, scale=0.1)
+    assert gen.digest(b.points.coords, b.points.ids) == w["input_sha"]
+    assert gen.digest(b.offsets.astype(np.uint64)) == w["offsets_sha"]
+    c, i = oracle.build_events(b.points.coords, b.points.ids, b.offsets)
+    assert gen.digest(c, i) == w["tree_sha"]
+
+
+# ------------------------------------------------- live reference (here)
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("kind", ["mixed", "shuffled_ids", "negative", "zeros"])
+def test_oracle_matches_compiled_reference_bfs_order(kind):
+    """Same tree, and the same ids in the same BFS order as search_in_the_box
+    (for "zeros" the boxes hold thousands of points, which overflows the
+    reference's 1024-slot frontier, FQueue.h:158: compared with _recursive)."""
+    p = adversarial_points(kind, 2_000 if kind == "zeros" else 20_000)
+    rt = oracle.RefTree(p.coords, p.ids)
+    rt.build()
+    rc, ri = rt.tree()
+    c, i = oracle.build(p.coords, p.ids)
+    assert np.array_equal(c.view(np.uint32), rc.view(np.uint32)) and np.array_equal(i, ri)
+    boxes = self_boxes(p, 0.01)
+    off, ids = oracle.search(c, i, boxes.mins, boxes.maxs)
+    if kind == "zeros":
+        roff, rids = rt.search(boxes.mins, boxes.maxs, variant=2)
+        assert np.array_equal(off, roff)
+        assert np.array_equal(oracle.sort_segments(off, ids), oracle.sort_segments(roff, rids))
+    else:
+        roff, rids = rt.search(boxes.mins, boxes.maxs, variant=0)
+        assert np.array_equal(off, roff) and np.array_equal(ids, rids)
+
+
+# ------------------------------------------------- independent pins
+
+@pytest.mark.parametrize("kind", ["uniform", "equal", "zeros", "grid", "shuffled_ids", "huge_values"])
+@pytest.mark.parametrize("dim", [1, 2, 3, 4])
+def test_oracle_build_vs_brute_force_builder(kind, dim):
+    for n in (1, 2, 3, 4, 5, 6, 7, 8, 9, 13, 16, 31, 64, 100):
+        p = adversarial_points(kind, n, dim=dim)
+        c, i = oracle.build(p.coords, p.ids)
+        bc, bi = brute_force_tree(p.coords, p.ids)
+        assert np.array_equal(i, bi), (kind, dim, n)
+        assert np.array_equal(c.view(np.uint32), bc.view(np.uint32)), (kind, dim, n)
+
+
+@pytest.mark.parametrize("kind", ["mixed", "grid", "zeros", "shuffled_ids", "negative"])
+def test_oracle_build_subtree_bounds(kind):
+    p = adversarial_points(kind, 50_000)
+    c, i = oracle.build(p.coords, p.ids)
+    check_subtree_bounds(c, i)
+    # and the stored payload keeps the original bits (add_at_position)
+    src = {int(a): k for k, a in enumerate(p.ids)}
+    idx = np.array([src[int(a)] for a in i])
+    assert np.array_equal(c.view(np.uint32), p.coords[:, idx].view(np.uint32))
+
+
+@pytest.mark.parametrize("kind", ["mixed", "equal", "zeros", "grid", "huge_values"])
+def test_oracle_search_vs_brute_force(kind):
+    p = adversarial_points(kind, 3000)
+    c, i = oracle.build(p.coords, p.ids)
+    for boxes in (self_boxes(p, 0.05), special_boxes(p), self_boxes(p, 1.5)):
+        off, ids = oracle.search(c, i, boxes.mins, boxes.maxs)
+        boff, bids = oracle.brute(p.coords, p.ids, boxes.mins, boxes.maxs)
+        assert np.array_equal(off, boff)
+        assert np.array_equal(oracle.sort_segments(off, ids), oracle.sort_segments(boff, bids))
+
+
+def test_oracle_visited_counts_vs_level_sweep():
+    p = adversarial_points("mixed", 3000)
+    c, i = oracle.build(p.coords, p.ids)
+    boxes = self_boxes(p, 0.05)
+    sub = gen.Boxes(np.ascontiguousarray(boxes.mins[:, :40]), np.ascontiguousarray(boxes.maxs[:, :40]))
+    sp = special_boxes(p)
+    for b in (sub, sp):
+        _, vis = oracle.search_counts(c, i, b.mins, b.maxs)
+        assert np.array_equal(vis.astype(np.int64), visited_counts(c, b.mins, b.maxs))
+
+
+# ------------------------------------------------- planted mistakes
+
+MUTANTS = {
+    1: "tie broken by descending id",
+    2: "split dimension shifted by one depth",
+    3: "right son pushed without the inter test",
+    4: "box test open on the max side",
+}
+
+
+@pytest.fixture(scope="module")
+def mutants():
+    d = tempfile.mkdtemp(prefix="orc_mut_")
+    libs = {}
+    src = os.path.join(os.path.dirname(oracle.__file__), "fkd_oracle.cpp")
+    for m in MUTANTS:
+        out = os.path.join(d, f"liboracle_m{m}.so")
+        subprocess.run(["g++", "-std=c++14", "-O2", "-fPIC", "-fopenmp", "-shared", f"-DORC_MUTATE={m}",
+                        "-o", out, src], check=True)
+        libs[m] = out
+    return libs
+
+
+def _with_lib(path, fn):
+    saved = oracle._lib
+    try:
+        oracle._lib = None
+        oracle.ORACLE_SO_SAVED = oracle.ORACLE_SO
+        oracle.ORACLE_SO = path
+        return fn()
+    finally:
+        oracle.ORACLE_SO = oracle.ORACLE_SO_SAVED
+        oracle._lib = saved
+
+
+@pytest.mark.parametrize("m", sorted(MUTANTS))
+def test_planted_mistake_is_caught(mutants, m):
+    """Each plausible mistake fails at least one of the pins above."""
+    def run():
+        caught = []
+        for key in ("grid/3/4097", "equal/3/100", "mixed/3/4097", "zeros/3/100"):
+            kind, dim, n = key.split("/")
+            p = adversarial_points(kind, int(n), dim=int(dim))
+            c, i = oracle.build(p.coords, p.ids)
+            caught.append(gen.digest(c, i) != MANIFEST["trees"][key]["tree_sha"])
+            for bname in ("self0.05", "special"):
+                sk = f"{key}/{bname}"
+                if sk in MANIFEST["searches"]:
+                    boxes = self_boxes(p, 0.05) if bname == "self0.05" else special_boxes(p)
+                    off, ids = oracle.search(c, i, boxes.mins, boxes.maxs)
+                    caught.append(result_digest(off, ids) != MANIFEST["searches"][sk]["result_sha"])
+        return any(caught)
+    assert _with_lib(mutants[m], run), f"mutant {m} ({MUTANTS[m]}) not caught"
