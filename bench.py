@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--workload", default="mixed_1m", choices=["mixed_1m", "uniform_100k", "uniform_4m"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 256 MiB copy kernel (counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -60,6 +62,16 @@ def main():
     from fkdtree_amd import DeviceArray, Stream, Timer
 
     info = F.device_info()
+    if args.calibrate:
+        from fkdtree_amd._lib import check, lib
+        n = 64 << 20
+        a, b = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32)
+        s = Stream()
+        for _ in range(3):
+            check(lib().fkd_copy_u32(b.ptr, a.ptr, n, s.handle))
+        s.sync()
+        print(json.dumps({"calibration": "copy_u32_kernel", "bytes_read": 4 * n, "bytes_written": 4 * n}))
+        return 0
     pts, boxes = workload(args.workload)
     n, q, dim = pts.n, boxes.q, pts.dim
     stream = Stream()

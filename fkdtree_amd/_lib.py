@@ -37,6 +37,7 @@ _SIGS = {
     "fkd_free": (I32, [P]),
     "fkd_memcpy_htod": (I32, [P, P, SZ, P]),
     "fkd_memcpy_dtoh": (I32, [P, P, SZ, P]),
+    "fkd_copy_u32": (I32, [P, P, U64, P]),
     "fkd_stream_create": (I32, [C.POINTER(P)]),
     "fkd_stream_destroy": (I32, [P]),
     "fkd_stream_sync": (I32, [P]),

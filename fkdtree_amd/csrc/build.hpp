@@ -54,6 +54,38 @@ __global__ void keys_kernel(const float* __restrict__ X, const uint32_t* __restr
   if (nan) atomicOr(err, 1u);
 }
 
+// Key-space bounding box of all events from the sorted arrays (first / last
+// element of every event): bbox[m] = min key, bbox[DIM + m] = max key.
+template <int DIM>
+__global__ void bbox_kernel(const float* __restrict__ X, const uint32_t* __restrict__ A, uint64_t stride,
+                            const uint32_t* __restrict__ off, uint32_t n_events, uint32_t P, uint32_t* bbox) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n_events; e += gridDim.x * blockDim.x) {
+    const uint32_t a = off[e], b = off[e + 1];
+    if (a == b) continue;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      atomicMin(&bbox[m], fkey(X[(uint64_t)m * P + A[m * stride + a]]));
+      atomicMax(&bbox[DIM + m], fkey(X[(uint64_t)m * P + A[m * stride + b - 1]]));
+    }
+  }
+}
+
+// Rank planes of the level arrays: array m at position j holds the ranks of
+// its point in every dimension k: (k == m) ? j : R[k][A_m[j]].
+template <int DIM>
+__global__ void planes_kernel(const uint32_t* __restrict__ A, uint64_t astride, const uint32_t* __restrict__ R,
+                              uint32_t P, uint32_t* __restrict__ planes, uint64_t pstride) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      const uint32_t p = A[m * astride + j];
+      uint32_t* dst = planes + m * pstride;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * P + j] = (k == m) ? j : R[(uint64_t)k * P + p];
+    }
+  }
+}
+
 // Keys and values for the id sort (tie order inside every event).
 __global__ void id_keys_kernel(const uint32_t* __restrict__ ids, uint32_t P, uint32_t* __restrict__ K,
                                uint32_t* __restrict__ V) {
@@ -70,14 +102,14 @@ template <int DIM>
 struct LevelArgs {
   uint32_t L;                 // depth whose intervals are split
   uint32_t d;                 // split dimension L % DIM
-  const uint32_t* Ad;         // array d (not moved at this depth)
-  const uint32_t* Rd;         // ranks in dimension d
-  const uint32_t* src[3];     // the DIM-1 other arrays ...
+  const uint32_t* Ad;         // rank planes of array d (not moved at this depth)
+  const uint32_t* src[3];     // rank planes of the DIM-1 other arrays ...
   uint32_t* dst[3];           // ... and where their partitioned copies go
+  const uint32_t* S0;         // rank in dimension 0 -> point index
   TileMap tm;                 // tiles of kLevelTile points, event aligned
   const float* X;             // input points
   const uint32_t* ids;
-  uint32_t P;
+  uint32_t P;                 // plane stride
   float* Tc;                  // tree outputs
   uint32_t* Tid;
   float* nodes;
@@ -146,12 +178,14 @@ enum : uint32_t { kNone = 0, kLeft = 1, kMed = 2, kRight = 3 };
 // Block b handles moving array b % (DIM-1) over tile b / (DIM-1) (b is the
 // dynamic block index, so look-back waits only on blocks already running).
 // For each interval [start, start+size) of depth L with median rank
-// k = left_size(size) (FKDTree_cpu.h:239): the median is the point at
+// k = left_size(size) (FKDTree_cpu.h:239): the median is the element at
 // start+k of array d; every other array is stably partitioned into
-// [start, start+k) (rank below the median's), start+k (the median) and
-// (start+k, start+size) (rank above).  Blocks of array 0 also store the
-// depth-L nodes (add_at_position, FKDTree_cpu.h:253-255) and, on the last
-// split depth, the leaves (FKDTree_cpu.h:271-276).
+// [start, start+k) (rank in dimension d below the median's), start+k (the
+// median) and (start+k, start+size) (rank above).  Elements carry their
+// ranks in every dimension, so the partition reads and writes only
+// coalesced planes.  Blocks of array 0 also store the depth-L nodes
+// (add_at_position, FKDTree_cpu.h:253-255) and, on the last split depth,
+// the leaves (FKDTree_cpu.h:271-276).
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   constexpr uint32_t M = DIM - 1;
@@ -168,42 +202,47 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   const Shape sh = shape_of(n);
   const bool split = a.L < sh.D;
   const bool root_only = (n == 1 && a.L == 0);
+  const uint64_t P = a.P;
   if (!split) {
     if (M && threadIdx.x == 0) publish(&a.desc[bid], kStPrefix | seg_pack(SegCount{1u, 0u, 0u}));
-    if (root_only && arr == 0 && threadIdx.x == 0) write_node(a, o, 0u, a.Ad[o]);
+    if (root_only && arr == 0 && threadIdx.x == 0) write_node(a, o, 0u, a.S0[a.Ad[o]]);
     return;
   }
   const uint32_t first = lt * (uint32_t)kLevelTile + threadIdx.x * kLevelItems;
   const uint32_t lvl0 = (1u << a.L) - 1u;
   const bool last = (a.L + 1 == sh.D);
+  const uint32_t* Ad_d = a.Ad + (uint64_t)a.d * P;  // plane d of array d
+  const uint32_t* S = M ? a.src[arr] : nullptr;
 
-  uint32_t cls[kLevelItems], pt[kLevelItems], dest0[kLevelItems];
+  uint32_t cls[kLevelItems], dest0[kLevelItems], rk[kLevelItems][DIM];
   SegCount agg{0u, 0u, 0u};
 #pragma unroll
   for (int i = 0; i < kLevelItems; ++i) {
     const uint32_t jl = first + i;
     cls[i] = kNone;
-    pt[i] = 0;
     dest0[i] = 0;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) rk[i][k] = 0;
     if (jl < n) {
       const uint32_t ts = subtree_at(sh, a.L, jl);
       uint32_t start, size;
       subtree_range(sh, a.L, ts, start, size);
       if (jl - start < size) {
         const uint32_t mpos = start + left_size(size);
-        const uint32_t pm = a.Ad[o + mpos];
         if (arr == 0) {
           if (jl == mpos) {
-            write_node(a, o, lvl0 + ts, pm);
+            write_node(a, o, lvl0 + ts, a.S0[a.Ad[o + mpos]]);
           } else if (last) {
-            write_node(a, o, 2u * (lvl0 + ts) + 1u + (jl > mpos ? 1u : 0u), a.Ad[o + jl]);
+            write_node(a, o, 2u * (lvl0 + ts) + 1u + (jl > mpos ? 1u : 0u), a.S0[a.Ad[o + jl]]);
           }
         }
         if (M) {
-          const uint32_t rm = a.Rd[pm];
-          const uint32_t p = a.src[arr][o + jl];
-          const uint32_t r = a.Rd[p];
-          pt[i] = p;
+          const uint32_t rm = Ad_d[o + mpos];
+#pragma unroll
+          for (int k = 0; k < DIM; ++k) rk[i][k] = S[(uint64_t)k * P + o + jl];
+          uint32_t r = rk[i][0];
+#pragma unroll
+          for (int k = 1; k < DIM; ++k) r = (a.d == (uint32_t)k) ? rk[i][k] : r;
           if (jl == start) agg = SegCount{1u, 0u, 0u};
           if (r < rm) {
             cls[i] = kLeft;
@@ -277,11 +316,15 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
       nl = 0;
       nr = 0;
     }
+    uint32_t dest = dest0[i];
     switch (c & 3u) {
-      case kLeft: D[dest0[i] + nl] = pt[i]; ++nl; break;
-      case kRight: D[dest0[i] + nr] = pt[i]; ++nr; break;
-      case kMed: D[dest0[i]] = pt[i]; break;
+      case kLeft: dest += nl; ++nl; break;
+      case kRight: dest += nr; ++nr; break;
       default: break;
+    }
+    if ((c & 3u) != kNone) {
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) D[(uint64_t)k * P + dest] = rk[i][k];
     }
   }
 }

@@ -67,6 +67,10 @@ __device__ inline uint32_t fkey(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+__device__ inline float fkey_inv(uint32_t k) {
+  return (k & 0x80000000u) ? __uint_as_float(k & 0x7fffffffu) : __uint_as_float(~k);
+}
+
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
 template <class T>
@@ -109,6 +113,13 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;
+}
+
+// Plain streaming copy (used to calibrate traffic counters).
+__global__ void copy_u32_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t v, uint64_t n) {

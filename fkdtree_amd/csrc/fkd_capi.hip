@@ -134,42 +134,52 @@ struct fkd_tree_s {
   Tiling radix_tiles, level_tiles;
   DevBuf X, ids;               // input copy
   DevBuf Tc, Tid, nodes;       // outputs
-  DevBuf A, K, R;              // build scratch
-  DevBuf counts, offsets, desc, ctr, err;
+  DevBuf A, K, R, planes;      // build scratch
+  DevBuf counts, offsets, desc, ctr, err, bbox;
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev;  // search scratch
+  DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets, q_desc;
+  uint32_t q_plan_nq = 0;  // queries whose processing order q_perm holds
   uint64_t last_nq = 0, last_total = 0;
 };
 
 namespace {
+
+// One stable LSD digit pass over M key arrays (count, scan, scatter).
+int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts, uint32_t* offsets,
+               uint64_t* desc, uint32_t* ctr, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
+               uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift, bool write_keys,
+               bool write_ranks, hipStream_t s) {
+  const uint64_t nc = (uint64_t)M * 256u * tm.T;
+  dim3 grid(tm.T, M);
+  fkd::radix_count_kernel<<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts);
+  FKD_TRY(launch_check("radix_count"));
+  const uint64_t st = fkd::scan_tiles(nc);
+  FKD_TRY(fill_u64(desc, 0, st, s));
+  FKD_TRY(fill_u32(ctr, 0, 1, s));
+  fkd::scan_exclusive_kernel<uint32_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(counts, offsets, nc, desc, ctr);
+  FKD_TRY(launch_check("radix_scan"));
+  if (write_keys && !write_ranks)
+    fkd::radix_scatter_kernel<true, false><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks, P,
+                                                                          shift, tm, offsets);
+  else if (!write_keys && write_ranks)
+    fkd::radix_scatter_kernel<false, true><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks, P,
+                                                                          shift, tm, offsets);
+  else if (write_keys && write_ranks)
+    fkd::radix_scatter_kernel<true, true><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks, P,
+                                                                         shift, tm, offsets);
+  else
+    fkd::radix_scatter_kernel<false, false><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks, P,
+                                                                           shift, tm, offsets);
+  return launch_check("radix_scatter");
+}
 
 template <int DIM>
 int run_radix_pass(fkd_tree_s* t, uint32_t M, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
                    uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift, bool write_keys,
                    bool write_ranks, hipStream_t s) {
   const fkd::TileMap tm = t->radix_tiles.map(t->off.as<uint32_t>());
-  const uint64_t nc = (uint64_t)M * 256u * tm.T;
-  dim3 grid(tm.T, M);
-  fkd::radix_count_kernel<<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, t->counts.as<uint32_t>());
-  FKD_TRY(launch_check("radix_count"));
-  const uint64_t st = fkd::scan_tiles(nc);
-  FKD_TRY(fill_u64(t->desc.as<uint64_t>(), 0, st, s));
-  FKD_TRY(fill_u32(t->ctr.as<uint32_t>(), 0, 1, s));
-  fkd::scan_exclusive_kernel<uint32_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
-      t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), nc, t->desc.as<uint64_t>(), t->ctr.as<uint32_t>());
-  FKD_TRY(launch_check("radix_scan"));
-  if (write_keys && !write_ranks)
-    fkd::radix_scatter_kernel<true, false><<<grid, fkd::kThreads, 0, s>>>(
-        kin, kout, vin, vout, stride, ranks, t->P, shift, tm, t->offsets.as<uint32_t>());
-  else if (!write_keys && write_ranks)
-    fkd::radix_scatter_kernel<false, true><<<grid, fkd::kThreads, 0, s>>>(
-        kin, kout, vin, vout, stride, ranks, t->P, shift, tm, t->offsets.as<uint32_t>());
-  else if (write_keys && write_ranks)
-    fkd::radix_scatter_kernel<true, true><<<grid, fkd::kThreads, 0, s>>>(
-        kin, kout, vin, vout, stride, ranks, t->P, shift, tm, t->offsets.as<uint32_t>());
-  else
-    fkd::radix_scatter_kernel<false, false><<<grid, fkd::kThreads, 0, s>>>(
-        kin, kout, vin, vout, stride, ranks, t->P, shift, tm, t->offsets.as<uint32_t>());
-  return launch_check("radix_scatter");
+  return radix_pass(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), t->desc.as<uint64_t>(),
+                    t->ctr.as<uint32_t>(), kin, kout, vin, vout, stride, ranks, shift, write_keys, write_ranks, s);
 }
 
 template <int DIM>
@@ -207,7 +217,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>());
   FKD_TRY(launch_check("keys"));
   // 3. Stable LSD sort of every coordinate inside every event; the last pass
-  //    also stores each point's rank.
+  //    stores each point's rank R[m][p] (the values end in buffer 0).
   {
     int c = 0;
     for (int pass = 0; pass < 4; ++pass) {
@@ -217,10 +227,18 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
       c = 1 - c;
     }
   }
-  // The rank scatter writes R[m][p] for m < DIM with stride P: radix passes
-  // use stride 2*S for keys/values, so ranks were addressed m * P.  Values are
-  // now in buffer 0 of every array.
-  // 4. One partition launch per depth.
+  // Bounding box (used to order search queries) from the sorted arrays.
+  FKD_TRY(fill_u32(t->bbox.as<uint32_t>(), 0xffffffffu, DIM, s));
+  FKD_TRY(fill_u32(t->bbox.as<uint32_t>() + DIM, 0u, DIM, s));
+  fkd::bbox_kernel<DIM><<<grid_for(t->n_events), 256, 0, s>>>(X, Abuf(0, 0), 2 * S, t->off.as<uint32_t>(),
+                                                               t->n_events, P, t->bbox.as<uint32_t>());
+  FKD_TRY(launch_check("bbox"));
+  // 4. Rank planes: every array element carries its ranks in all dimensions.
+  uint32_t* PL = t->planes.as<uint32_t>();  // PL[m][c][k][P]
+  auto Pbuf = [&](int m, int c) { return PL + (((uint64_t)m * 2 + c) * DIM) * S; };
+  fkd::planes_kernel<DIM><<<grid, 256, 0, s>>>(Abuf(0, 0), 2 * S, R, P, Pbuf(0, 0), 2 * (uint64_t)DIM * S);
+  FKD_TRY(launch_check("planes"));
+  // 5. One partition launch per depth.
   int cur[4] = {0, 0, 0, 0};
   const uint32_t levels = std::max<uint32_t>(1, t->Dmax);
   const fkd::TileMap tm = t->level_tiles.map(t->off.as<uint32_t>());
@@ -229,14 +247,13 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     fkd::LevelArgs<DIM> a;
     a.L = L;
     a.d = L % DIM;
-    a.Ad = Abuf(a.d, cur[a.d]);
-    a.Rd = R + (uint64_t)a.d * S;
+    a.Ad = Pbuf(a.d, cur[a.d]);
     int k = 0;
     int moved[3] = {0, 0, 0};
     for (int m = 0; m < DIM; ++m) {
       if ((uint32_t)m == a.d) continue;
-      a.src[k] = Abuf(m, cur[m]);
-      a.dst[k] = Abuf(m, 1 - cur[m]);
+      a.src[k] = Pbuf(m, cur[m]);
+      a.dst[k] = Pbuf(m, 1 - cur[m]);
       moved[k] = m;
       ++k;
     }
@@ -244,6 +261,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
       a.src[k] = nullptr;
       a.dst[k] = nullptr;
     }
+    a.S0 = Abuf(0, 0);
     a.tm = tm;
     a.X = X;
     a.ids = t->ids.as<uint32_t>();
@@ -261,6 +279,78 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     for (uint32_t i = 0; i < M; ++i) cur[moved[i]] ^= 1;
   }
   t->built = true;
+  t->q_plan_nq = 0;
+  return FKD_OK;
+}
+
+constexpr int kQueryKeyBits = 8;  // Morton bits per dimension of the query order
+
+template <int DIM, int MODE>
+int launch_search(fkd_tree_s* t, const fkd::SearchArgs& a, hipStream_t s, const char* what) {
+  const uint32_t blocks = (a.nq + fkd::kThreads - 1) / fkd::kThreads;
+  const uint64_t bytes = (uint64_t)t->P * fkd::NodeRec<DIM>::kFloats * 4;
+  fkd::SearchArgs b = a;
+  b.node_bytes = (uint32_t)std::min<uint64_t>(bytes, 0xffffffffull);
+  if (bytes < (1ull << 32))
+    fkd::search_kernel<DIM, MODE, true><<<blocks, fkd::kThreads, 0, s>>>(b);
+  else
+    fkd::search_kernel<DIM, MODE, false><<<blocks, fkd::kThreads, 0, s>>>(b);
+  return launch_check(what);
+}
+
+template <int DIM>
+fkd::SearchArgs search_args(fkd_tree_s* t, uint32_t nq, const uint32_t* qev) {
+  fkd::SearchArgs a{};
+  a.nodes = t->nodes.as<float>();
+  a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
+  a.n_single = t->P;
+  a.qev = qev;
+  a.nq = nq;
+  return a;
+}
+
+// Processing order of the queries (spatial sort of the box centres) and the
+// boxes permuted into it.  Kept in the tree for the fill pass that follows.
+template <int DIM>
+int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, hipStream_t s) {
+  const size_t n = std::max<uint32_t>(nq, 1);
+  FKD_TRY(t->q_keys.ensure(n * 2 * 4));
+  FKD_TRY(t->q_perm.ensure(n * 2 * 4));
+  FKD_TRY(t->q_sbox.ensure(n * 2 * DIM * 4));
+  const uint32_t T = (uint32_t)((nq + fkd::kRadixTile - 1) / fkd::kRadixTile);
+  const uint64_t nc = 256ull * std::max<uint32_t>(T, 1);
+  FKD_TRY(t->q_rcounts.ensure(nc * 4));
+  FKD_TRY(t->q_roffsets.ensure((nc + 1) * 4));
+  FKD_TRY(t->q_desc.ensure(std::max<uint64_t>(fkd::scan_tiles(nc), fkd::scan_tiles(nq)) * 8));
+  uint32_t* keys = t->q_keys.as<uint32_t>();
+  uint32_t* perm = t->q_perm.as<uint32_t>();
+  const uint32_t g = grid_for(nq);
+  fkd::query_keys_kernel<DIM, kQueryKeyBits><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), keys, perm);
+  FKD_TRY(launch_check("query_keys"));
+  fkd::TileMap tm{};
+  tm.tile_ev = nullptr;
+  tm.T = T;
+  tm.single_n = nq;
+  constexpr int passes = (kQueryKeyBits * DIM + 7) / 8;
+  int c = 0;
+  for (int pass = 0; pass < passes; ++pass) {
+    const bool lastp = pass == passes - 1;
+    FKD_TRY(radix_pass(tm, nq, 1, t->q_rcounts.as<uint32_t>(), t->q_roffsets.as<uint32_t>(),
+                       t->q_desc.as<uint64_t>(), t->ctr.as<uint32_t>(), keys + (uint64_t)c * n,
+                       keys + (uint64_t)(1 - c) * n, perm + (uint64_t)c * n, perm + (uint64_t)(1 - c) * n, n,
+                       nullptr, 8u * pass, !lastp, false, s));
+    c = 1 - c;
+  }
+  if (c != 0) {
+    // leave the final order in perm[0..nq)
+    fkd::copy_u32_kernel<<<g, 256, 0, s>>>(perm, perm + n, nq);
+    FKD_TRY(launch_check("perm_copy"));
+  }
+  float* smins = t->q_sbox.as<float>();
+  float* smaxs = smins + (uint64_t)DIM * n;
+  fkd::permute_boxes_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, perm, nq, smins, smaxs);
+  FKD_TRY(launch_check("permute_boxes"));
+  t->q_plan_nq = nq;
   return FKD_OK;
 }
 
@@ -269,17 +359,14 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
                       uint64_t* qoff, hipStream_t s) {
   FKD_TRY(t->q_counts.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
   if (nq > 0) {
-    fkd::SearchArgs a{};
-    a.nodes = t->nodes.as<float>();
-    a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
-    a.n_single = t->P;
-    a.qev = qev;
-    a.nq = nq;
-    a.mins = mins;
-    a.maxs = maxs;
+    FKD_TRY(plan_queries<DIM>(t, nq, mins, maxs, s));
+    fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
+    const size_t n = std::max<uint32_t>(nq, 1);
+    a.perm = t->q_perm.as<uint32_t>();
+    a.mins = t->q_sbox.as<float>();
+    a.maxs = a.mins + (uint64_t)DIM * n;
     a.counts = t->q_counts.as<uint32_t>();
-    fkd::search_kernel<DIM, fkd::kCount><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
-    FKD_TRY(launch_check("search_count"));
+    FKD_TRY((launch_search<DIM, fkd::kCount>(t, a, s, "search_count")));
   }
   const uint64_t st = fkd::scan_tiles(nq);
   FKD_TRY(t->desc.ensure(st * 8));
@@ -294,37 +381,30 @@ template <int DIM>
 int search_stats_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                       uint32_t* counts, uint32_t* visited, hipStream_t s) {
   if (nq == 0) return FKD_OK;
-  fkd::SearchArgs a{};
-  a.nodes = t->nodes.as<float>();
-  a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
-  a.n_single = t->P;
-  a.qev = qev;
-  a.nq = nq;
+  fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
   a.mins = mins;
   a.maxs = maxs;
   a.counts = counts;
   a.visited = visited;
-  fkd::search_kernel<DIM, fkd::kStats><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
-  return launch_check("search_stats");
+  return launch_search<DIM, fkd::kStats>(t, a, s, "search_stats");
 }
 
 template <int DIM>
 int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                      const uint64_t* qoff, uint32_t* out, uint64_t cap, hipStream_t s) {
   if (nq == 0) return FKD_OK;
-  fkd::SearchArgs a{};
-  a.nodes = t->nodes.as<float>();
-  a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
-  a.n_single = t->P;
-  a.qev = qev;
-  a.nq = nq;
-  a.mins = mins;
-  a.maxs = maxs;
+  if (t->q_plan_nq != nq)
+    return fail(FKD_ERR_STATE, "fkd_search_fill must follow fkd_search_count with the same queries");
+  fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
+  const size_t n = std::max<uint32_t>(nq, 1);
+  a.perm = t->q_perm.as<uint32_t>();
+  a.mins = t->q_sbox.as<float>();
+  a.maxs = a.mins + (uint64_t)DIM * n;
   a.qoff = qoff;
   a.out = out;
   a.cap = cap;
-  fkd::search_kernel<DIM, fkd::kFill><<<(nq + fkd::kThreads - 1) / fkd::kThreads, fkd::kThreads, 0, s>>>(a);
-  return launch_check("search_fill");
+  a.out_aligned = (reinterpret_cast<uintptr_t>(out) & 63u) == 0;
+  return launch_search<DIM, fkd::kFill>(t, a, s, "search_fill");
 }
 
 __global__ void check_input_kernel(const float* X, const uint32_t* ids, const uint32_t* off,
@@ -386,6 +466,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(t->A.ensure(S * dim * 2 * 4));
   FKD_TRY(t->K.ensure(S * dim * 2 * 4));
   FKD_TRY(t->R.ensure(S * dim * 4));
+  FKD_TRY(t->planes.ensure(S * dim * dim * 2 * 4));
   const uint64_t nc = (uint64_t)dim * 256u * std::max<uint32_t>(t->radix_tiles.T, 1);
   FKD_TRY(t->counts.ensure(nc * 4));
   FKD_TRY(t->offsets.ensure((nc + 1) * 4));
@@ -393,6 +474,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(t->desc.ensure(ndesc * 8));
   FKD_TRY(t->ctr.ensure(64));
   FKD_TRY(t->err.ensure(64));
+  FKD_TRY(t->bbox.ensure(64));
   FKD_TRY(fill_u32(t->err.as<uint32_t>(), 0, 1, s));
   if (P) {
     check_input_kernel<<<grid_for(P), 256, 0, s>>>(t->X.as<float>(), t->ids.as<uint32_t>(), t->off.as<uint32_t>(),
@@ -633,7 +715,7 @@ int fkd_device_info(char* name, size_t name_len, int* compute_units, size_t* fre
   hipDeviceProp_t prop;
   FKD_HIP(hipGetDeviceProperties(&prop, dev));
   if (name && name_len) {
-    std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    std::snprintf(name, name_len, "%s%s(%s)", prop.name, prop.name[0] ? " " : "", prop.gcnArchName);
   }
   if (compute_units) *compute_units = prop.multiProcessorCount;
   size_t fr = 0, tot = 0;
@@ -661,6 +743,11 @@ int fkd_memcpy_dtoh(void* dst, const void* src, size_t bytes, fkd_stream_t strea
   if (bytes) FKD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
   FKD_HIP(hipStreamSynchronize(s));
   return FKD_OK;
+}
+int fkd_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, fkd_stream_t stream) {
+  if (n == 0) return FKD_OK;
+  fkd::copy_u32_kernel<<<2048, 256, 0, static_cast<hipStream_t>(stream)>>>(dst, src, n);
+  return launch_check("copy_u32");
 }
 int fkd_stream_create(fkd_stream_t* stream) {
   hipStream_t s;

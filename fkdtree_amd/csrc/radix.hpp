@@ -15,10 +15,11 @@ constexpr int kRadixTile = kThreads * kRadixItems;
 constexpr int kRadixPerWave = kRadixTile / (kThreads / kWave);
 
 struct TileMap {
-  const uint32_t* tile_ev;  // tile -> event
+  const uint32_t* tile_ev;  // tile -> event; null: one event of single_n keys
   const uint32_t* tb;       // event -> first tile (n_events + 1 entries)
   const uint32_t* off;      // event -> first point (n_events + 1 entries)
   uint32_t T;               // number of tiles
+  uint32_t single_n;
 };
 
 struct TileInfo {
@@ -28,6 +29,14 @@ struct TileInfo {
 template <int TILE>
 __device__ inline TileInfo tile_info(const TileMap& tm, uint32_t t) {
   TileInfo ti;
+  if (!tm.tile_ev) {
+    ti.e = 0;
+    ti.lt = t;
+    ti.nt = tm.T;
+    ti.base = t * (uint32_t)TILE;
+    ti.end = min(ti.base + (uint32_t)TILE, tm.single_n);
+    return ti;
+  }
   ti.e = tm.tile_ev[t];
   const uint32_t tb0 = tm.tb[ti.e];
   ti.lt = t - tb0;

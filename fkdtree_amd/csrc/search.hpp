@@ -9,6 +9,11 @@
 // every lane walks its query's visited set depth first without a stack: a
 // 32-bit mask remembers on which depths a right son is still pending, and the
 // heap index of that son follows from the current index by a shift.
+//
+// Queries are processed in spatial (Morton) order of their box centres so
+// that the lanes of a wave walk nearly the same nodes: loads coalesce, the
+// upper tree stays in L2, and lanes finish together.  Results stay in query
+// order (CSR offsets come from a scan of the per-query counts).
 #pragma once
 #include "build.hpp"
 #include "common.hpp"
@@ -20,20 +25,51 @@ struct SearchArgs {
   const uint32_t* off;    // event -> first node (n_events + 1), or null
   uint32_t n_single;      // node count when off == null
   const uint32_t* qev;    // query -> event, or null (event 0)
+  const uint32_t* perm;   // processing slot -> query (null: identity)
   uint32_t nq;
-  const float* mins;      // [DIM][nq]
+  const float* mins;      // [DIM][nq], indexed by processing slot
   const float* maxs;
-  uint32_t* counts;       // count pass output
+  uint32_t* counts;       // count pass output (query order)
   uint32_t* visited;      // stats pass: nodes visited per query
   const uint64_t* qoff;   // fill pass: first output slot of every query
   uint32_t* out;          // fill pass output
   uint64_t cap;           // capacity of out
+  uint32_t out_aligned;   // out is 64-byte aligned
+  uint32_t node_bytes;    // size of the node records (buffer resource range)
 };
 
+// depth % DIM for depth < 32, two bits per depth.
 template <int DIM>
-__device__ inline void load_rec(const float* nodes, uint32_t i, float (&c)[DIM], uint32_t& id) {
+__device__ inline uint32_t depth_dim(uint32_t depth) {
+  constexpr uint64_t tab = [] {
+    uint64_t t = 0;
+    for (int b = 0; b < 32; ++b) t |= (uint64_t)(b % DIM) << (2 * b);
+    return t;
+  }();
+  return (uint32_t)(tab >> (2u * depth)) & 3u;
+}
+
+// Node records are read through a buffer resource (32-bit byte offsets, reads
+// past num_records return zero) when the records fit in 4 GiB, else through a
+// flat pointer.
+struct NodeSource {
+  const float* flat;
+  __amdgpu_buffer_rsrc_t rsrc;
+};
+
+// Record i of the tree that starts at record `base` of the batch.  The
+// buffer resource spans the whole batch (uniform across the wave), the
+// per-lane tree offset goes into the 32-bit byte offset.
+template <int DIM, bool BUF>
+__device__ inline void load_rec(const NodeSource& src, uint32_t base, uint32_t i, float (&c)[DIM],
+                                uint32_t& id) {
   constexpr int RF = NodeRec<DIM>::kFloats;
-  const float4 v = *reinterpret_cast<const float4*>(nodes + (uint64_t)i * RF);
+  float4 v;
+  if (BUF)
+    v = __builtin_bit_cast(float4,
+                           __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, (base + i) * (uint32_t)(RF * 4), 0, 0));
+  else
+    v = *reinterpret_cast<const float4*>(src.flat + ((uint64_t)base + i) * RF);
   c[0] = v.x;
   if (DIM > 1) c[DIM > 1 ? 1 : 0] = v.y;
   if (DIM > 2) c[DIM > 2 ? 2 : 0] = v.z;
@@ -41,17 +77,27 @@ __device__ inline void load_rec(const float* nodes, uint32_t i, float (&c)[DIM],
     id = __float_as_uint(v.w);
   } else {
     c[DIM > 3 ? 3 : 0] = v.w;
-    id = __float_as_uint(*(nodes + (uint64_t)i * RF + 4));
+    if (BUF)
+      id = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, (base + i) * (uint32_t)(RF * 4) + 16u, 0, 0);
+    else
+      id = __float_as_uint(src.flat[((uint64_t)base + i) * RF + 4]);
   }
 }
 
 enum SearchMode { kCount = 0, kFill = 1, kStats = 2 };
 
-template <int DIM, int MODE>
+constexpr int kStage = 16;  // ids staged per lane: one 64-byte line
+
+// The loop body is branch free except for the emission of a hit in the fill
+// pass: the next node is selected among (left son, right son, pending right
+// son of the deepest pending depth) with conditional moves.
+template <int DIM, int MODE, bool BUF>
 __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   constexpr bool FILL = MODE == kFill;
-  const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
-  if (q >= a.nq) return;
+  __shared__ uint32_t stage[FILL ? kThreads : 1][kStage + 1];
+  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  if (slot >= a.nq) return;
+  const uint32_t q = a.perm ? a.perm[slot] : slot;
   uint32_t base = 0, n = a.n_single;
   if (a.off) {
     const uint32_t e = a.qev ? a.qev[q] : 0u;
@@ -61,58 +107,125 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   float mn[DIM], mx[DIM];
 #pragma unroll
   for (int m = 0; m < DIM; ++m) {
-    mn[m] = a.mins[(uint64_t)m * a.nq + q];
-    mx[m] = a.maxs[(uint64_t)m * a.nq + q];
+    mn[m] = a.mins[(uint64_t)m * a.nq + slot];
+    mx[m] = a.maxs[(uint64_t)m * a.nq + slot];
   }
-  const float* nodes = a.nodes + (uint64_t)base * NodeRec<DIM>::kFloats;
-  uint64_t w = FILL ? a.qoff[q] : 0;
+  NodeSource src;
+  src.flat = a.nodes;
+  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.nodes, (short)0, (int)a.node_bytes, 0x00020000);
+  uint64_t w = FILL ? a.qoff[q] : 0;  // next output slot not yet written
+  uint32_t nb = 0;                    // ids staged
   uint32_t c = 0;
+  uint32_t* st = stage[FILL ? threadIdx.x : 0];
+  auto flush = [&]() {
+    if (nb == kStage && a.out_aligned && (w & (kStage - 1)) == 0 && w + kStage <= a.cap) {
+      uint4* dst = reinterpret_cast<uint4*>(a.out + w);
+#pragma unroll
+      for (int i = 0; i < kStage / 4; ++i)
+        dst[i] = make_uint4(st[4 * i], st[4 * i + 1], st[4 * i + 2], st[4 * i + 3]);
+    } else {
+      for (uint32_t i = 0; i < nb; ++i)
+        if (w + i < a.cap) a.out[w + i] = st[i];
+    }
+    w += nb;
+    nb = 0;
+  };
   uint32_t node = 0, depth = 0, dd = 0, pending = 0, vis = 0;
-  while (n != 0) {
+  bool alive = n != 0;
+  while (alive) {
     if (MODE == kStats) ++vis;
     float x[DIM];
     uint32_t id;
-    load_rec<DIM>(nodes, node, x, id);
+    load_rec<DIM, BUF>(src, base, node, x, id);
     float v = x[0], lo = mn[0], hi = mx[0];
 #pragma unroll
     for (int m = 1; m < DIM; ++m) {
-      if (dd == (uint32_t)m) {
-        v = x[m];
-        lo = mn[m];
-        hi = mx[m];
-      }
+      const bool sel = dd == (uint32_t)m;
+      v = sel ? x[m] : v;
+      lo = sel ? mn[m] : lo;
+      hi = sel ? mx[m] : hi;
     }
-    const bool lower = v < lo;
-    const bool inter = v <= hi && v >= lo;
-    if (inter) {
-      bool in = true;
+    bool in = true;
 #pragma unroll
-      for (int m = 0; m < DIM; ++m) in = in && (x[m] <= mx[m] && x[m] >= mn[m]);
+    for (int m = 0; m < DIM; ++m) in = in & (x[m] <= mx[m]) & (x[m] >= mn[m]);
+    const bool lower = v < lo;
+    const bool inter = (v <= hi) & (v >= lo);
+    if (FILL) {
       if (in) {
-        if (FILL && w < a.cap) a.out[w] = id;
-        ++w;
-        ++c;
+        st[nb++] = id;
+        if (((w + nb) & (kStage - 1)) == 0) flush();
       }
+    } else {
+      c += in ? 1u : 0u;
     }
     const uint32_t lc = 2u * node + 1u;
-    const bool go_l = !lower && lc < n;
-    const bool go_r = (lower || inter) && lc + 1u < n;
-    if (go_l || go_r) {
-      if (go_l && go_r) pending |= 2u << depth;
-      node = go_l ? lc : lc + 1u;
-      ++depth;
-      dd = (dd + 1u == (uint32_t)DIM) ? 0u : dd + 1u;
-    } else {
-      if (pending == 0u) break;
-      const uint32_t b = 31u - (uint32_t)__builtin_clz(pending);
-      pending &= ~(1u << b);
-      node = (node + 1u) >> (depth - b);
-      depth = b;
-      dd = b % (uint32_t)DIM;
-    }
+    const bool go_l = (!lower) & (lc < n);
+    const bool go_r = (lower | inter) & (lc + 1u < n);
+    const bool desc = go_l | go_r;
+    pending |= (go_l & go_r) ? (2u << depth) : 0u;
+    const uint32_t b = 31u - (uint32_t)__builtin_clz(pending | 1u);
+    const uint32_t up = (node + 1u) >> (depth - b);
+    alive = desc | (pending != 0u);
+    node = go_l ? lc : (go_r ? lc + 1u : up);
+    pending = desc ? pending : (pending & ~(1u << b));
+    const uint32_t dn = (dd + 1u == (uint32_t)DIM) ? 0u : dd + 1u;
+    dd = desc ? dn : depth_dim<DIM>(b);
+    depth = desc ? depth + 1u : b;
   }
+  if (FILL && nb) flush();
   if (!FILL) a.counts[q] = c;
   if (MODE == kStats) a.visited[q] = vis;
+}
+
+// Spatial sort key of every query: Morton interleave of its box centre,
+// quantised to KB bits per dimension inside the tree's bounding box.
+// Any key order gives the same results; this one only groups similar work.
+template <int DIM, int KB>
+__global__ void query_keys_kernel(const float* __restrict__ mins, const float* __restrict__ maxs,
+                                  uint32_t nq, const uint32_t* __restrict__ bbox,
+                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  float lo[DIM], sc[DIM];
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+    const float a = fkey_inv(bbox[m]), b = fkey_inv(bbox[DIM + m]);
+    const float ext = b - a;
+    lo[m] = a;
+    sc[m] = (ext > 0.f && ext < 3.0e38f) ? (float)(1u << KB) / ext : 0.f;
+    if (!(lo[m] > -3.0e38f && lo[m] < 3.0e38f)) sc[m] = 0.f;
+  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+    uint32_t key = 0;
+    uint32_t u[DIM];
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      const float c = 0.5f * mins[(uint64_t)m * nq + i] + 0.5f * maxs[(uint64_t)m * nq + i];
+      float f = (c - lo[m]) * sc[m];
+      f = (f >= 0.f) ? f : 0.f;  // also maps NaN to 0
+      f = f < (float)((1u << KB) - 1u) ? f : (float)((1u << KB) - 1u);
+      u[m] = (uint32_t)f;
+    }
+#pragma unroll
+    for (int b = 0; b < KB; ++b)
+#pragma unroll
+      for (int m = 0; m < DIM; ++m) key |= ((u[m] >> b) & 1u) << (b * DIM + m);
+    keys[i] = key;
+    vals[i] = i;
+  }
+}
+
+// Boxes in processing order: smin[d][i] = mins[d][perm[i]].
+template <int DIM>
+__global__ void permute_boxes_kernel(const float* __restrict__ mins, const float* __restrict__ maxs,
+                                     const uint32_t* __restrict__ perm, uint32_t nq,
+                                     float* __restrict__ smins, float* __restrict__ smaxs) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+    const uint32_t q = perm[i];
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      smins[(uint64_t)m * nq + i] = mins[(uint64_t)m * nq + q];
+      smaxs[(uint64_t)m * nq + i] = maxs[(uint64_t)m * nq + q];
+    }
+  }
 }
 
 }  // namespace fkd

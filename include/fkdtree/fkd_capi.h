@@ -99,7 +99,8 @@ int fkd_search_count(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const 
 /* Fill pass: ids of the points inside box q go to
  * dev_ids_out[dev_offsets[q] .. dev_offsets[q+1]) (slots >= capacity are
  * dropped; check dev_offsets[nq] <= capacity).  Within a query the order is
- * unspecified.  Enqueues only. */
+ * unspecified.  Must follow fkd_search_count on the same tree with the same
+ * boxes (it reuses the processing order computed there).  Enqueues only. */
 int fkd_search_fill(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
                     const uint32_t* dev_query_events, const uint64_t* dev_offsets,
                     uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream);
@@ -126,6 +127,8 @@ int fkd_malloc(void** ptr, size_t bytes);
 int fkd_free(void* ptr);
 int fkd_memcpy_htod(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
 int fkd_memcpy_dtoh(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
+/* Streaming device copy of n uint32 (a plain kernel, for counter calibration). */
+int fkd_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, fkd_stream_t stream);
 int fkd_stream_create(fkd_stream_t* stream);
 int fkd_stream_destroy(fkd_stream_t stream);
 int fkd_stream_sync(fkd_stream_t stream);
