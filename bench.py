@@ -94,9 +94,16 @@ def main():
 
     # --- search: size the output once ------------------------------------
     d_off = DeviceArray(q + 1, np.uint64)
-    tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, None, 0, stream=stream, fill=False)
-    total = int(d_off.download(stream)[-1])
-    d_out = DeviceArray(max(total, 1), np.uint32)
+    d_out = DeviceArray(1, np.uint32)
+    total, ok = 0, False
+    for _ in range(3):
+        tree.search_single_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, d_out.nbytes // 4, stream=stream)
+        total, ok = tree.search_status(stream)
+        if ok:
+            break
+        if total > d_out.nbytes // 4:
+            d_out = DeviceArray(total, np.uint32)
+    assert ok, "search output incomplete after regrowing"
     d_cnt = DeviceArray(q, np.uint32)
     d_vis = DeviceArray(q, np.uint32)
     from fkdtree_amd._lib import check, lib
@@ -106,36 +113,41 @@ def main():
     assert int(hits.sum()) == total
 
     def step():
-        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
+        tree.search_single_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
 
     for _ in range(args.warmup):
         step()
     stream.sync()
+    # per-step HIP events around the traversal kernel, on its stream
+    ktimers = [Timer() for _ in range(args.steps)]
     t_wall = time.perf_counter()
     tm.start(stream)
-    for _ in range(args.steps):
+    for k in range(args.steps):
+        check(lib().fkd_trace_events(tree._h, ktimers[k].a, ktimers[k].b))
         step()
     tm.stop(stream)
     search_ms = tm.ms() / args.steps
     wall_ms = (time.perf_counter() - t_wall) * 1e3 / args.steps
+    check(lib().fkd_trace_events(tree._h, None, None))
+    fill_ms = sum(t.ms() for t in ktimers) / args.steps
+    total2, ok2 = tree.search_status(stream)
+    assert ok2 and total2 == total
 
-    # --- per-kernel timing of the two traversal kernels ------------------
+    # --- the two-pass form (count + fill) for comparison ----------------
+    tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, None, 0, stream=stream, fill=False)
     tm.start(stream)
     for _ in range(args.steps):
-        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, None, 0, stream=stream, fill=False)
+        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
     tm.stop(stream)
-    count_ms = tm.ms() / args.steps  # count kernel + scan
-    tm.start(stream)
-    for _ in range(args.steps):
-        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream, count=False)
-    tm.stop(stream)
-    fill_ms = tm.ms() / args.steps
+    two_pass_ms = tm.ms() / args.steps
+
 
     V = int(visited.sum())
     H = total
-    # Algorithmic bytes per query of the fill kernel: 24 B box + 8 B offset +
-    # 12 B per visited node (3 coordinates) + 8 B per hit (id read + write).
-    fill_bytes = 32 * q + 12 * V + 8 * H
+    # Algorithmic bytes per query of the traversal kernel: 24 B box + 4 B
+    # count + 12 B per visited node (3 coordinates) + 8 B per hit (id read +
+    # id write).
+    fill_bytes = 28 * q + 12 * V + 8 * H
     # Whole search (count + fill), the per-query model 28 + 12 V + 8 H.
     search_bytes = 28 * q + 12 * V + 8 * H
     # Build: per depth a 4 B key read plus a 16 B read and write per point.
@@ -162,13 +174,13 @@ def main():
         "build_ms": build_ms,
         "search_ms": search_ms,
         "search_wall_ms": wall_ms,
-        "count_scan_ms": count_ms,
-        "fill_ms": fill_ms,
+        "two_pass_search_ms": two_pass_ms,
+        "traversal_kernel_ms": fill_ms,
         "hits": H,
         "visited_nodes": V,
         "search_algorithmic_GBs": search_bytes / (search_ms * 1e-3) / 1e9,
         "build_algorithmic_GBs": build_bytes / (build_ms * 1e-3) / 1e9,
-        "roofline": {"bound": "hbm", "kernel": "search_kernel<3,kFill>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "search_kernel<3,kPool> (single-pass traversal)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": None},
         "device": info["name"],

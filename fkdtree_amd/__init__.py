@@ -229,6 +229,20 @@ class _TreeBase:
         if fill:
             check(lib().fkd_search_fill(self._h, nq, d_mins, d_maxs, d_qev, d_offsets, d_out, capacity, s))
 
+    def search_single_device(self, nq: int, d_mins: int, d_maxs: int, d_offsets: int, d_out: Optional[int],
+                             capacity: int, d_qev: Optional[int] = None, stream=None) -> None:
+        """Enqueue the single-pass search (fkd_search) on device buffers."""
+        s = _sh(stream if stream is not None else self.stream)
+        check(lib().fkd_search(self._h, nq, d_mins, d_maxs, d_qev, d_offsets, d_out, capacity, s))
+
+    def search_status(self, stream=None) -> Tuple[int, bool]:
+        """(total hits, complete) of the last single-pass search (synchronises)."""
+        tot = C.c_uint64()
+        ok = C.c_int()
+        s = _sh(stream if stream is not None else self.stream)
+        check(lib().fkd_search_status(self._h, s, C.byref(tot), C.byref(ok)))
+        return tot.value, bool(ok.value)
+
     def close(self):
         if self._h:
             lib().fkd_destroy(self._h)

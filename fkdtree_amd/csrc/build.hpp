@@ -95,7 +95,7 @@ __global__ void id_keys_kernel(const uint32_t* __restrict__ ids, uint32_t P, uin
   }
 }
 
-constexpr int kLevelItems = 8;
+constexpr int kLevelItems = 4;
 constexpr int kLevelTile = kThreads * kLevelItems;
 
 template <int DIM>
@@ -106,6 +106,7 @@ struct LevelArgs {
   const uint32_t* src[3];     // rank planes of the DIM-1 other arrays ...
   uint32_t* dst[3];           // ... and where their partitioned copies go
   const uint32_t* S0;         // rank in dimension 0 -> point index
+  uint32_t* F;                // in-order position -> rank in dimension 0 of its node
   TileMap tm;                 // tiles of kLevelTile points, event aligned
   const float* X;             // input points
   const uint32_t* ids;
@@ -205,22 +206,26 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   const uint64_t P = a.P;
   if (!split) {
     if (M && threadIdx.x == 0) publish(&a.desc[bid], kStPrefix | seg_pack(SegCount{1u, 0u, 0u}));
-    if (root_only && arr == 0 && threadIdx.x == 0) write_node(a, o, 0u, a.S0[a.Ad[o]]);
+    if (root_only && arr == 0 && threadIdx.x == 0) a.F[o] = a.Ad[o];
     return;
   }
   const uint32_t first = lt * (uint32_t)kLevelTile + threadIdx.x * kLevelItems;
-  const uint32_t lvl0 = (1u << a.L) - 1u;
   const bool last = (a.L + 1 == sh.D);
   const uint32_t* Ad_d = a.Ad + (uint64_t)a.d * P;  // plane d of array d
   const uint32_t* S = M ? a.src[arr] : nullptr;
 
-  uint32_t cls[kLevelItems], dest0[kLevelItems], rk[kLevelItems][DIM];
-  SegCount agg{0u, 0u, 0u};
+  // Phase 1: interval of every item and all loads (no stores in between so
+  // the loads of all items can be in flight together).
+  uint32_t cls[kLevelItems], dest0[kLevelItems], rk[kLevelItems][DIM], rm[kLevelItems], fr[kLevelItems];
+  uint32_t st0[kLevelItems], mp[kLevelItems];
 #pragma unroll
   for (int i = 0; i < kLevelItems; ++i) {
     const uint32_t jl = first + i;
     cls[i] = kNone;
-    dest0[i] = 0;
+    st0[i] = 0;
+    mp[i] = 0;
+    rm[i] = 0;
+    fr[i] = 0;
 #pragma unroll
     for (int k = 0; k < DIM; ++k) rk[i][k] = 0;
     if (jl < n) {
@@ -229,37 +234,45 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
       subtree_range(sh, a.L, ts, start, size);
       if (jl - start < size) {
         const uint32_t mpos = start + left_size(size);
-        if (arr == 0) {
-          if (jl == mpos) {
-            write_node(a, o, lvl0 + ts, a.S0[a.Ad[o + mpos]]);
-          } else if (last) {
-            write_node(a, o, 2u * (lvl0 + ts) + 1u + (jl > mpos ? 1u : 0u), a.S0[a.Ad[o + jl]]);
-          }
-        }
+        st0[i] = start;
+        mp[i] = mpos;
+        cls[i] = 8u;  // in an interval
         if (M) {
-          const uint32_t rm = Ad_d[o + mpos];
+          rm[i] = Ad_d[o + mpos];
 #pragma unroll
           for (int k = 0; k < DIM; ++k) rk[i][k] = S[(uint64_t)k * P + o + jl];
-          uint32_t r = rk[i][0];
-#pragma unroll
-          for (int k = 1; k < DIM; ++k) r = (a.d == (uint32_t)k) ? rk[i][k] : r;
-          if (jl == start) agg = SegCount{1u, 0u, 0u};
-          if (r < rm) {
-            cls[i] = kLeft;
-            dest0[i] = o + start;
-            agg.l++;
-          } else if (r > rm) {
-            cls[i] = kRight;
-            dest0[i] = o + mpos + 1u;
-            agg.r++;
-          } else {
-            cls[i] = kMed;
-            dest0[i] = o + mpos;
-          }
-          if (jl == start) cls[i] |= 4u;  // interval starts here
         }
+        if (arr == 0 && (jl == mpos || last)) fr[i] = a.Ad[o + jl];  // plane 0 of array d
       }
     }
+  }
+  // Phase 2: classification and the thread's segmented counts.
+  SegCount agg{0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < kLevelItems; ++i) {
+    const uint32_t jl = first + i;
+    if (!(cls[i] & 8u)) continue;
+    if (arr == 0 && (jl == mp[i] || last)) a.F[o + jl] = fr[i];  // node of depth L, or a leaf
+    if (!M) continue;
+    uint32_t r = rk[i][0];
+#pragma unroll
+    for (int k = 1; k < DIM; ++k) r = (a.d == (uint32_t)k) ? rk[i][k] : r;
+    uint32_t c;
+    if (jl == st0[i]) agg = SegCount{1u, 0u, 0u};
+    if (r < rm[i]) {
+      c = kLeft;
+      dest0[i] = o + st0[i];
+      agg.l++;
+    } else if (r > rm[i]) {
+      c = kRight;
+      dest0[i] = o + mp[i] + 1u;
+      agg.r++;
+    } else {
+      c = kMed;
+      dest0[i] = o + mp[i];
+    }
+    if (jl == st0[i]) c |= 4u;  // interval starts here
+    cls[i] = c;
   }
   if (!M) return;
 
@@ -290,20 +303,37 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   }
   excl = seg_op(wcarry, excl);
 
-  if (threadIdx.x == 0) {
+  if (wave == 0) {
+    // Windowed look-back: lane l polls the descriptor of tile lt-1-base-l of
+    // the same array; the nearest tile that has its prefix or an interval
+    // start ends the walk.  Values combine far-to-near with seg_op.
     SegCount carry{0u, 0u, 0u};
     if (lt == 0) {
-      publish(&a.desc[bid], kStPrefix | seg_pack(tile_agg));
+      if (lane == 0) publish(&a.desc[bid], kStPrefix | seg_pack(tile_agg));
     } else {
-      publish(&a.desc[bid], kStAggregate | seg_pack(tile_agg));
-      for (uint32_t j = bid - M;; j -= M) {
-        const uint64_t w = poll(&a.desc[j]);
-        carry = seg_op(seg_unpack(w), carry);
-        if ((w & kStMask) == kStPrefix || carry.f) break;
+      if (lane == 0) publish(&a.desc[bid], kStAggregate | seg_pack(tile_agg));
+      for (uint32_t base = 0;; base += kWave) {
+        const int64_t j = (int64_t)lt - 1 - (int64_t)base - (int64_t)lane;
+        const uint64_t w = j >= 0 ? poll(&a.desc[bid - (base + lane + 1u) * M]) : kStPrefix;
+        SegCount v = seg_unpack(w);
+        const bool stop = (w & kStMask) == kStPrefix || v.f;
+        const uint64_t sm = __ballot(stop);
+        const uint32_t first = sm ? (uint32_t)__builtin_ctzll(sm) : (uint32_t)kWave;
+        if (lane > first) v = SegCount{0u, 0u, 0u};
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          SegCount u;
+          u.f = __shfl_down(v.f, o);
+          u.l = __shfl_down(v.l, o);
+          u.r = __shfl_down(v.r, o);
+          if (lane + o < (uint32_t)kWave) v = seg_op(u, v);
+        }
+        carry = seg_op(v, carry);
+        if (sm) break;
       }
-      publish(&a.desc[bid], kStPrefix | seg_pack(seg_op(carry, tile_agg)));
+      if (lane == 0) publish(&a.desc[bid], kStPrefix | seg_pack(seg_op(carry, tile_agg)));
     }
-    s_carry = carry;
+    if (lane == 0) s_carry = carry;
   }
   __syncthreads();
   const SegCount run0 = seg_op(s_carry, excl);
@@ -311,7 +341,7 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   uint32_t* D = a.dst[arr];
 #pragma unroll
   for (int i = 0; i < kLevelItems; ++i) {
-    const uint32_t c = cls[i];
+    const uint32_t c = (cls[i] & 8u) ? 0u : cls[i];
     if (c & 4u) {
       nl = 0;
       nr = 0;
@@ -326,6 +356,28 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
 #pragma unroll
       for (int k = 0; k < DIM; ++k) D[(uint64_t)k * P + dest] = rk[i][k];
     }
+  }
+}
+
+// Tree arrays from the in-order node table: heap node v of every event sits
+// at in-order position start + left_size(size) of its subtree; its point is
+// S0[F[that position]].  One thread per node, writes coalesced.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a) {
+  const uint32_t t = blockIdx.x;
+  const uint32_t e = a.tm.tile_ev[t];
+  const uint32_t lt = t - a.tm.tb[e];
+  const uint32_t o = a.tm.off[e];
+  const uint32_t n = a.tm.off[e + 1] - o;
+  const Shape sh = shape_of(n);
+  for (uint32_t v = lt * (uint32_t)kLevelTile + threadIdx.x; v < min(n, (lt + 1) * (uint32_t)kLevelTile);
+       v += kThreads) {
+    const uint32_t L = floor_log2(v + 1u);
+    const uint32_t ts = v - ((1u << L) - 1u);
+    uint32_t start, size;
+    subtree_range(sh, L, ts, start, size);
+    const uint32_t j = start + left_size(size);
+    write_node(a, o, v, a.S0[a.F[o + j]]);
   }
 }
 

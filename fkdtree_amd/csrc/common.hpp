@@ -84,17 +84,19 @@ __device__ inline T wave_incl_sum(T v) {
   return v;
 }
 
-// Publication of decoupled look-back descriptors: status lives in the top two
-// bits of a 64-bit word written with one release store.
+// Publication of decoupled look-back descriptors: status and value share one
+// 64-bit word (status in the top two bits), so relaxed device-scope atomics
+// suffice -- no other data is published through them.  Release/acquire
+// fences would write back / invalidate the caches of an XCD on every tile.
 constexpr uint64_t kStInvalid = 0, kStAggregate = 1ull << 62, kStPrefix = 2ull << 62;
 constexpr uint64_t kStMask = 3ull << 62;
 
 __device__ inline void publish(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline uint64_t poll(uint64_t* p) {
   uint64_t v;
-  while (((v = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) & kStMask) ==
+  while (((v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & kStMask) ==
          kStInvalid)
     __builtin_amdgcn_s_sleep(1);
   return v;

@@ -134,10 +134,16 @@ struct fkd_tree_s {
   Tiling radix_tiles, level_tiles;
   DevBuf X, ids;               // input copy
   DevBuf Tc, Tid, nodes;       // outputs
-  DevBuf A, K, R, planes;      // build scratch
+  DevBuf A, K, R, planes, F;   // build scratch
   DevBuf counts, offsets, desc, ctr, err, bbox;
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev;  // search scratch
   DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets, q_desc;
+  DevBuf pool, pool_meta, pool_ctr;  // single-pass search chunk pool
+  DevBuf q_counts_slot, q_soff;
+  uint32_t pool_chunks = 0;
+  uint64_t last_capacity = 0;
+  uint64_t* last_offsets = nullptr;
+  hipEvent_t trace_start = nullptr, trace_stop = nullptr;  // around the traversal kernel
   uint32_t q_plan_nq = 0;  // queries whose processing order q_perm holds
   uint64_t last_nq = 0, last_total = 0;
 };
@@ -262,6 +268,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
       a.dst[k] = nullptr;
     }
     a.S0 = Abuf(0, 0);
+    a.F = t->F.as<uint32_t>();
     a.tm = tm;
     a.X = X;
     a.ids = t->ids.as<uint32_t>();
@@ -277,6 +284,11 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     fkd::level_kernel<DIM><<<blocks, fkd::kThreads, 0, s>>>(a);
     FKD_TRY(launch_check("level"));
     for (uint32_t i = 0; i < M; ++i) cur[moved[i]] ^= 1;
+    if (L + 1 == levels) {
+      // 6. Tree arrays from the in-order node table.
+      fkd::output_kernel<DIM><<<tm.T, fkd::kThreads, 0, s>>>(a);
+      FKD_TRY(launch_check("output"));
+    }
   }
   t->built = true;
   t->q_plan_nq = 0;
@@ -377,6 +389,70 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
   return launch_check("search_scan");
 }
 
+// pool_chunks counts the chunks of ONE region (the pool has kPoolRegions).
+int grow_pool(fkd_tree_s* t, uint64_t region_chunks) {
+  region_chunks = std::min<uint64_t>(region_chunks, (0xffffffffull / fkd::kStage) / fkd::kPoolRegions);
+  if (region_chunks <= t->pool_chunks && t->pool.p) return FKD_OK;
+  const uint64_t chunks = region_chunks * fkd::kPoolRegions;
+  FKD_TRY(t->pool.ensure((size_t)chunks * fkd::kStage * 4));
+  FKD_TRY(t->pool_meta.ensure((size_t)chunks * 8));
+  FKD_TRY(t->pool_ctr.ensure(fkd::kPoolRegions * 4));
+  t->pool_chunks = (uint32_t)region_chunks;
+  return FKD_OK;
+}
+
+// Single pass: traversal into the chunk pool, scan of the counts, compaction.
+template <int DIM>
+int search_single_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                       uint64_t* qoff, uint32_t* out, uint64_t cap, hipStream_t s) {
+  FKD_TRY(t->q_counts.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
+  FKD_TRY(t->q_counts_slot.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
+  FKD_TRY(t->q_soff.ensure((size_t)std::max<uint32_t>(nq, 1) * 8));
+  // region size: at least twice the boxes' share plus two reservations
+  const uint64_t want = std::max<uint64_t>(((uint64_t)nq * 2) / fkd::kPoolRegions + 2 * fkd::kPoolBatch,
+                                           t->pool_chunks);
+  FKD_TRY(grow_pool(t, want));
+  FKD_TRY(fill_u32(t->pool_ctr.as<uint32_t>(), 0, fkd::kPoolRegions, s));
+  if (nq > 0) {
+    FKD_TRY(plan_queries<DIM>(t, nq, mins, maxs, s));
+    fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
+    const size_t n = std::max<uint32_t>(nq, 1);
+    a.perm = t->q_perm.as<uint32_t>();
+    a.mins = t->q_sbox.as<float>();
+    a.maxs = a.mins + (uint64_t)DIM * n;
+    a.counts = t->q_counts.as<uint32_t>();
+    a.pool = t->pool.as<uint32_t>();
+    a.chunk_meta = t->pool_meta.as<uint2>();
+    a.pool_ctr = t->pool_ctr.as<uint32_t>();
+    a.region_chunks = t->pool_chunks;
+    a.counts_slot = t->q_counts_slot.as<uint32_t>();
+    if (t->trace_start) FKD_HIP(hipEventRecord(t->trace_start, s));
+    FKD_TRY((launch_search<DIM, fkd::kPool>(t, a, s, "search_pool")));
+    if (t->trace_stop) FKD_HIP(hipEventRecord(t->trace_stop, s));
+  }
+  const uint64_t st = fkd::scan_tiles(nq);
+  FKD_TRY(t->desc.ensure(st * 8));
+  FKD_TRY(fill_u64(t->desc.as<uint64_t>(), 0, st, s));
+  FKD_TRY(fill_u32(t->ctr.as<uint32_t>(), 0, 1, s));
+  fkd::scan_exclusive_kernel<uint64_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
+      t->q_counts.as<uint32_t>(), qoff, nq, t->desc.as<uint64_t>(), t->ctr.as<uint32_t>());
+  FKD_TRY(launch_check("search_scan"));
+  if (nq > 0) {
+    fkd::gather_offsets_kernel<<<grid_for(nq), 256, 0, s>>>(qoff, t->q_perm.as<uint32_t>(), nq,
+                                                             t->q_soff.as<uint64_t>());
+    FKD_TRY(launch_check("gather_offsets"));
+    fkd::compact_kernel<<<4096, fkd::kThreads, 0, s>>>(t->pool.as<uint32_t>(), t->pool_meta.as<uint2>(),
+                                                      t->pool_ctr.as<uint32_t>(), t->pool_chunks, nq,
+                                                      t->q_counts_slot.as<uint32_t>(), t->q_soff.as<uint64_t>(),
+                                                      out, cap);
+    FKD_TRY(launch_check("compact"));
+  }
+  t->last_nq = nq;
+  t->last_capacity = cap;
+  t->last_offsets = qoff;
+  return FKD_OK;
+}
+
 template <int DIM>
 int search_stats_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                       uint32_t* counts, uint32_t* visited, hipStream_t s) {
@@ -467,6 +543,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(t->K.ensure(S * dim * 2 * 4));
   FKD_TRY(t->R.ensure(S * dim * 4));
   FKD_TRY(t->planes.ensure(S * dim * dim * 2 * 4));
+  FKD_TRY(t->F.ensure(S * 4));
   const uint64_t nc = (uint64_t)dim * 256u * std::max<uint32_t>(t->radix_tiles.T, 1);
   FKD_TRY(t->counts.ensure(nc * 4));
   FKD_TRY(t->offsets.ensure((nc + 1) * 4));
@@ -664,6 +741,43 @@ int fkd_search_stats(fkd_tree_t t, uint32_t nq, const float* mins, const float* 
   });
 }
 
+int fkd_search(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+               uint64_t* dev_offsets, uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  if (!dev_offsets || (nq && (!mins || !maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
+  if (!qev && t->n_events > 1) return fail(FKD_ERR_INVALID, "query events required for a batch of trees");
+  if (capacity && !dev_ids_out) return fail(FKD_ERR_INVALID, "dev_ids_out is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dispatch_dim(t->dim, [&](auto D) {
+    return search_single_impl<decltype(D)::value>(t, nq, mins, maxs, qev, dev_offsets, dev_ids_out, capacity, s);
+  });
+}
+
+int fkd_trace_events(fkd_tree_t t, fkd_event_t start, fkd_event_t stop) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  t->trace_start = static_cast<hipEvent_t>(start);
+  t->trace_stop = static_cast<hipEvent_t>(stop);
+  return FKD_OK;
+}
+
+int fkd_search_status(fkd_tree_t t, fkd_stream_t stream, uint64_t* total, int* complete) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!t->last_offsets) return fail(FKD_ERR_STATE, "no fkd_search to report on");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint64_t tot = 0;
+  std::vector<uint32_t> used(fkd::kPoolRegions, 0);
+  FKD_HIP(hipMemcpyAsync(&tot, t->last_offsets + t->last_nq, 8, hipMemcpyDeviceToHost, s));
+  FKD_HIP(hipMemcpyAsync(used.data(), t->pool_ctr.p, fkd::kPoolRegions * 4, hipMemcpyDeviceToHost, s));
+  FKD_HIP(hipStreamSynchronize(s));
+  const uint32_t umax = t->last_nq ? *std::max_element(used.begin(), used.end()) : 0u;
+  const bool pool_ok = umax <= t->pool_chunks;
+  if (!pool_ok) FKD_TRY(grow_pool(t, (uint64_t)umax + umax / 4 + 2 * fkd::kPoolBatch));
+  if (total) *total = tot;
+  if (complete) *complete = (pool_ok && tot <= t->last_capacity) ? 1 : 0;
+  return FKD_OK;
+}
+
 int fkd_search_host(fkd_tree_t t, uint32_t nq, const float* host_mins, const float* host_maxs,
                     const uint32_t* host_qev, uint64_t* host_offsets, fkd_stream_t stream) {
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
@@ -685,13 +799,16 @@ int fkd_search_host(fkd_tree_t t, uint32_t nq, const float* host_mins, const flo
     }
   }
   FKD_TRY(t->q_off.ensure(((size_t)nq + 1) * 8));
-  FKD_TRY(fkd_search_count(t, nq, dmins, dmaxs, dqev, t->q_off.as<uint64_t>(), stream));
+  uint64_t total = 0;
+  int complete = 0;
+  for (int attempt = 0; attempt < 3 && !complete; ++attempt) {
+    const uint64_t cap = t->q_hits.bytes / 4;
+    FKD_TRY(fkd_search(t, nq, dmins, dmaxs, dqev, t->q_off.as<uint64_t>(), t->q_hits.as<uint32_t>(), cap, stream));
+    FKD_TRY(fkd_search_status(t, stream, &total, &complete));
+    if (!complete) FKD_TRY(t->q_hits.ensure(std::max<uint64_t>(total, 1) * 4));
+  }
+  if (!complete) return fail(FKD_ERR_CAPACITY, "search output did not fit after regrowing");
   FKD_HIP(hipMemcpyAsync(host_offsets, t->q_off.p, ((size_t)nq + 1) * 8, hipMemcpyDeviceToHost, s));
-  FKD_HIP(hipStreamSynchronize(s));
-  const uint64_t total = host_offsets[nq];
-  FKD_TRY(t->q_hits.ensure(std::max<uint64_t>(total, 1) * 4));
-  FKD_TRY(fkd_search_fill(t, nq, dmins, dmaxs, dqev, t->q_off.as<uint64_t>(), t->q_hits.as<uint32_t>(), total,
-                          stream));
   FKD_HIP(hipStreamSynchronize(s));
   t->last_nq = nq;
   t->last_total = total;

@@ -43,20 +43,29 @@ __global__ __launch_bounds__(kThreads) void scan_exclusive_kernel(const uint32_t
     if (w < wave) wave_off += s_wave[w];
     total += s_wave[w];
   }
-  if (threadIdx.x == 0) {
+  if (wave == 0) {
+    // Look back over a window of 64 predecessors per step: lane l polls tile
+    // tile-1-base-l; the sum stops at the nearest tile that has its prefix.
     uint64_t carry = 0;
     if (tile == 0) {
-      publish(&desc[0], kStPrefix | total);
+      if (lane_id() == 0) publish(&desc[0], kStPrefix | total);
     } else {
-      publish(&desc[tile], kStAggregate | total);
-      for (int64_t j = (int64_t)tile - 1;; --j) {
-        const uint64_t d = poll(&desc[j]);
-        carry += d & ~kStMask;
-        if ((d & kStMask) == kStPrefix) break;
+      if (lane_id() == 0) publish(&desc[tile], kStAggregate | total);
+      for (int64_t base = 0;; base += kWave) {
+        const int64_t j = (int64_t)tile - 1 - base - (int64_t)lane_id();
+        const uint64_t d = j >= 0 ? poll(&desc[j]) : kStPrefix;
+        const bool stop = (d & kStMask) == kStPrefix;
+        const uint64_t sm = __ballot(stop);
+        const uint32_t first = sm ? (uint32_t)__builtin_ctzll(sm) : (uint32_t)kWave;
+        uint64_t v = lane_id() <= first ? (d & ~kStMask) : 0ull;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        carry += v;
+        if (sm) break;
       }
-      publish(&desc[tile], kStPrefix | (carry + total));
+      if (lane_id() == 0) publish(&desc[tile], kStPrefix | (carry + total));
     }
-    s_carry = carry;
+    if (lane_id() == 0) s_carry = carry;
   }
   __syncthreads();
   uint64_t run = s_carry + wave_off + incl - sum;

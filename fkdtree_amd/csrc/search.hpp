@@ -30,13 +30,25 @@ struct SearchArgs {
   const float* mins;      // [DIM][nq], indexed by processing slot
   const float* maxs;
   uint32_t* counts;       // count pass output (query order)
+  uint32_t* counts_slot;  // pool mode: counts in processing order
   uint32_t* visited;      // stats pass: nodes visited per query
   const uint64_t* qoff;   // fill pass: first output slot of every query
   uint32_t* out;          // fill pass output
   uint64_t cap;           // capacity of out
   uint32_t out_aligned;   // out is 64-byte aligned
   uint32_t node_bytes;    // size of the node records (buffer resource range)
+  // single-pass mode: hits go to 16-id chunks of a pool split into
+  // kPoolRegions regions of region_chunks chunks; block b allocates in
+  // region b % kPoolRegions, a wave reserves kPoolBatch chunks at a time.
+  uint32_t* pool;         // kPoolRegions * region_chunks * 16 ids
+  uint2* chunk_meta;      // per chunk: (processing slot, index of the chunk in its query)
+  uint32_t* pool_ctr;     // per region: chunks reserved (may exceed region_chunks)
+  uint32_t region_chunks;
 };
+
+constexpr uint32_t kPoolRegions = 256;
+constexpr uint32_t kPoolBatch = 32;
+constexpr uint32_t kNoQuery = 0xffffffffu;
 
 // depth % DIM for depth < 32, two bits per depth.
 template <int DIM>
@@ -84,7 +96,7 @@ __device__ inline void load_rec(const NodeSource& src, uint32_t base, uint32_t i
   }
 }
 
-enum SearchMode { kCount = 0, kFill = 1, kStats = 2 };
+enum SearchMode { kCount = 0, kFill = 1, kStats = 2, kPool = 3 };
 
 constexpr int kStage = 16;  // ids staged per lane: one 64-byte line
 
@@ -94,7 +106,9 @@ constexpr int kStage = 16;  // ids staged per lane: one 64-byte line
 template <int DIM, int MODE, bool BUF>
 __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   constexpr bool FILL = MODE == kFill;
-  __shared__ uint32_t stage[FILL ? kThreads : 1][kStage + 1];
+  constexpr bool POOL = MODE == kPool;
+  constexpr bool STAGE = FILL || POOL;
+  __shared__ uint32_t stage[STAGE ? kThreads : 1][kStage + 1];
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
   if (slot >= a.nq) return;
   const uint32_t q = a.perm ? a.perm[slot] : slot;
@@ -116,7 +130,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   uint64_t w = FILL ? a.qoff[q] : 0;  // next output slot not yet written
   uint32_t nb = 0;                    // ids staged
   uint32_t c = 0;
-  uint32_t* st = stage[FILL ? threadIdx.x : 0];
+  uint32_t* st = stage[STAGE ? threadIdx.x : 0];
   auto flush = [&]() {
     if (nb == kStage && a.out_aligned && (w & (kStage - 1)) == 0 && w + kStage <= a.cap) {
       uint4* dst = reinterpret_cast<uint4*>(a.out + w);
@@ -129,6 +143,57 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     }
     w += nb;
     nb = 0;
+  };
+  uint32_t nchunk = 0;  // chunks of this query handed out so far (pool mode)
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x / kWave;
+  __shared__ uint32_t s_wbase[POOL ? kThreads / kWave : 1], s_wleft[POOL ? kThreads / kWave : 1];
+  const uint32_t region = blockIdx.x % kPoolRegions;
+  if (POOL && lane == 0) {
+    s_wbase[wid] = 0;
+    s_wleft[wid] = 0;
+  }
+  // Chunk allocation for the lanes with `need` set; called by every active
+  // lane of the wave.  The wave's current reservation lives in LDS so that
+  // lanes that already left the traversal loop see it too.
+  auto pool_flush = [&](bool need) {
+    const uint64_t m = __ballot(need);
+    if (m == 0ull) return;
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+    uint32_t wbase = s_wbase[wid], wleft = s_wleft[wid];
+    if (cnt > wleft) {
+      const uint32_t grab = cnt > kPoolBatch ? cnt : kPoolBatch;
+      uint32_t base = 0;
+      if (lane == leader) {
+        base = atomicAdd(&a.pool_ctr[region], grab);
+        // mark the unused rest of the old reservation
+        for (uint32_t i = 0; i < wleft; ++i) {
+          const uint32_t k = wbase + i;
+          if (k < a.region_chunks)
+            a.chunk_meta[(uint64_t)region * a.region_chunks + k] = make_uint2(kNoQuery, 0u);
+        }
+      }
+      wbase = __shfl(base, (int)leader);
+      wleft = grab;
+    }
+    if (lane == leader) {
+      s_wbase[wid] = wbase + cnt;
+      s_wleft[wid] = wleft - cnt;
+    }
+    if (need) {
+      const uint32_t k = wbase + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      if (k < a.region_chunks) {
+        const uint64_t ch = (uint64_t)region * a.region_chunks + k;
+        uint4* dst = reinterpret_cast<uint4*>(a.pool + ch * kStage);
+#pragma unroll
+        for (int i = 0; i < kStage / 4; ++i)
+          dst[i] = make_uint4(st[4 * i], st[4 * i + 1], st[4 * i + 2], st[4 * i + 3]);
+        a.chunk_meta[ch] = make_uint2(slot, nchunk);
+      }
+      ++nchunk;
+      nb = 0;
+    }
   };
   uint32_t node = 0, depth = 0, dd = 0, pending = 0, vis = 0;
   bool alive = n != 0;
@@ -155,6 +220,10 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
         st[nb++] = id;
         if (((w + nb) & (kStage - 1)) == 0) flush();
       }
+    } else if (POOL) {
+      if (in) st[nb++] = id;
+      c += in ? 1u : 0u;
+      pool_flush(nb == (uint32_t)kStage);
     } else {
       c += in ? 1u : 0u;
     }
@@ -173,8 +242,82 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     depth = desc ? depth + 1u : b;
   }
   if (FILL && nb) flush();
+  if (POOL) {
+    pool_flush(nb != 0u);
+    // mark the unused rest of this wave's reservation (by one lane: the
+    // wave may be partly empty)
+    const uint32_t first = (uint32_t)__builtin_ctzll(__ballot(true));
+    if (lane == first) {
+      const uint32_t wb = s_wbase[wid], wl = s_wleft[wid];
+      for (uint32_t i = 0; i < wl; ++i) {
+        const uint32_t k = wb + i;
+        if (k < a.region_chunks) a.chunk_meta[(uint64_t)region * a.region_chunks + k] = make_uint2(kNoQuery, 0u);
+      }
+    }
+  }
   if (!FILL) a.counts[q] = c;
+  if (POOL) a.counts_slot[slot] = c;
   if (MODE == kStats) a.visited[q] = vis;
+}
+
+// Copies the pooled chunks into the query-ordered CSR: slot t of chunk k
+// (processing slot s, k-th chunk j of its query) goes to
+// out[soff[s] + 16 j + t] when 16 j + t < counts[s]; soff / counts are in
+// processing order, so neighbouring chunks read neighbouring entries.  Region r's used chunks are its first
+// min(pool_ctr[r], region_chunks); reserved but unused ones carry kNoQuery.
+// Every block forms the prefix sum of the used chunks per region in LDS and
+// walks only used chunks, 16 threads per chunk.
+__global__ __launch_bounds__(kThreads) void compact_kernel(const uint32_t* __restrict__ pool,
+                                                           const uint2* __restrict__ meta,
+                                                           const uint32_t* __restrict__ pool_ctr,
+                                                           uint32_t region_chunks, uint32_t nq,
+                                                           const uint32_t* __restrict__ counts,
+                                                           const uint64_t* __restrict__ soff,
+                                                           uint32_t* __restrict__ out, uint64_t cap) {
+  static_assert(kPoolRegions == kThreads, "one thread per region");
+  __shared__ uint32_t pre[kPoolRegions + 1];
+  __shared__ uint32_t s_wave[kThreads / kWave];
+  const uint32_t used = min(pool_ctr[threadIdx.x], region_chunks);
+  const uint32_t incl = wave_incl_sum(used);
+  const int wave = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t woff = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / kWave; ++w) woff += (w < wave) ? s_wave[w] : 0u;
+  pre[threadIdx.x + 1] = woff + incl;
+  if (threadIdx.x == 0) pre[0] = 0;
+  __syncthreads();
+  const uint32_t chunks = pre[kPoolRegions];
+  for (uint32_t kc = blockIdx.x * kThreads + threadIdx.x; kc < chunks; kc += gridDim.x * kThreads) {
+    uint32_t lo = 0, hi = kPoolRegions;  // largest r with pre[r] <= kc
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pre[mid] <= kc) lo = mid; else hi = mid;
+    }
+    const uint64_t k = (uint64_t)lo * region_chunks + (kc - pre[lo]);
+    const uint2 mq = meta[k];
+    const uint4* src = reinterpret_cast<const uint4*>(pool + k * kStage);
+    uint4 v[kStage / 4];
+#pragma unroll
+    for (int i = 0; i < kStage / 4; ++i) v[i] = src[i];
+    if (mq.x >= nq) continue;  // kNoQuery: reserved but unused
+    const uint32_t pos = mq.y * kStage;
+    const uint32_t cnt = counts[mq.x];
+    const uint64_t dst = soff[mq.x] + pos;
+    const uint32_t m = cnt > pos ? min(cnt - pos, (uint32_t)kStage) : 0u;
+    const uint32_t* ids = reinterpret_cast<const uint32_t*>(v);
+#pragma unroll
+    for (int i = 0; i < kStage; ++i)
+      if ((uint32_t)i < m && dst + i < cap) out[dst + i] = ids[i];
+  }
+}
+
+// Offsets in processing order: soff[i] = qoff[perm[i]].
+__global__ void gather_offsets_kernel(const uint64_t* __restrict__ qoff, const uint32_t* __restrict__ perm,
+                                      uint32_t nq, uint64_t* __restrict__ soff) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
+    soff[i] = qoff[perm[i]];
 }
 
 // Spatial sort key of every query: Morton interleave of its box centre,

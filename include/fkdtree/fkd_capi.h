@@ -96,6 +96,22 @@ int fkd_search_count(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const 
                      const uint32_t* dev_query_events, uint64_t* dev_offsets,
                      fkd_stream_t stream);
 
+/* Single-pass search: ids of the points inside box q go to
+ * dev_ids_out[dev_offsets[q] .. dev_offsets[q+1]), dev_offsets[nq] = total.
+ * One traversal per box (hits are staged in a library-owned pool, then
+ * compacted into query order).  Enqueues only: fkd_search_status reports
+ * whether the results are complete (the internal pool and `capacity` were
+ * large enough); when not, it grows the pool and the same call must be
+ * repeated (with a capacity of at least the reported total).
+ * Replaces: search_in_the_box (FKDTree_cpu.h:123-169) for nq boxes at once. */
+int fkd_search(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
+               const uint32_t* dev_query_events, uint64_t* dev_offsets, uint32_t* dev_ids_out,
+               uint64_t capacity, fkd_stream_t stream);
+
+/* Synchronises `stream`; *total = hits of the last fkd_search on this tree,
+ * *complete = 1 when every id was written. */
+int fkd_search_status(fkd_tree_t tree, fkd_stream_t stream, uint64_t* total, int* complete);
+
 /* Fill pass: ids of the points inside box q go to
  * dev_ids_out[dev_offsets[q] .. dev_offsets[q+1]) (slots >= capacity are
  * dropped; check dev_offsets[nq] <= capacity).  Within a query the order is
@@ -137,6 +153,10 @@ int fkd_event_create(fkd_event_t* ev);
 int fkd_event_destroy(fkd_event_t ev);
 int fkd_event_record(fkd_event_t ev, fkd_stream_t stream);
 int fkd_event_elapsed_ms(fkd_event_t start, fkd_event_t stop, float* ms);
+
+/* Record `start` / `stop` (NULL: none) on the stream right before and after
+ * the traversal kernel of every following fkd_search on this tree. */
+int fkd_trace_events(fkd_tree_t tree, fkd_event_t start, fkd_event_t stop);
 
 /* ---- host-side helpers (no GPU needed) --------------------------------- */
 

@@ -78,3 +78,51 @@ def test_events_search(require_gpu):
     ooff, oids = oracle.search_events(tc, ti, b.offsets, qev, boxes.mins, boxes.maxs)
     assert np.array_equal(goff, ooff)
     assert np.array_equal(sorted_sets(goff, gids), sorted_sets(ooff, oids))
+
+
+def _device_search(t, boxes, single: bool, capacity=None):
+    from fkdtree_amd import DeviceArray
+    q = boxes.q
+    dmi = DeviceArray.from_host(boxes.mins)
+    dma = DeviceArray.from_host(boxes.maxs)
+    off = DeviceArray(q + 1, np.uint64)
+    if single:
+        out = DeviceArray(max(capacity or 1, 1), np.uint32)
+        t.search_single_device(q, dmi.ptr, dma.ptr, off.ptr, out.ptr, capacity or 0)
+        total, ok = t.search_status()
+        return off, out, total, ok, (dmi, dma)
+    t.search_device(q, dmi.ptr, dma.ptr, off.ptr, None, 0, fill=False)
+    total = int(off.download()[-1])
+    out = DeviceArray(max(total, 1), np.uint32)
+    t.search_device(q, dmi.ptr, dma.ptr, off.ptr, out.ptr, total, count=False)
+    return off, out, total, True, (dmi, dma)
+
+
+def test_single_pass_regrows_and_matches_two_pass(require_gpu):
+    """The single-pass search reports incomplete output (capacity 0, then an
+    internal pool too small for ~100 hits per box), grows, and then matches
+    the two-pass search and the oracle exactly."""
+    p = adversarial_points("mixed", 60_000)
+    boxes = self_boxes(p, 0.03)
+    print(f"[search] single-pass regrow n={p.n} q={boxes.q}", flush=True)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    t.build()
+    off2, out2, total2, _, _ = _device_search(t, boxes, single=False)
+    _, _, total, ok, _ = _device_search(t, boxes, single=True, capacity=0)
+    assert total == total2 and not ok
+    attempts = 0
+    while True:
+        off1, out1, total, ok, keep = _device_search(t, boxes, single=True, capacity=total)
+        attempts += 1
+        if ok or attempts > 3:
+            break
+    assert ok and total == total2
+    o1, o2 = off1.download(), off2.download()
+    assert np.array_equal(o1, o2)
+    i1 = out1.download(count=total)
+    i2 = out2.download(count=total)
+    assert np.array_equal(sorted_sets(o1, i1), sorted_sets(o2, i2))
+    tc, ti = oracle.build(p.coords, p.ids)
+    ooff, oids = oracle.search(tc, ti, boxes.mins, boxes.maxs)
+    assert np.array_equal(o1, ooff)
+    assert np.array_equal(sorted_sets(o1, i1), sorted_sets(ooff, oids))
