@@ -6,7 +6,9 @@ Workload (default ``mixed_1m``): N = 1,048,576 points in 3-D, half uniform in
 grid; 1,048,576 closed boxes of half-width 0.01 centred on random points.
 
 A timed step is one box search of all queries against the resident tree:
-count pass + exclusive scan + fill pass, writing every hit id (CSR output).
+spatial ordering of the boxes, count pass, exclusive scan, fill pass, writing
+every hit id (CSR output).  The single-pass form (one traversal into a chunk
+pool + compaction) is timed beside it.
 ``value`` is box queries per second over the timed steps.  Builds are timed
 in their own loop (``builds_per_s``).  Inputs and outputs stay in HBM; timing
 uses HIP events on the stream the kernels run on.
@@ -38,6 +40,9 @@ def parse():
     ap.add_argument("--workload", default="mixed_1m", choices=["mixed_1m", "uniform_100k", "uniform_4m"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--events", type=int, default=256,
+                    help="events of the batched workload (first N of the 2,048-event set; 0 skips)")
+    ap.add_argument("--event-steps", type=int, default=3)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 256 MiB copy kernel (counter calibration) and exit")
     return ap.parse_args()
@@ -112,35 +117,35 @@ def main():
     hits = d_cnt.download(stream).astype(np.int64)
     assert int(hits.sum()) == total
 
-    def step():
+    def single():
         tree.search_single_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
 
-    for _ in range(args.warmup):
-        step()
-    stream.sync()
-    # per-step HIP events around the traversal kernel, on its stream
-    ktimers = [Timer() for _ in range(args.steps)]
-    t_wall = time.perf_counter()
-    tm.start(stream)
-    for k in range(args.steps):
-        check(lib().fkd_trace_events(tree._h, ktimers[k].a, ktimers[k].b))
-        step()
-    tm.stop(stream)
-    search_ms = tm.ms() / args.steps
-    wall_ms = (time.perf_counter() - t_wall) * 1e3 / args.steps
-    check(lib().fkd_trace_events(tree._h, None, None))
-    fill_ms = sum(t.ms() for t in ktimers) / args.steps
+    def two_pass():
+        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
+
+    def timed(fn, traced):
+        for _ in range(args.warmup):
+            fn()
+        stream.sync()
+        ktimers = [Timer() for _ in range(args.steps)]
+        t_wall = time.perf_counter()
+        tm.start(stream)
+        for k in range(args.steps):
+            if traced:
+                check(lib().fkd_trace_events(tree._h, ktimers[k].a, ktimers[k].b))
+            fn()
+        tm.stop(stream)
+        step_ms = tm.ms() / args.steps
+        wall = (time.perf_counter() - t_wall) * 1e3 / args.steps
+        check(lib().fkd_trace_events(tree._h, None, None))
+        kms = sum(t.ms() for t in ktimers) / args.steps if traced else None
+        return step_ms, wall, kms
+
+    single_ms, _, pool_kernel_ms = timed(single, True)
     total2, ok2 = tree.search_status(stream)
     assert ok2 and total2 == total
-
-    # --- the two-pass form (count + fill) for comparison ----------------
-    tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, None, 0, stream=stream, fill=False)
-    tm.start(stream)
-    for _ in range(args.steps):
-        tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
-    tm.stop(stream)
-    two_pass_ms = tm.ms() / args.steps
-
+    search_ms, wall_ms, fill_ms = timed(two_pass, True)
+    two_pass_ms = search_ms
 
     V = int(visited.sum())
     H = total
@@ -174,22 +179,139 @@ def main():
         "build_ms": build_ms,
         "search_ms": search_ms,
         "search_wall_ms": wall_ms,
-        "two_pass_search_ms": two_pass_ms,
-        "traversal_kernel_ms": fill_ms,
+        "search_mode": "two-pass (count + scan + fill)",
+        "single_pass_search_ms": single_ms,
+        "fill_kernel_ms": fill_ms,
+        "single_pass_traversal_kernel_ms": pool_kernel_ms,
         "hits": H,
         "visited_nodes": V,
         "search_algorithmic_GBs": search_bytes / (search_ms * 1e-3) / 1e9,
         "build_algorithmic_GBs": build_bytes / (build_ms * 1e-3) / 1e9,
-        "roofline": {"bound": "hbm", "kernel": "search_kernel<3,kPool> (single-pass traversal)", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "search_kernel<3,kFill> (fill traversal)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": None},
         "device": info["name"],
     }
+    tree.close()
+    del d_out
+    if args.events:
+        res["events"] = bench_events(args.events, args.event_steps)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads)
     print(json.dumps(res), flush=True)
-    tree.close()
     return 0
+
+
+def bench_events(n_events: int, steps: int) -> dict:
+    """Events per second of build + self-search (every point of an event is
+    the centre of one box of half-width 0.01 searched in its own tree), for
+    the first n_events events of the 2,048-event workload (sizes log-uniform
+    in [1,000, 200,000], the mixed distribution).  Three forms:
+      batched - all events in shared launches (FKDEvents),
+      graph   - the batched pipeline captured in one HIP graph,
+      plain   - one tree per event, built and searched one after another."""
+    import fkdtree_amd as F
+    from fkdtree_amd import DeviceArray, Stream, Timer, gen
+    from fkdtree_amd._lib import check, lib
+
+    seed = gen.workload_seed("events_2048")
+    sizes = gen.event_sizes(2048, seed, 1000, gen.LOG2_200)[:n_events]
+    evs = [gen.mixed_event(int(sz), seed, e) for e, sz in enumerate(sizes)]
+    batch = gen.concat_events(evs)
+    boxes = gen.event_self_boxes(batch, 0.01)
+    P = batch.points.n
+    qev = np.repeat(np.arange(n_events, dtype=np.uint32), np.diff(batch.offsets))
+    s = Stream()
+    d_c = DeviceArray.from_host(batch.points.coords, s)
+    d_i = DeviceArray.from_host(batch.points.ids, s)
+    d_mi = DeviceArray.from_host(boxes.mins, s)
+    d_ma = DeviceArray.from_host(boxes.maxs, s)
+    d_qev = DeviceArray.from_host(qev, s)
+    ev = F.FKDEvents.from_device(3, batch.offsets, d_c.ptr, d_i.ptr, s)
+    d_off = DeviceArray(P + 1, np.uint64)
+    cap = 1
+    d_out = DeviceArray(cap, np.uint32)
+    for _ in range(4):
+        ev.build_async()
+        ev.search_single_device(P, d_mi.ptr, d_ma.ptr, d_off.ptr, d_out.ptr, cap, d_qev=d_qev.ptr, stream=s)
+        total, ok = ev.search_status(s)
+        if ok:
+            break
+        if total > cap:
+            cap = total
+            d_out = DeviceArray(cap, np.uint32)
+    assert ok
+    tm = Timer()
+
+    def batched():
+        ev.build_async()
+        ev.search_single_device(P, d_mi.ptr, d_ma.ptr, d_off.ptr, d_out.ptr, cap, d_qev=d_qev.ptr, stream=s)
+
+    batched()
+    s.sync()
+    tm.start(s)
+    for _ in range(steps):
+        batched()
+    tm.stop(s)
+    batched_ms = tm.ms() / steps
+    g = F.Graph(ev, P, d_mi.ptr, d_ma.ptr, d_off.ptr, d_out.ptr, cap, s, d_qev=d_qev.ptr)
+    g.launch()
+    s.sync()
+    tm.start(s)
+    for _ in range(steps):
+        g.launch()
+    tm.stop(s)
+    graph_ms = tm.ms() / steps
+    tot_g, ok_g = ev.search_status(s)
+    assert ok_g and tot_g == total
+    g.close()
+    ev.close()
+    del d_out
+
+    # plain: one tree per event, launch after launch
+    trees, bufs = [], []
+    for e in range(n_events):
+        a, b = int(batch.offsets[e]), int(batch.offsets[e + 1])
+        n = b - a
+        pe = batch.event(e)
+        t = F.FKDTree(coords=pe.coords, ids=pe.ids, stream=s)
+        bx = gen.boxes_around(pe.coords, 0.01)
+        dmi, dma = DeviceArray.from_host(bx.mins, s), DeviceArray.from_host(bx.maxs, s)
+        doff = DeviceArray(n + 1, np.uint64)
+        t.build()
+        t.search_single_device(n, dmi.ptr, dma.ptr, doff.ptr, None, 0, stream=s)
+        tot_e, _ = t.search_status(s)
+        dout = DeviceArray(max(tot_e, 1), np.uint32)
+        for _ in range(3):
+            t.search_single_device(n, dmi.ptr, dma.ptr, doff.ptr, dout.ptr, tot_e, stream=s)
+            tot_e, ok_e = t.search_status(s)
+            if ok_e:
+                break
+        trees.append(t)
+        bufs.append((n, dmi, dma, doff, dout, tot_e))
+
+    def plain():
+        for t, (n, dmi, dma, doff, dout, tot_e) in zip(trees, bufs):
+            t.build_async()
+            t.search_single_device(n, dmi.ptr, dma.ptr, doff.ptr, dout.ptr, tot_e, stream=s)
+
+    plain()
+    s.sync()
+    tm.start(s)
+    for _ in range(max(1, steps // 2)):
+        plain()
+    tm.stop(s)
+    plain_ms = tm.ms() / max(1, steps // 2)
+    plain_hits = sum(b[5] for b in bufs)
+    assert plain_hits == total
+    for t in trees:
+        t.close()
+    return {"n_events": n_events, "points": P, "queries": P, "hits": total,
+            "batched_ms": batched_ms, "graph_ms": graph_ms, "plain_ms": plain_ms,
+            "batched_events_per_s": n_events / (batched_ms * 1e-3),
+            "graph_events_per_s": n_events / (graph_ms * 1e-3),
+            "plain_events_per_s": n_events / (plain_ms * 1e-3),
+            "batched_over_plain": plain_ms / batched_ms}
 
 
 def cpu_baseline(pts, boxes, threads):

@@ -26,7 +26,7 @@ import numpy as np
 from . import gen  # noqa: F401
 from ._lib import FKDError, check, lib
 
-__all__ = ["FKDTree", "FKDEvents", "DeviceArray", "Stream", "Timer", "FKDError", "device_info"]
+__all__ = ["FKDTree", "FKDEvents", "DeviceArray", "Stream", "Timer", "Graph", "FKDError", "device_info"]
 
 
 def device_info() -> dict:
@@ -321,3 +321,31 @@ class FKDEvents(_TreeBase):
     def search(self, mins, maxs, query_events) -> Tuple[np.ndarray, np.ndarray]:
         """Query q searches the tree of event query_events[q]: (offsets, ids)."""
         return self._search_host(mins, maxs, query_events)
+
+
+class Graph:
+    """Build + single-pass search of fixed device boxes captured as one HIP
+    graph (fkd_graph_create); launch() replays the whole pipeline."""
+
+    def __init__(self, tree: _TreeBase, nq: int, d_mins: int, d_maxs: int, d_offsets: int, d_out: int,
+                 capacity: int, stream: "Stream", d_qev: Optional[int] = None):
+        h = C.c_void_p()
+        check(lib().fkd_graph_create(tree._h, nq, d_mins, d_maxs, d_qev, d_offsets, d_out, capacity,
+                                     _sh(stream), C.byref(h)))
+        self.h = h.value
+        self.tree = tree
+        self.stream = stream
+
+    def launch(self, stream=None) -> None:
+        check(lib().fkd_graph_launch(self.h, _sh(stream if stream is not None else self.stream)))
+
+    def close(self):
+        if self.h:
+            lib().fkd_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

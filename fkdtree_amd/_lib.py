@@ -33,6 +33,9 @@ _SIGS = {
     "fkd_search": (I32, [P, U32, P, P, P, P, P, U64, P]),
     "fkd_search_status": (I32, [P, P, C.POINTER(U64), C.POINTER(I32)]),
     "fkd_trace_events": (I32, [P, P, P]),
+    "fkd_graph_create": (I32, [P, U32, P, P, P, P, P, U64, P, C.POINTER(P)]),
+    "fkd_graph_launch": (I32, [P, P]),
+    "fkd_graph_destroy": (None, [P]),
     "fkd_search_host": (I32, [P, U32, P, P, P, P, P]),
     "fkd_fetch_results": (I32, [P, P, P]),
     "fkd_device_info": (I32, [C.c_char_p, SZ, C.POINTER(I32), C.POINTER(SZ), C.POINTER(SZ)]),

@@ -148,6 +148,15 @@ struct fkd_tree_s {
   uint64_t last_nq = 0, last_total = 0;
 };
 
+struct fkd_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~fkd_graph_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 namespace {
 
 // One stable LSD digit pass over M key arrays (count, scan, scatter).
@@ -480,7 +489,10 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
   a.out = out;
   a.cap = cap;
   a.out_aligned = (reinterpret_cast<uintptr_t>(out) & 63u) == 0;
-  return launch_search<DIM, fkd::kFill>(t, a, s, "search_fill");
+  if (t->trace_start) FKD_HIP(hipEventRecord(t->trace_start, s));
+  FKD_TRY((launch_search<DIM, fkd::kFill>(t, a, s, "search_fill")));
+  if (t->trace_stop) FKD_HIP(hipEventRecord(t->trace_stop, s));
+  return FKD_OK;
 }
 
 __global__ void check_input_kernel(const float* X, const uint32_t* ids, const uint32_t* off,
@@ -777,6 +789,58 @@ int fkd_search_status(fkd_tree_t t, fkd_stream_t stream, uint64_t* total, int* c
   if (complete) *complete = (pool_ok && tot <= t->last_capacity) ? 1 : 0;
   return FKD_OK;
 }
+
+int fkd_graph_create(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                     uint64_t* dev_offsets, uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream,
+                     fkd_graph_t* out) {
+  if (!t || !out) return fail(FKD_ERR_INVALID, "tree / out is NULL");
+  *out = nullptr;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!s) return fail(FKD_ERR_INVALID, "graph capture needs a created stream, not the null stream");
+  hipEvent_t ts = t->trace_start, te = t->trace_stop;
+  t->trace_start = t->trace_stop = nullptr;
+  // Eager runs until every buffer has its final size and the results are
+  // complete; then the same sequence is captured.
+  int complete = 0;
+  uint64_t total = 0;
+  for (int attempt = 0; attempt < 3 && !complete; ++attempt) {
+    FKD_TRY(fkd_build(t, stream));
+    FKD_TRY(fkd_search(t, nq, mins, maxs, qev, dev_offsets, dev_ids_out, capacity, stream));
+    FKD_TRY(fkd_search_status(t, stream, &total, &complete));
+    if (!complete && total > capacity) {
+      t->trace_start = ts;
+      t->trace_stop = te;
+      return fail(FKD_ERR_CAPACITY, "capacity " + std::to_string(capacity) + " < " + std::to_string(total) + " hits");
+    }
+  }
+  if (!complete) return fail(FKD_ERR_CAPACITY, "search pool did not settle");
+  fkd_graph_s* g = new fkd_graph_s();
+  int rc = FKD_OK;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = fkd_build(t, stream);
+    if (rc == FKD_OK) rc = fkd_search(t, nq, mins, maxs, qev, dev_offsets, dev_ids_out, capacity, stream);
+    hipError_t e2 = hipStreamEndCapture(s, &g->graph);
+    if (rc == FKD_OK && e2 != hipSuccess) e = e2;
+  }
+  if (rc == FKD_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  t->trace_start = ts;
+  t->trace_stop = te;
+  if (rc != FKD_OK || e != hipSuccess) {
+    delete g;
+    return rc != FKD_OK ? rc : fail(FKD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return FKD_OK;
+}
+
+int fkd_graph_launch(fkd_graph_t g, fkd_stream_t stream) {
+  if (!g) return fail(FKD_ERR_INVALID, "graph is NULL");
+  FKD_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  return FKD_OK;
+}
+
+void fkd_graph_destroy(fkd_graph_t g) { delete g; }
 
 int fkd_search_host(fkd_tree_t t, uint32_t nq, const float* host_mins, const float* host_maxs,
                     const uint32_t* host_qev, uint64_t* host_offsets, fkd_stream_t stream) {

@@ -1,0 +1,143 @@
+// FKDTree_HIP<float, dim>: the MI355X backend of the FKDTree interface
+// (FKDTree.h), a header-only C++ wrapper over the C ABI of fkd_capi.h
+// (link with fkdtree_amd/libfkdtree_hip.so).  It takes the place of the
+// reference's FKDTree_CPU / FKDTree_OpenCL classes (FKDTree_cpu.h:16-327,
+// FKDTree_opencl.h:17-290) with the same constructor and virtual methods.
+//
+// Differences from the reference, by design:
+//  * float only (static_assert); ids must be unique; NaN coordinates are
+//    rejected at construction (std::runtime_error), as are HIP errors;
+//  * search_in_the_box_multiple returns per-box counts, the only result the
+//    reference implements (FKDTree_opencl.h:123-164);
+//    search_in_the_box_multiple_csr returns every box's ids;
+//  * the order of ids within one result is unspecified (the set is exact);
+//  * all calls on one object are serialised by a mutex, so concurrent
+//    search_in_the_box calls from many host threads (main.cpp:351-359) are
+//    safe; the host copies behind getDimensionVector / getIdVector are made
+//    once per build.
+#ifndef FKDTREE_AMD_FKDTREE_HIP_H_
+#define FKDTREE_AMD_FKDTREE_HIP_H_
+
+#include <cstdint>
+#include <mutex>
+#include <type_traits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "FKDPoint.h"
+#include "FKDTree.h"
+#include "fkd_capi.h"
+
+template <class TYPE, unsigned int numberOfDimensions>
+class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
+  static_assert(std::is_same<TYPE, float>::value, "FKDTree_HIP supports float coordinates only");
+  static_assert(numberOfDimensions >= 1 && numberOfDimensions <= 4, "dimension must be 1..4");
+  typedef FKDPoint<TYPE, numberOfDimensions> Point;
+  static_assert(sizeof(Point) == 4 * (numberOfDimensions + 1), "unexpected FKDPoint layout");
+
+ public:
+  explicit FKDTree_HIP(const std::vector<Point>& points, fkd_stream_t stream = nullptr)
+      : FKDTree<TYPE, numberOfDimensions>(points), stream_(stream) {
+    check(fkd_create(numberOfDimensions, static_cast<uint32_t>(points.size()),
+                     points.empty() ? nullptr : points.data(), stream_, &tree_));
+    n_ = points.size();
+  }
+
+  ~FKDTree_HIP() override {
+    if (tree_) fkd_destroy(tree_);
+  }
+
+  FKDTree_HIP(const FKDTree_HIP&) = delete;
+  FKDTree_HIP& operator=(const FKDTree_HIP&) = delete;
+
+  void build() override {
+    std::lock_guard<std::mutex> g(mu_);
+    check(fkd_build(tree_, stream_));
+    check(fkd_stream_sync(stream_));
+    built_ = true;
+    mirror_ = false;
+  }
+
+  std::vector<unsigned int> search_in_the_box(const Point& minPoint, const Point& maxPoint) override {
+    std::vector<Point> lo(1, minPoint), hi(1, maxPoint);
+    std::vector<uint64_t> off;
+    std::vector<unsigned int> ids;
+    search_csr(lo, hi, off, ids);
+    return ids;
+  }
+
+  std::vector<unsigned int> search_in_the_box_multiple(const std::vector<Point>& minPoints,
+                                                       const std::vector<Point>& maxPoints) override {
+    std::vector<uint64_t> off;
+    std::vector<unsigned int> ids;
+    search_csr(minPoints, maxPoints, off, ids);
+    std::vector<unsigned int> counts(minPoints.size());
+    for (size_t q = 0; q < counts.size(); ++q) counts[q] = static_cast<unsigned int>(off[q + 1] - off[q]);
+    return counts;
+  }
+
+  // Ids of box q are ids[offsets[q] .. offsets[q+1]).
+  void search_in_the_box_multiple_csr(const std::vector<Point>& minPoints, const std::vector<Point>& maxPoints,
+                                      std::vector<uint64_t>& offsets, std::vector<unsigned int>& ids) {
+    search_csr(minPoints, maxPoints, offsets, ids);
+  }
+
+  std::vector<TYPE> const& getDimensionVector(const int dimension) override {
+    mirror();
+    return dims_[(dimension >= 0 && static_cast<unsigned>(dimension) < numberOfDimensions) ? dimension : 0];
+  }
+
+  std::vector<unsigned int> const& getIdVector() override {
+    mirror();
+    return ids_;
+  }
+
+  fkd_tree_t handle() const { return tree_; }
+
+ private:
+  static void check(int rc) {
+    if (rc != FKD_OK) throw std::runtime_error(std::string("fkdtree: ") + fkd_last_error());
+  }
+
+  void search_csr(const std::vector<Point>& minPoints, const std::vector<Point>& maxPoints,
+                  std::vector<uint64_t>& offsets, std::vector<unsigned int>& ids) {
+    if (minPoints.size() != maxPoints.size()) throw std::invalid_argument("fkdtree: box count mismatch");
+    const size_t nq = minPoints.size();
+    std::vector<float> lo(nq * numberOfDimensions), hi(nq * numberOfDimensions);
+    for (size_t q = 0; q < nq; ++q)
+      for (unsigned d = 0; d < numberOfDimensions; ++d) {
+        lo[d * nq + q] = minPoints[q][d];
+        hi[d * nq + q] = maxPoints[q][d];
+      }
+    offsets.assign(nq + 1, 0);
+    std::lock_guard<std::mutex> g(mu_);
+    if (!built_) throw std::logic_error("fkdtree: search before build()");
+    check(fkd_search_host(tree_, static_cast<uint32_t>(nq), lo.data(), hi.data(), nullptr, offsets.data(),
+                          stream_));
+    ids.resize(offsets[nq]);
+    check(fkd_fetch_results(tree_, ids.empty() ? nullptr : ids.data(), stream_));
+  }
+
+  void mirror() {
+    std::lock_guard<std::mutex> g(mu_);
+    if (mirror_) return;
+    std::vector<float> c(n_ * numberOfDimensions);
+    ids_.assign(n_, 0);
+    if (built_) check(fkd_get_tree(tree_, c.data(), ids_.data(), stream_));
+    for (unsigned d = 0; d < numberOfDimensions; ++d)
+      dims_[d].assign(c.begin() + d * n_, c.begin() + (d + 1) * n_);
+    mirror_ = true;
+  }
+
+  fkd_tree_t tree_ = nullptr;
+  fkd_stream_t stream_ = nullptr;
+  size_t n_ = 0;
+  bool built_ = false;
+  bool mirror_ = false;
+  std::mutex mu_;
+  std::vector<TYPE> dims_[numberOfDimensions];
+  std::vector<unsigned int> ids_;
+};
+
+#endif  // FKDTREE_AMD_FKDTREE_HIP_H_

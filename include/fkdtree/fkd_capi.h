@@ -38,6 +38,7 @@ extern "C" {
 #define FKD_ERR_CAPACITY (-5) /* output buffer too small */
 
 typedef struct fkd_tree_s* fkd_tree_t;
+typedef struct fkd_graph_s* fkd_graph_t;
 typedef void* fkd_stream_t; /* hipStream_t */
 
 const char* fkd_last_error(void);
@@ -112,6 +113,19 @@ int fkd_search(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float*
  * *complete = 1 when every id was written. */
 int fkd_search_status(fkd_tree_t tree, fkd_stream_t stream, uint64_t* total, int* complete);
 
+/* Build + single-pass search of the same boxes captured as one HIP graph.
+ * Runs the sequence eagerly first (so every internal buffer has its final
+ * size; FKD_ERR_CAPACITY when `capacity` is below the hits), then captures
+ * it on `stream` (a created stream).  fkd_graph_launch replays it: a full
+ * rebuild of every tree of the handle and the search.  The boxes, outputs and
+ * the tree must stay alive while the graph exists.  No reference
+ * counterpart (the reference launches kernel by kernel). */
+int fkd_graph_create(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
+                     const uint32_t* dev_query_events, uint64_t* dev_offsets, uint32_t* dev_ids_out,
+                     uint64_t capacity, fkd_stream_t stream, fkd_graph_t* out);
+int fkd_graph_launch(fkd_graph_t graph, fkd_stream_t stream);
+void fkd_graph_destroy(fkd_graph_t graph);
+
 /* Fill pass: ids of the points inside box q go to
  * dev_ids_out[dev_offsets[q] .. dev_offsets[q+1]) (slots >= capacity are
  * dropped; check dev_offsets[nq] <= capacity).  Within a query the order is
@@ -155,7 +169,8 @@ int fkd_event_record(fkd_event_t ev, fkd_stream_t stream);
 int fkd_event_elapsed_ms(fkd_event_t start, fkd_event_t stop, float* ms);
 
 /* Record `start` / `stop` (NULL: none) on the stream right before and after
- * the traversal kernel of every following fkd_search on this tree. */
+ * the traversal kernel of every following fkd_search / fkd_search_fill on
+ * this tree. */
 int fkd_trace_events(fkd_tree_t tree, fkd_event_t start, fkd_event_t stop);
 
 /* ---- host-side helpers (no GPU needed) --------------------------------- */

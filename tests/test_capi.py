@@ -1,0 +1,55 @@
+"""CPU checks of the C ABI: the library loads, exports every function that
+include/fkdtree/fkd_capi.h declares, and its host-only entry points behave."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from fkdtree_amd import _lib
+
+HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "fkdtree", "fkd_capi.h")
+
+
+def declared():
+    txt = open(HDR).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(fkd_\w+)\(", txt, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "fkd_create" in names and "fkd_search" in names and "fkd_build" in names
+    assert len(names) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert set(declared()) <= set(_lib.EXPORTS)
+
+
+def test_error_paths_without_gpu():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.fkd_create(7, 0, None, None, C.byref(h)) == -1
+    assert b"dim" in L.fkd_last_error()
+    assert L.fkd_build(None, None) == -1
+    offs = (C.c_uint64 * 3)(0, 5, 3)
+    assert L.fkd_create_events(3, 2, offs, None, None, C.byref(h)) == -1
+    assert b"monotone" in L.fkd_last_error()
+    big = (C.c_uint64 * 2)(0, (1 << 30) + 1)
+    assert L.fkd_create_device(3, 1, big, None, None, None, C.byref(h)) == -1
+    assert b"2^30" in L.fkd_last_error()
+    t, st, sz = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    assert L.fkd_shape_locate(10, 5, 0, C.byref(t)) == -1
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(RuntimeError, match="missing"):
+        _lib.lib()
