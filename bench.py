@@ -159,6 +159,7 @@ def main():
     D = int(np.floor(np.log2(max(n, 1))))
     build_bytes = 36 * n * D
     achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload, "fkd::search_kernel<3, 1, true>")
 
     res = {
         "metric": "box queries/s + builds/s at 1M 3D pts; batched vs plain events/s; p99 ms",
@@ -189,7 +190,7 @@ def main():
         "build_algorithmic_GBs": build_bytes / (build_ms * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": "search_kernel<3,kFill> (fill traversal)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None},
+                     "traffic": traffic, "algorithmic_bytes": fill_bytes},
         "device": info["name"],
     }
     tree.close()
@@ -200,6 +201,19 @@ def main():
         res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads)
     print(json.dumps(res), flush=True)
     return 0
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, calibrated on
+    a copy kernel) from the committed rocprofv3 summary of this workload, or
+    None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    k = json.load(open(path)).get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return k.get("fetch_bytes", 0.0) + k.get("write_bytes", 0.0)
 
 
 def bench_events(n_events: int, steps: int) -> dict:
