@@ -26,7 +26,8 @@ import numpy as np
 from . import gen  # noqa: F401
 from ._lib import FKDError, check, lib
 
-__all__ = ["FKDTree", "FKDEvents", "DeviceArray", "Stream", "Timer", "Graph", "FKDError", "device_info"]
+__all__ = ["FKDTree", "FKDEvents", "DeviceArray", "Stream", "Timer", "Event", "Graph", "FKDError",
+           "device_info"]
 
 
 def device_info() -> dict:
@@ -84,6 +85,30 @@ class Timer:
         try:
             lib().fkd_event_destroy(self.a)
             lib().fkd_event_destroy(self.b)
+        except Exception:
+            pass
+
+
+class Event:
+    """A HIP event: record on a stream, poll without blocking."""
+
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib().fkd_event_create(C.byref(e)))
+        self.h = e.value
+
+    def record(self, stream=None):
+        check(lib().fkd_event_record(self.h, _sh(stream)))
+
+    def done(self) -> bool:
+        r = lib().fkd_event_query(self.h)
+        if r < 0:
+            check(r)
+        return r == 1
+
+    def __del__(self):
+        try:
+            lib().fkd_event_destroy(self.h)
         except Exception:
             pass
 
@@ -242,6 +267,15 @@ class _TreeBase:
         s = _sh(stream if stream is not None else self.stream)
         check(lib().fkd_search_status(self._h, s, C.byref(tot), C.byref(ok)))
         return tot.value, bool(ok.value)
+
+    def trim(self) -> None:
+        """Release build / search scratch; the tree stays resident."""
+        check(lib().fkd_trim(self._h, _sh(self.stream)))
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        check(lib().fkd_device_bytes(self._h, C.byref(b)))
+        return b.value
 
     def close(self):
         if self._h:

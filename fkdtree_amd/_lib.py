@@ -27,6 +27,8 @@ _SIGS = {
     "fkd_get_tree": (I32, [P, P, P, P]),
     "fkd_tree_device": (I32, [P, C.POINTER(P), C.POINTER(P)]),
     "fkd_tree_info": (I32, [P, C.POINTER(I32), C.POINTER(U32), C.POINTER(U64)]),
+    "fkd_trim": (I32, [P, P]),
+    "fkd_device_bytes": (I32, [P, C.POINTER(U64)]),
     "fkd_search_count": (I32, [P, U32, P, P, P, P, P]),
     "fkd_search_fill": (I32, [P, U32, P, P, P, P, P, U64, P]),
     "fkd_search_stats": (I32, [P, U32, P, P, P, P, P, P]),
@@ -51,6 +53,7 @@ _SIGS = {
     "fkd_event_destroy": (I32, [P]),
     "fkd_event_record": (I32, [P, P]),
     "fkd_event_elapsed_ms": (I32, [P, P, C.POINTER(C.c_float)]),
+    "fkd_event_query": (I32, [P]),
     "fkd_shape_subtree": (I32, [U32, U32, U32, C.POINTER(U32), C.POINTER(U32)]),
     "fkd_shape_locate": (I32, [U32, U32, U32, C.POINTER(U32)]),
 }

@@ -61,6 +61,11 @@ struct DevBuf {
   T* as() const {
     return static_cast<T*>(p);
   }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
 };
 
 int launch_check(const char* what) {
@@ -159,6 +164,8 @@ struct fkd_graph_s {
 
 namespace {
 
+int ensure_build_scratch(fkd_tree_s* t);
+
 // One stable LSD digit pass over M key arrays (count, scan, scatter).
 int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts, uint32_t* offsets,
                uint64_t* desc, uint32_t* ctr, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
@@ -204,6 +211,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     t->built = true;
     return FKD_OK;
   }
+  FKD_TRY(ensure_build_scratch(t));
   const uint64_t S = P;  // stride of one array
   uint32_t* A = t->A.as<uint32_t>();  // A[m][2][P]
   uint32_t* K = t->K.as<uint32_t>();  // K[m][2][P]
@@ -532,6 +540,23 @@ int validate_offsets(uint32_t n_events, const uint64_t* event_offsets, uint64_t*
 }
 
 // Allocates everything, given the input already in t->X / t->ids; validates.
+// Scratch of the build (released by fkd_trim, re-created on the next build).
+int ensure_build_scratch(fkd_tree_s* t) {
+  const size_t S = std::max<uint32_t>(t->P, 1);
+  const int dim = t->dim;
+  FKD_TRY(t->A.ensure(S * dim * 2 * 4));
+  FKD_TRY(t->K.ensure(S * dim * 2 * 4));
+  FKD_TRY(t->R.ensure(S * dim * 4));
+  FKD_TRY(t->planes.ensure(S * dim * dim * 2 * 4));
+  FKD_TRY(t->F.ensure(S * 4));
+  const uint64_t nc = (uint64_t)dim * 256u * std::max<uint32_t>(t->radix_tiles.T, 1);
+  FKD_TRY(t->counts.ensure(nc * 4));
+  FKD_TRY(t->offsets.ensure((nc + 1) * 4));
+  const uint64_t ndesc = std::max<uint64_t>(fkd::scan_tiles(nc), (uint64_t)dim * t->level_tiles.T + 1);
+  FKD_TRY(t->desc.ensure(ndesc * 8));
+  return FKD_OK;
+}
+
 int finish_create(fkd_tree_s* t, hipStream_t s) {
   const uint32_t P = t->P, E = t->n_events;
   const int dim = t->dim;
@@ -551,16 +576,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(t->Tc.ensure(S * dim * 4));
   FKD_TRY(t->Tid.ensure(S * 4));
   FKD_TRY(t->nodes.ensure(S * rf * 4));
-  FKD_TRY(t->A.ensure(S * dim * 2 * 4));
-  FKD_TRY(t->K.ensure(S * dim * 2 * 4));
-  FKD_TRY(t->R.ensure(S * dim * 4));
-  FKD_TRY(t->planes.ensure(S * dim * dim * 2 * 4));
-  FKD_TRY(t->F.ensure(S * 4));
-  const uint64_t nc = (uint64_t)dim * 256u * std::max<uint32_t>(t->radix_tiles.T, 1);
-  FKD_TRY(t->counts.ensure(nc * 4));
-  FKD_TRY(t->offsets.ensure((nc + 1) * 4));
-  const uint64_t ndesc = std::max<uint64_t>(fkd::scan_tiles(nc), (uint64_t)dim * t->level_tiles.T + 1);
-  FKD_TRY(t->desc.ensure(ndesc * 8));
+  FKD_TRY(ensure_build_scratch(t));
   FKD_TRY(t->ctr.ensure(64));
   FKD_TRY(t->err.ensure(64));
   FKD_TRY(t->bbox.ensure(64));
@@ -842,6 +858,36 @@ int fkd_graph_launch(fkd_graph_t g, fkd_stream_t stream) {
 
 void fkd_graph_destroy(fkd_graph_t g) { delete g; }
 
+int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
+                    &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
+                    &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta, &t->q_counts_slot, &t->q_soff})
+    b->release();
+  t->desc.release();
+  FKD_TRY(t->desc.ensure(256));
+  t->pool_chunks = 0;
+  t->q_plan_nq = 0;
+  t->last_offsets = nullptr;
+  t->last_total = 0;
+  return FKD_OK;
+}
+
+int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
+  if (!t || !bytes) return fail(FKD_ERR_INVALID, "tree / bytes is NULL");
+  uint64_t b = 0;
+  for (const DevBuf* d : {&t->off, &t->radix_tiles.tile_ev, &t->radix_tiles.tb, &t->level_tiles.tile_ev,
+                          &t->level_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->A, &t->K, &t->R,
+                          &t->planes, &t->F, &t->counts, &t->offsets, &t->desc, &t->ctr, &t->err, &t->bbox,
+                          &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
+                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta,
+                          &t->pool_ctr, &t->q_counts_slot, &t->q_soff})
+    b += d->bytes;
+  *bytes = b;
+  return FKD_OK;
+}
+
 int fkd_search_host(fkd_tree_t t, uint32_t nq, const float* host_mins, const float* host_maxs,
                     const uint32_t* host_qev, uint64_t* host_offsets, fkd_stream_t stream) {
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
@@ -957,6 +1003,12 @@ int fkd_event_destroy(fkd_event_t ev) {
 int fkd_event_record(fkd_event_t ev, fkd_stream_t stream) {
   FKD_HIP(hipEventRecord(static_cast<hipEvent_t>(ev), static_cast<hipStream_t>(stream)));
   return FKD_OK;
+}
+int fkd_event_query(fkd_event_t ev) {
+  const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(ev));
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(FKD_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
 }
 int fkd_event_elapsed_ms(fkd_event_t a, fkd_event_t b, float* ms) {
   FKD_HIP(hipEventSynchronize(static_cast<hipEvent_t>(b)));

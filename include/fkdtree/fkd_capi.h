@@ -85,6 +85,15 @@ int fkd_tree_device(fkd_tree_t tree, const float** dev_coords, const uint32_t** 
 /* Shape queries. */
 int fkd_tree_info(fkd_tree_t tree, int* dim, uint32_t* n_events, uint64_t* n_points);
 
+/* Release the build and search scratch of a built tree (synchronises
+ * `stream`): the tree arrays and input points stay, about 48 B per point at
+ * dim 3.  Later builds and searches re-create what they need.  For keeping
+ * many trees resident.  No reference counterpart. */
+int fkd_trim(fkd_tree_t tree, fkd_stream_t stream);
+
+/* Device memory held by the handle, in bytes. */
+int fkd_device_bytes(fkd_tree_t tree, uint64_t* bytes);
+
 /* ---- search ---------------------------------------------------------- */
 
 /* Count pass + scan: dev_offsets[q] = first output slot of query q,
@@ -167,6 +176,8 @@ int fkd_event_create(fkd_event_t* ev);
 int fkd_event_destroy(fkd_event_t ev);
 int fkd_event_record(fkd_event_t ev, fkd_stream_t stream);
 int fkd_event_elapsed_ms(fkd_event_t start, fkd_event_t stop, float* ms);
+/* 1 when all work before the event has completed, 0 when not yet, <0 on error. */
+int fkd_event_query(fkd_event_t ev);
 
 /* Record `start` / `stop` (NULL: none) on the stream right before and after
  * the traversal kernel of every following fkd_search / fkd_search_fill on

@@ -1,0 +1,69 @@
+"""Continuous-batching server: every event's result is the same alone and in
+the mix (exact id sets), batches overlap on the device, HBM admission and
+eviction work, and resident trees answer later queries."""
+import numpy as np
+import pytest
+
+import oracle
+from fkdtree_amd import FKDTree, gen
+from fkdtree_amd.serving import EventServer, poisson_requests
+from tests.helpers import sorted_sets
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(req, hw=0.01):
+    t = FKDTree(coords=req.coords, ids=req.ids)
+    t.build()
+    b = gen.boxes_around(req.coords, hw)
+    off, ids = t.search_in_the_box_multiple_csr(b.mins, b.maxs)
+    t.close()
+    return off, ids
+
+
+def test_mix_equals_alone(require_gpu):
+    reqs = poisson_requests(60, rate_hz=20_000.0, size_scale=0.02)
+    print(f"[serve] events=60 points={sum(r.n for r in reqs)}", flush=True)
+    srv = EventServer(n_streams=3, max_batch_points=200_000, max_batch_events=16, keep_results=True)
+    rep = srv.run(reqs)
+    s = rep.summary()
+    print(s, flush=True)
+    assert len(rep.results) == 60
+    assert rep.n_batches >= 4 and max(rep.batch_sizes) > 1
+    for r in reqs:
+        got = rep.results[r.event]
+        off, ids = alone(r)
+        assert np.array_equal(got.offsets, off), r.event
+        assert np.array_equal(sorted_sets(got.offsets, got.ids), sorted_sets(off, ids)), r.event
+    for r in reqs[:5]:
+        tc, ti = oracle.build(r.coords, r.ids)
+        b = gen.boxes_around(r.coords, 0.01)
+        ooff, oids = oracle.search(tc, ti, b.mins, b.maxs)
+        got = rep.results[r.event]
+        assert np.array_equal(sorted_sets(got.offsets, got.ids), sorted_sets(ooff, oids))
+    # resident trees answer new queries
+    r = reqs[7]
+    b = gen.boxes_around(r.coords[:, :50], 0.05)
+    qoff, qids = srv.query(r.event, b.mins, b.maxs)
+    t = FKDTree(coords=r.coords, ids=r.ids)
+    t.build()
+    aoff, aids = t.search_in_the_box_multiple_csr(b.mins, b.maxs)
+    assert np.array_equal(sorted_sets(qoff, qids), sorted_sets(aoff, aids))
+    srv.close()
+
+
+def test_tiny_budget_forces_single_event_batches_and_eviction(require_gpu):
+    reqs = poisson_requests(12, rate_hz=50_000.0, size_scale=0.01)
+    biggest = max(r.n for r in reqs)
+    from fkdtree_amd.serving import batch_bytes
+    budget = batch_bytes(biggest, 64) + (1 << 20)
+    srv = EventServer(n_streams=2, hbm_budget=budget, hits_per_point=64, keep_results=True)
+    rep = srv.run(reqs)
+    assert len(rep.results) == 12
+    assert max(rep.batch_sizes) <= 2
+    assert rep.evictions > 0
+    for r in reqs[:4]:
+        off, ids = alone(r)
+        got = rep.results[r.event]
+        assert np.array_equal(sorted_sets(got.offsets, got.ids), sorted_sets(off, ids))
+    srv.close()
