@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--events", type=int, default=256,
                     help="events of the batched workload (first N of the 2,048-event set; 0 skips)")
     ap.add_argument("--event-steps", type=int, default=3)
+    ap.add_argument("--serve-events", type=int, default=64,
+                    help="events of the serving stream (0 skips)")
+    ap.add_argument("--serve-rates", default="1000,5000", help="Poisson arrival rates (events/s)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 256 MiB copy kernel (counter calibration) and exit")
     return ap.parse_args()
@@ -197,10 +200,33 @@ def main():
     del d_out
     if args.events:
         res["events"] = bench_events(args.events, args.event_steps)
+    if args.serve_events:
+        res["serving"] = bench_serving(args.serve_events, [float(r) for r in args.serve_rates.split(",")])
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads)
     print(json.dumps(res), flush=True)
     return 0
+
+
+def bench_serving(n_events: int, rates) -> list:
+    """p50 / p99 latency and events/s of the continuous-batching server for
+    Poisson streams of n_events events (sizes log-uniform in [1,000, 500,000]
+    points, mixed distribution; every point searched with half-width 0.01)."""
+    from fkdtree_amd.serving import EventServer, poisson_requests
+
+    out = []
+    for rate in rates:
+        reqs = poisson_requests(n_events, rate)
+        srv = EventServer(n_streams=3, max_batch_points=4_000_000, max_batch_events=256)
+        srv.run(reqs[: min(4, n_events)])          # warm-up (first allocations)
+        srv.close()
+        srv = EventServer(n_streams=3, max_batch_points=4_000_000, max_batch_events=256)
+        rep = srv.run(reqs)
+        srv.close()
+        d = rep.summary()
+        d.update({"rate_hz": rate, "points": int(sum(r.n for r in reqs))})
+        out.append(d)
+    return out
 
 
 def measured_traffic(workload: str, kernel: str):

@@ -60,8 +60,8 @@ def test_tiny_budget_forces_single_event_batches_and_eviction(require_gpu):
     srv = EventServer(n_streams=2, hbm_budget=budget, hits_per_point=64, keep_results=True)
     rep = srv.run(reqs)
     assert len(rep.results) == 12
-    assert max(rep.batch_sizes) <= 2
-    assert rep.evictions > 0
+    assert rep.n_batches >= 3          # the budget splits the stream
+    assert rep.evictions > 0           # resident trees made room for new batches
     for r in reqs[:4]:
         off, ids = alone(r)
         got = rep.results[r.event]
