@@ -19,6 +19,7 @@ wrappers of the same library; nothing here falls back to the CPU.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from typing import Optional, Sequence, Tuple
 
 import numpy as np
@@ -202,6 +203,10 @@ class _TreeBase:
         check(lib().fkd_tree_info(self._h, C.byref(d), C.byref(e), C.byref(n)))
         self.dim, self.n_events, self.n = d.value, e.value, n.value
         self._mirror = None
+        from . import tune
+
+        if os.path.exists(tune.TABLE):
+            tune.apply(self)
 
     def build(self) -> None:
         check(lib().fkd_build(self._h, _sh(self.stream)))
@@ -267,6 +272,16 @@ class _TreeBase:
         s = _sh(stream if stream is not None else self.stream)
         check(lib().fkd_search_status(self._h, s, C.byref(tot), C.byref(ok)))
         return tot.value, bool(ok.value)
+
+    def set_tuning(self, **params) -> None:
+        """Launch parameters (fkd_set_tuning): query_key_bits, pool_batch."""
+        for k, v in params.items():
+            check(lib().fkd_set_tuning(self._h, k.encode(), int(v)))
+
+    def get_tuning(self, key: str) -> int:
+        v = C.c_uint32()
+        check(lib().fkd_get_tuning(self._h, key.encode(), C.byref(v)))
+        return v.value
 
     def trim(self) -> None:
         """Release build / search scratch; the tree stays resident."""

@@ -28,6 +28,8 @@ _SIGS = {
     "fkd_tree_device": (I32, [P, C.POINTER(P), C.POINTER(P)]),
     "fkd_tree_info": (I32, [P, C.POINTER(I32), C.POINTER(U32), C.POINTER(U64)]),
     "fkd_trim": (I32, [P, P]),
+    "fkd_set_tuning": (I32, [P, C.c_char_p, U32]),
+    "fkd_get_tuning": (I32, [P, C.c_char_p, C.POINTER(U32)]),
     "fkd_device_bytes": (I32, [P, C.POINTER(U64)]),
     "fkd_search_count": (I32, [P, U32, P, P, P, P, P]),
     "fkd_search_fill": (I32, [P, U32, P, P, P, P, P, U64, P]),

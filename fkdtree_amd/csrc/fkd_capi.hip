@@ -127,7 +127,13 @@ struct Tiling {
 
 }  // namespace
 
+struct fkd_tuning {
+  uint32_t pool_batch = 32;     // single-pass search: chunks a wave reserves per atomic
+  uint32_t query_key_bits = 8;  // search: Morton bits per dimension of the box order
+};
+
 struct fkd_tree_s {
+  fkd_tuning tune;
   int dim = 3;
   uint32_t n_events = 0;
   uint32_t P = 0;
@@ -312,13 +318,12 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   return FKD_OK;
 }
 
-constexpr int kQueryKeyBits = 8;  // Morton bits per dimension of the query order
 
 template <int DIM, int MODE>
 int launch_search(fkd_tree_s* t, const fkd::SearchArgs& a, hipStream_t s, const char* what) {
-  const uint32_t blocks = (a.nq + fkd::kThreads - 1) / fkd::kThreads;
-  const uint64_t bytes = (uint64_t)t->P * fkd::NodeRec<DIM>::kFloats * 4;
   fkd::SearchArgs b = a;
+  const uint32_t blocks = std::max<uint32_t>(1, (a.nq + fkd::kThreads - 1) / fkd::kThreads);
+  const uint64_t bytes = (uint64_t)t->P * fkd::NodeRec<DIM>::kFloats * 4;
   b.node_bytes = (uint32_t)std::min<uint64_t>(bytes, 0xffffffffull);
   if (bytes < (1ull << 32))
     fkd::search_kernel<DIM, MODE, true><<<blocks, fkd::kThreads, 0, s>>>(b);
@@ -354,13 +359,14 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   uint32_t* keys = t->q_keys.as<uint32_t>();
   uint32_t* perm = t->q_perm.as<uint32_t>();
   const uint32_t g = grid_for(nq);
-  fkd::query_keys_kernel<DIM, kQueryKeyBits><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), keys, perm);
+  const uint32_t kb = t->tune.query_key_bits;
+  fkd::query_keys_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), kb, keys, perm);
   FKD_TRY(launch_check("query_keys"));
   fkd::TileMap tm{};
   tm.tile_ev = nullptr;
   tm.T = T;
   tm.single_n = nq;
-  constexpr int passes = (kQueryKeyBits * DIM + 7) / 8;
+  const int passes = (int)(kb * DIM + 7) / 8;
   int c = 0;
   for (int pass = 0; pass < passes; ++pass) {
     const bool lastp = pass == passes - 1;
@@ -426,7 +432,7 @@ int search_single_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const floa
   FKD_TRY(t->q_counts_slot.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
   FKD_TRY(t->q_soff.ensure((size_t)std::max<uint32_t>(nq, 1) * 8));
   // region size: at least twice the boxes' share plus two reservations
-  const uint64_t want = std::max<uint64_t>(((uint64_t)nq * 2) / fkd::kPoolRegions + 2 * fkd::kPoolBatch,
+  const uint64_t want = std::max<uint64_t>(((uint64_t)nq * 2) / fkd::kPoolRegions + 2 * t->tune.pool_batch,
                                            t->pool_chunks);
   FKD_TRY(grow_pool(t, want));
   FKD_TRY(fill_u32(t->pool_ctr.as<uint32_t>(), 0, fkd::kPoolRegions, s));
@@ -442,6 +448,7 @@ int search_single_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const floa
     a.chunk_meta = t->pool_meta.as<uint2>();
     a.pool_ctr = t->pool_ctr.as<uint32_t>();
     a.region_chunks = t->pool_chunks;
+    a.pool_batch = t->tune.pool_batch;
     a.counts_slot = t->q_counts_slot.as<uint32_t>();
     if (t->trace_start) FKD_HIP(hipEventRecord(t->trace_start, s));
     FKD_TRY((launch_search<DIM, fkd::kPool>(t, a, s, "search_pool")));
@@ -800,7 +807,7 @@ int fkd_search_status(fkd_tree_t t, fkd_stream_t stream, uint64_t* total, int* c
   FKD_HIP(hipStreamSynchronize(s));
   const uint32_t umax = t->last_nq ? *std::max_element(used.begin(), used.end()) : 0u;
   const bool pool_ok = umax <= t->pool_chunks;
-  if (!pool_ok) FKD_TRY(grow_pool(t, (uint64_t)umax + umax / 4 + 2 * fkd::kPoolBatch));
+  if (!pool_ok) FKD_TRY(grow_pool(t, (uint64_t)umax + umax / 4 + 2 * t->tune.pool_batch));
   if (total) *total = tot;
   if (complete) *complete = (pool_ok && tot <= t->last_capacity) ? 1 : 0;
   return FKD_OK;
@@ -885,6 +892,30 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
                           &t->pool_ctr, &t->q_counts_slot, &t->q_soff})
     b += d->bytes;
   *bytes = b;
+  return FKD_OK;
+}
+
+int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
+  if (!t || !key) return fail(FKD_ERR_INVALID, "tree / key is NULL");
+  const std::string k(key);
+  if (k == "pool_batch") {
+    if (value < 1 || value > 1024) return fail(FKD_ERR_INVALID, "pool_batch must be 1..1024");
+    t->tune.pool_batch = value;
+  } else if (k == "query_key_bits") {
+    if (value > 32u / (uint32_t)t->dim) return fail(FKD_ERR_INVALID, "query_key_bits * dim must be <= 32");
+    t->tune.query_key_bits = value;
+  } else {
+    return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
+  }
+  return FKD_OK;
+}
+
+int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
+  if (!t || !key || !value) return fail(FKD_ERR_INVALID, "NULL argument");
+  const std::string k(key);
+  if (k == "pool_batch") *value = t->tune.pool_batch;
+  else if (k == "query_key_bits") *value = t->tune.query_key_bits;
+  else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;
 }
 

@@ -39,15 +39,15 @@ struct SearchArgs {
   uint32_t node_bytes;    // size of the node records (buffer resource range)
   // single-pass mode: hits go to 16-id chunks of a pool split into
   // kPoolRegions regions of region_chunks chunks; block b allocates in
-  // region b % kPoolRegions, a wave reserves kPoolBatch chunks at a time.
+  // region b % kPoolRegions, a wave reserves pool_batch chunks at a time.
   uint32_t* pool;         // kPoolRegions * region_chunks * 16 ids
   uint2* chunk_meta;      // per chunk: (processing slot, index of the chunk in its query)
   uint32_t* pool_ctr;     // per region: chunks reserved (may exceed region_chunks)
   uint32_t region_chunks;
+  uint32_t pool_batch;    // chunks a wave reserves per atomic
 };
 
 constexpr uint32_t kPoolRegions = 256;
-constexpr uint32_t kPoolBatch = 32;
 constexpr uint32_t kNoQuery = 0xffffffffu;
 
 // depth % DIM for depth < 32, two bits per depth.
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     const uint32_t leader = (uint32_t)__builtin_ctzll(m);
     uint32_t wbase = s_wbase[wid], wleft = s_wleft[wid];
     if (cnt > wleft) {
-      const uint32_t grab = cnt > kPoolBatch ? cnt : kPoolBatch;
+      const uint32_t grab = cnt > a.pool_batch ? cnt : a.pool_batch;
       uint32_t base = 0;
       if (lane == leader) {
         base = atomicAdd(&a.pool_ctr[region], grab);
@@ -321,19 +321,20 @@ __global__ void gather_offsets_kernel(const uint64_t* __restrict__ qoff, const u
 }
 
 // Spatial sort key of every query: Morton interleave of its box centre,
-// quantised to KB bits per dimension inside the tree's bounding box.
+// quantised to kb bits per dimension inside the tree's bounding box.
 // Any key order gives the same results; this one only groups similar work.
-template <int DIM, int KB>
+template <int DIM>
 __global__ void query_keys_kernel(const float* __restrict__ mins, const float* __restrict__ maxs,
-                                  uint32_t nq, const uint32_t* __restrict__ bbox,
+                                  uint32_t nq, const uint32_t* __restrict__ bbox, uint32_t kb,
                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   float lo[DIM], sc[DIM];
+  const float cells = (float)(1u << kb);
 #pragma unroll
   for (int m = 0; m < DIM; ++m) {
     const float a = fkey_inv(bbox[m]), b = fkey_inv(bbox[DIM + m]);
     const float ext = b - a;
     lo[m] = a;
-    sc[m] = (ext > 0.f && ext < 3.0e38f) ? (float)(1u << KB) / ext : 0.f;
+    sc[m] = (ext > 0.f && ext < 3.0e38f) ? cells / ext : 0.f;
     if (!(lo[m] > -3.0e38f && lo[m] < 3.0e38f)) sc[m] = 0.f;
   }
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
@@ -344,11 +345,10 @@ __global__ void query_keys_kernel(const float* __restrict__ mins, const float* _
       const float c = 0.5f * mins[(uint64_t)m * nq + i] + 0.5f * maxs[(uint64_t)m * nq + i];
       float f = (c - lo[m]) * sc[m];
       f = (f >= 0.f) ? f : 0.f;  // also maps NaN to 0
-      f = f < (float)((1u << KB) - 1u) ? f : (float)((1u << KB) - 1u);
+      f = f < cells - 1.f ? f : cells - 1.f;
       u[m] = (uint32_t)f;
     }
-#pragma unroll
-    for (int b = 0; b < KB; ++b)
+    for (uint32_t b = 0; b < kb; ++b)
 #pragma unroll
       for (int m = 0; m < DIM; ++m) key |= ((u[m] >> b) & 1u) << (b * DIM + m);
     keys[i] = key;

@@ -1,0 +1,158 @@
+"""Autotuner of the search launch parameters.
+
+    python -m fkdtree_amd.tune [--workload mixed_1m] [--reps 3]
+
+Sweeps `query_key_bits` (Morton bits per dimension of the box order; 0
+keeps the caller's order) for the two-pass search and `pool_batch` (chunks a
+wave reserves per atomic) for the single-pass search on a workload, timing
+each with HIP events, and stores the
+fastest setting in `tuning.json` next to this file, keyed by GPU
+architecture, dimension and log2 of the point count.  Trees created by this
+package apply the nearest stored entry (`apply`).  Results never depend on
+these parameters: only time does.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import math
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning.json")
+
+CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10)}
+POOL_CANDIDATES = {"pool_batch": (8, 32, 128, 512)}
+
+
+def _arch() -> str:
+    from . import device_info
+
+    name = device_info()["name"]
+    return name.split("(")[-1].split(":")[0].rstrip(")") or "unknown"
+
+
+def _key(dim: int, n: int) -> str:
+    return f"dim{dim}_log2n{int(round(math.log2(max(n, 2))))}"
+
+
+def load_table(path: str = TABLE) -> Dict:
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def lookup(arch: str, dim: int, n: int, table: Optional[Dict] = None) -> Optional[Dict[str, int]]:
+    """Stored parameters of the nearest size bucket for (arch, dim), if any."""
+    table = load_table() if table is None else table
+    entries = table.get(arch, {})
+    best, dist = None, None
+    want = math.log2(max(n, 2))
+    for k, v in entries.items():
+        if not k.startswith(f"dim{dim}_"):
+            continue
+        d = abs(int(k.split("log2n")[1]) - want)
+        if dist is None or d < dist:
+            best, dist = v["params"], d
+    return best
+
+
+def apply(tree, arch: Optional[str] = None) -> Optional[Dict[str, int]]:
+    """Set the stored parameters on a tree handle; returns them (or None)."""
+    try:
+        params = lookup(arch or _arch(), tree.dim, max(tree.n // max(tree.n_events, 1), 1))
+    except Exception:
+        return None
+    if params:
+        tree.set_tuning(**params)
+    return params
+
+
+def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=None, d_qev: Optional[int] = None,
+          single: bool = False):
+    """Time the two-pass (or single-pass) search for every candidate;
+    [(params, ms)] sorted."""
+    from . import DeviceArray, Stream, Timer
+
+    cands = candidates or CANDIDATES
+    s = Stream()
+    d_off = DeviceArray(nq + 1, np.uint64)
+    tree.search_device(nq, d_mins, d_maxs, d_off.ptr, None, 0, d_qev=d_qev, stream=s, fill=False)
+    total = int(d_off.download(s)[-1])
+    d_out = DeviceArray(max(total, 1), np.uint32)
+    tm = Timer()
+    out = []
+    keys = list(cands)
+    for values in itertools.product(*(cands[k] for k in keys)):
+        params = dict(zip(keys, values))
+        tree.set_tuning(**params)
+
+        def run():
+            if single:
+                tree.search_single_device(nq, d_mins, d_maxs, d_off.ptr, d_out.ptr, total, d_qev=d_qev, stream=s)
+            else:
+                tree.search_device(nq, d_mins, d_maxs, d_off.ptr, d_out.ptr, total, d_qev=d_qev, stream=s)
+
+        run()
+        if single:
+            tree.search_status(s)   # lets the pool settle
+            run()
+        s.sync()
+        tm.start(s)
+        for _ in range(reps):
+            run()
+        tm.stop(s)
+        out.append((params, tm.ms() / reps))
+    out.sort(key=lambda x: x[1])
+    return out
+
+
+def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path: str = TABLE) -> Dict:
+    from . import DeviceArray, FKDTree, Stream, gen
+
+    pts, boxes = {"mixed_1m": gen.workload_mixed_1m, "uniform_100k": gen.workload_uniform_100k,
+                  "uniform_4m": gen.workload_uniform_4m}[workload]()
+    s = Stream()
+    d_c = DeviceArray.from_host(pts.coords, s)
+    d_i = DeviceArray.from_host(pts.ids, s)
+    d_mi = DeviceArray.from_host(boxes.mins, s)
+    d_ma = DeviceArray.from_host(boxes.maxs, s)
+    tree = FKDTree.from_device(pts.dim, pts.n, d_c.ptr, d_i.ptr, s)
+    tree.build()
+    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps)
+    tree.set_tuning(**res[0][0])
+    res_pool = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, candidates=POOL_CANDIDATES, single=True)
+    tree.close()
+    arch = _arch()
+    params = dict(res[0][0], **res_pool[0][0])
+    entry = {"params": params, "ms": res[0][1], "single_pass_ms": res_pool[0][1], "workload": workload,
+             "sweep": [{"params": p, "ms": ms} for p, ms in res],
+             "sweep_single_pass": [{"params": p, "ms": ms} for p, ms in res_pool]}
+    if write:
+        table = load_table(path)
+        table.setdefault(arch, {})[_key(pts.dim, pts.n)] = entry
+        with open(path, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+    return {"arch": arch, "key": _key(pts.dim, pts.n), **entry}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="mixed_1m")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--no-write", action="store_true")
+    ap.add_argument("--out", default=TABLE, help="tuning table to update")
+    a = ap.parse_args(argv)
+    r = autotune(a.workload, a.reps, write=not a.no_write, path=a.out)
+    print(json.dumps({k: r[k] for k in ("arch", "key", "params", "ms")}))
+    for row in r["sweep"] + r["sweep_single_pass"]:
+        print(json.dumps(row))
+
+
+if __name__ == "__main__":
+    main()

@@ -91,6 +91,15 @@ int fkd_tree_info(fkd_tree_t tree, int* dim, uint32_t* n_events, uint64_t* n_poi
  * many trees resident.  No reference counterpart. */
 int fkd_trim(fkd_tree_t tree, fkd_stream_t stream);
 
+/* Launch parameters of the handle (chosen by fkdtree_amd/tune.py; results
+ * never depend on them):
+ *   "pool_batch"      chunks of 16 ids a wave of the single-pass search
+ *                     reserves per atomic, 1..1024;
+ *   "query_key_bits"  Morton bits per dimension of the box order (0 keeps
+ *                     the caller's order), key bits * dim <= 32. */
+int fkd_set_tuning(fkd_tree_t tree, const char* key, uint32_t value);
+int fkd_get_tuning(fkd_tree_t tree, const char* key, uint32_t* value);
+
 /* Device memory held by the handle, in bytes. */
 int fkd_device_bytes(fkd_tree_t tree, uint64_t* bytes);
 

@@ -1,0 +1,69 @@
+"""Launch parameters change time only: the two-pass and single-pass searches
+return the oracle's id sets (FKDTree_cpu.h:123-169) for every query order
+(`query_key_bits`, 0 = the caller's order) and pool batch size."""
+import numpy as np
+import pytest
+
+import oracle
+from fkdtree_amd import DeviceArray, FKDTree
+from tests.helpers import adversarial_points, self_boxes, sorted_sets
+
+pytestmark = pytest.mark.gpu
+
+
+def _two_pass(t, dmi, dma, q):
+    off = DeviceArray(q + 1, np.uint64)
+    t.search_device(q, dmi.ptr, dma.ptr, off.ptr, None, 0, fill=False)
+    o = off.download()
+    out = DeviceArray(max(int(o[-1]), 1), np.uint32)
+    t.search_device(q, dmi.ptr, dma.ptr, off.ptr, out.ptr, int(o[-1]), count=False)
+    return o, out.download(count=int(o[-1]))
+
+
+def _single_pass(t, dmi, dma, q, cap):
+    off = DeviceArray(q + 1, np.uint64)
+    out = DeviceArray(max(cap, 1), np.uint32)
+    for _ in range(3):
+        t.search_single_device(q, dmi.ptr, dma.ptr, off.ptr, out.ptr, cap)
+        total, ok = t.search_status()
+        if ok:
+            break
+    assert ok and total <= cap
+    o = off.download()
+    return o, out.download(count=int(o[-1]))
+
+
+def test_tuning_keeps_results(require_gpu):
+    p = adversarial_points("mixed", 30_000)
+    boxes = self_boxes(p, 0.04)
+    print(f"[tuning] n={p.n} q={boxes.q}", flush=True)
+    tc, ti = oracle.build(p.coords, p.ids)
+    ooff, oids = oracle.search(tc, ti, boxes.mins, boxes.maxs)
+    want = sorted_sets(ooff, oids)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    t.build()
+    dmi = DeviceArray.from_host(boxes.mins)
+    dma = DeviceArray.from_host(boxes.maxs)
+    for kb in (0, 1, 5, 10):
+        t.set_tuning(query_key_bits=kb)
+        assert t.get_tuning("query_key_bits") == kb
+        o, ids = _two_pass(t, dmi, dma, boxes.q)
+        assert np.array_equal(o, ooff), f"key bits {kb}: counts differ"
+        assert np.array_equal(sorted_sets(o, ids), want), f"key bits {kb}: ids differ"
+    t.set_tuning(query_key_bits=8)
+    for pb in (1, 8, 1024):
+        t.set_tuning(pool_batch=pb)
+        assert t.get_tuning("pool_batch") == pb
+        o, ids = _single_pass(t, dmi, dma, boxes.q, int(ooff[-1]))
+        assert np.array_equal(o, ooff), f"pool batch {pb}: counts differ"
+        assert np.array_equal(sorted_sets(o, ids), want), f"pool batch {pb}: ids differ"
+    t.close()
+
+
+def test_tuning_rejects_bad_values(require_gpu):
+    p = adversarial_points("mixed", 100)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    for k, v in (("pool_batch", 0), ("pool_batch", 2000), ("query_key_bits", 11), ("no_such_key", 1)):
+        with pytest.raises(RuntimeError):
+            t.set_tuning(**{k: v})
+    t.close()
