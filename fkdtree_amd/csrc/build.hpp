@@ -18,9 +18,15 @@
 namespace fkd {
 
 // Record stride (floats) of a packed node: coordinates then the id bits.
+// Search records of a node, two arrays of kFloats floats per node: the DIM
+// coordinates, then at index kSplit either the coordinate on the node's
+// split dimension (depth % DIM) once more (`nodes`, for counting: the
+// traversal reads the split value without selecting by dimension) or the id
+// (`nodes_id`, for emitting hits without a second load).
 template <int DIM>
 struct NodeRec {
   static constexpr int kFloats = DIM <= 3 ? 4 : 8;
+  static constexpr int kSplit = DIM <= 3 ? 3 : 4;
 };
 
 // AoS FKDPoint<float, DIM> records (DIM floats then a uint32 id) -> SoA.
@@ -113,40 +119,50 @@ struct LevelArgs {
   uint32_t P;                 // plane stride
   float* Tc;                  // tree outputs
   uint32_t* Tid;
-  float* nodes;
+  float* nodes;               // search records (split value / id)
+  float* nodes_id;
   uint64_t* desc;             // look-back descriptors, zero at launch
   uint32_t* ctr;              // dynamic block counter, zero at launch
 };
 
 template <int DIM>
-__device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t node, uint32_t p) {
+__device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t node, uint32_t depth, uint32_t p) {
   constexpr int RF = NodeRec<DIM>::kFloats;
   float c[DIM];
+  float split = 0.f;
+  const uint32_t dd = depth % (uint32_t)DIM;
 #pragma unroll
   for (int m = 0; m < DIM; ++m) {
     c[m] = a.X[(uint64_t)m * a.P + p];
     a.Tc[(uint64_t)m * a.P + o + node] = c[m];
+    split = dd == (uint32_t)m ? c[m] : split;
   }
   const uint32_t id = a.ids[p];
   a.Tid[o + node] = id;
   float* rec = a.nodes + (uint64_t)(o + node) * RF;
+  float* rid = a.nodes_id + (uint64_t)(o + node) * RF;
   if (RF == 4) {
     float4 v;
     v.x = c[0];
     v.y = DIM > 1 ? c[DIM > 1 ? 1 : 0] : 0.f;
     v.z = DIM > 2 ? c[DIM > 2 ? 2 : 0] : 0.f;
-    v.w = __uint_as_float(id);
+    v.w = split;
     *reinterpret_cast<float4*>(rec) = v;
+    v.w = __uint_as_float(id);
+    *reinterpret_cast<float4*>(rid) = v;
   } else {
     float4 v0, v1;
     v0.x = c[0];
     v0.y = c[DIM > 1 ? 1 : 0];
     v0.z = c[DIM > 2 ? 2 : 0];
     v0.w = c[DIM > 3 ? 3 : 0];
-    v1.x = __uint_as_float(id);
+    v1.x = split;
     v1.y = v1.z = v1.w = 0.f;
     reinterpret_cast<float4*>(rec)[0] = v0;
     reinterpret_cast<float4*>(rec)[1] = v1;
+    v1.x = __uint_as_float(id);
+    reinterpret_cast<float4*>(rid)[0] = v0;
+    reinterpret_cast<float4*>(rid)[1] = v1;
   }
 }
 
@@ -377,7 +393,7 @@ __global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a) {
     uint32_t start, size;
     subtree_range(sh, L, ts, start, size);
     const uint32_t j = start + left_size(size);
-    write_node(a, o, v, a.S0[a.F[o + j]]);
+    write_node(a, o, v, L, a.S0[a.F[o + j]]);
   }
 }
 

@@ -144,7 +144,7 @@ struct fkd_tree_s {
   DevBuf off;                  // uint32 event offsets (n_events + 1)
   Tiling radix_tiles, level_tiles;
   DevBuf X, ids;               // input copy
-  DevBuf Tc, Tid, nodes;       // outputs
+  DevBuf Tc, Tid, nodes, nodes_id;  // outputs and search records
   DevBuf A, K, R, planes, F;   // build scratch
   DevBuf counts, offsets, desc, ctr, err, bbox;
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev;  // search scratch
@@ -299,6 +299,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     a.Tc = t->Tc.as<float>();
     a.Tid = t->Tid.as<uint32_t>();
     a.nodes = t->nodes.as<float>();
+    a.nodes_id = t->nodes_id.as<float>();
     a.desc = t->desc.as<uint64_t>();
     a.ctr = t->ctr.as<uint32_t>();
     const uint32_t blocks = (M ? M : 1u) * tm.T;
@@ -336,6 +337,7 @@ template <int DIM>
 fkd::SearchArgs search_args(fkd_tree_s* t, uint32_t nq, const uint32_t* qev) {
   fkd::SearchArgs a{};
   a.nodes = t->nodes.as<float>();
+  a.nodes_id = t->nodes_id.as<float>();
   a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
   a.n_single = t->P;
   a.qev = qev;
@@ -583,6 +585,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(t->Tc.ensure(S * dim * 4));
   FKD_TRY(t->Tid.ensure(S * 4));
   FKD_TRY(t->nodes.ensure(S * rf * 4));
+  FKD_TRY(t->nodes_id.ensure(S * rf * 4));
   FKD_TRY(ensure_build_scratch(t));
   FKD_TRY(t->ctr.ensure(64));
   FKD_TRY(t->err.ensure(64));
@@ -885,7 +888,7 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
   if (!t || !bytes) return fail(FKD_ERR_INVALID, "tree / bytes is NULL");
   uint64_t b = 0;
   for (const DevBuf* d : {&t->off, &t->radix_tiles.tile_ev, &t->radix_tiles.tb, &t->level_tiles.tile_ev,
-                          &t->level_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->A, &t->K, &t->R,
+                          &t->level_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R,
                           &t->planes, &t->F, &t->counts, &t->offsets, &t->desc, &t->ctr, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
                           &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta,

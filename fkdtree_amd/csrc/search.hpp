@@ -21,7 +21,8 @@
 namespace fkd {
 
 struct SearchArgs {
-  const float* nodes;     // packed records, NodeRec<DIM>::kFloats floats each
+  const float* nodes;     // records with the split value, NodeRec<DIM>::kFloats floats each
+  const float* nodes_id;  // records with the id
   const uint32_t* off;    // event -> first node (n_events + 1), or null
   uint32_t n_single;      // node count when off == null
   const uint32_t* qev;    // query -> event, or null (event 0)
@@ -50,49 +51,46 @@ struct SearchArgs {
 constexpr uint32_t kPoolRegions = 256;
 constexpr uint32_t kNoQuery = 0xffffffffu;
 
-// depth % DIM for depth < 32, two bits per depth.
+// depth % DIM for depth < 32 without a 32-bit multiply: for DIM 3,
+// floor(depth / 3) == (depth * 11) >> 5 on 0..31.
 template <int DIM>
 __device__ inline uint32_t depth_dim(uint32_t depth) {
-  constexpr uint64_t tab = [] {
-    uint64_t t = 0;
-    for (int b = 0; b < 32; ++b) t |= (uint64_t)(b % DIM) << (2 * b);
-    return t;
-  }();
-  return (uint32_t)(tab >> (2u * depth)) & 3u;
+  if (DIM == 1) return 0u;
+  if (DIM == 2) return depth & 1u;
+  if (DIM == 4) return depth & 3u;
+  if (DIM == 3) return depth - 3u * (__umul24(depth, 11u) >> 5);
+  return depth % (uint32_t)DIM;
 }
 
-// Node records are read through a buffer resource (32-bit byte offsets, reads
-// past num_records return zero) when the records fit in 4 GiB, else through a
-// flat pointer.
+// Node records are read through a buffer resource (32-bit byte offsets,
+// reads past num_records return zero) when the records fit in 4 GiB, else
+// through a flat pointer.  The resource spans the whole batch (uniform across
+// the wave); the per-lane record index goes into the byte offset.
 struct NodeSource {
   const float* flat;
   __amdgpu_buffer_rsrc_t rsrc;
 };
 
-// Record i of the tree that starts at record `base` of the batch.  The
-// buffer resource spans the whole batch (uniform across the wave), the
-// per-lane tree offset goes into the 32-bit byte offset.
+// Coordinates and the word at index kSplit (split value or id) of record `rec`.
 template <int DIM, bool BUF>
-__device__ inline void load_rec(const NodeSource& src, uint32_t base, uint32_t i, float (&c)[DIM],
-                                uint32_t& id) {
+__device__ inline void load_rec(const NodeSource& src, uint32_t rec, float (&c)[DIM], float& extra) {
   constexpr int RF = NodeRec<DIM>::kFloats;
-  float4 v;
+  float4 q;
   if (BUF)
-    v = __builtin_bit_cast(float4,
-                           __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, (base + i) * (uint32_t)(RF * 4), 0, 0));
+    q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, rec * (uint32_t)(RF * 4), 0, 0));
   else
-    v = *reinterpret_cast<const float4*>(src.flat + ((uint64_t)base + i) * RF);
-  c[0] = v.x;
-  if (DIM > 1) c[DIM > 1 ? 1 : 0] = v.y;
-  if (DIM > 2) c[DIM > 2 ? 2 : 0] = v.z;
+    q = *reinterpret_cast<const float4*>(src.flat + (uint64_t)rec * RF);
+  c[0] = q.x;
+  if (DIM > 1) c[DIM > 1 ? 1 : 0] = q.y;
+  if (DIM > 2) c[DIM > 2 ? 2 : 0] = q.z;
   if (RF == 4) {
-    id = __float_as_uint(v.w);
+    extra = q.w;
   } else {
-    c[DIM > 3 ? 3 : 0] = v.w;
+    c[DIM > 3 ? 3 : 0] = q.w;
     if (BUF)
-      id = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, (base + i) * (uint32_t)(RF * 4) + 16u, 0, 0);
+      extra = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(src.rsrc, rec * (uint32_t)(RF * 4) + 16u, 0, 0));
     else
-      id = __float_as_uint(src.flat[((uint64_t)base + i) * RF + 4]);
+      extra = src.flat[(uint64_t)rec * RF + NodeRec<DIM>::kSplit];
   }
 }
 
@@ -100,15 +98,30 @@ enum SearchMode { kCount = 0, kFill = 1, kStats = 2, kPool = 3 };
 
 constexpr int kStage = 16;  // ids staged per lane: one 64-byte line
 
-// The loop body is branch free except for the emission of a hit in the fill
-// pass: the next node is selected among (left son, right son, pending right
-// son of the deepest pending depth) with conditional moves.
+// Fill staging: hits go to a ring of kRing ids per lane in LDS, ring index
+// i holding the id for output slot wb + i (wb = the query's first output slot
+// rounded down to 4 ids) at ring slot i % kRing.  Whenever some lane's ring
+// is full, every lane writes its complete 4-id groups to `out` as 16-byte
+// stores, so that stores are issued a few times per wave instead of once per
+// lane and line.
+constexpr int kRing = 16;
+constexpr int kRingRow = 20;  // words per lane: 16-byte aligned rows, spread over the banks
+
+// The loop body is branch free except for writing out staged ids: the next
+// node is selected among (left son, right son, pending right son of the
+// deepest pending depth) with conditional moves.  Counting reads the split
+// value from the record and the box bounds on the split dimension from LDS
+// (DIM >= 3), so no per-lane select on the dimension is needed; emitting
+// passes read the id from the record and select the split value.
 template <int DIM, int MODE, bool BUF>
 __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   constexpr bool FILL = MODE == kFill;
   constexpr bool POOL = MODE == kPool;
   constexpr bool STAGE = FILL || POOL;
-  __shared__ uint32_t stage[STAGE ? kThreads : 1][kStage + 1];
+  constexpr bool LDS_BOX = DIM >= 3 && !STAGE;
+  __shared__ uint32_t stage[POOL ? kThreads : 1][kStage + 1];
+  __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
+  __shared__ float2 s_box[LDS_BOX ? DIM : 1][LDS_BOX ? kThreads : 1];
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
   if (slot >= a.nq) return;
   const uint32_t q = a.perm ? a.perm[slot] : slot;
@@ -123,27 +136,38 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   for (int m = 0; m < DIM; ++m) {
     mn[m] = a.mins[(uint64_t)m * a.nq + slot];
     mx[m] = a.maxs[(uint64_t)m * a.nq + slot];
+    if (LDS_BOX) s_box[LDS_BOX ? m : 0][LDS_BOX ? threadIdx.x : 0] = make_float2(mn[m], mx[m]);
   }
+  const float* recs = STAGE ? a.nodes_id : a.nodes;
   NodeSource src;
-  src.flat = a.nodes;
-  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.nodes, (short)0, (int)a.node_bytes, 0x00020000);
-  uint64_t w = FILL ? a.qoff[q] : 0;  // next output slot not yet written
-  uint32_t nb = 0;                    // ids staged
+  src.flat = recs;
+  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)a.node_bytes, 0x00020000);
   uint32_t c = 0;
-  uint32_t* st = stage[STAGE ? threadIdx.x : 0];
-  auto flush = [&]() {
-    if (nb == kStage && a.out_aligned && (w & (kStage - 1)) == 0 && w + kStage <= a.cap) {
-      uint4* dst = reinterpret_cast<uint4*>(a.out + w);
+  uint32_t* st = stage[POOL ? threadIdx.x : 0];
+  uint32_t* rw = &ring[FILL ? threadIdx.x * kRingRow : 0];
+  const uint64_t w0 = FILL ? a.qoff[q] : 0;
+  const uint64_t wb = w0 & ~(uint64_t)3;
+  const uint32_t i0 = (uint32_t)w0 & 3u;  // first ring index of this query
+  uint32_t wr = i0;                       // next ring index
+  uint32_t fq = 0;                        // first ring index not written out (multiple of 4)
+  // Writes the complete 4-id groups; the query's first group is partial when
+  // its first slot is not 16-byte aligned.
+  auto write_groups = [&]() {
+    while (fq + 4u <= wr) {
+      const uint4 g = *reinterpret_cast<const uint4*>(rw + (fq & (kRing - 1)));
+      const uint64_t p = wb + fq;
+      if (fq >= i0 && a.out_aligned && p + 4u <= a.cap) {
+        *reinterpret_cast<uint4*>(a.out + p) = g;
+      } else {
+        const uint32_t e[4] = {g.x, g.y, g.z, g.w};
 #pragma unroll
-      for (int i = 0; i < kStage / 4; ++i)
-        dst[i] = make_uint4(st[4 * i], st[4 * i + 1], st[4 * i + 2], st[4 * i + 3]);
-    } else {
-      for (uint32_t i = 0; i < nb; ++i)
-        if (w + i < a.cap) a.out[w + i] = st[i];
+        for (uint32_t j = 0; j < 4u; ++j)
+          if (fq + j >= i0 && p + j < a.cap) a.out[p + j] = e[j];
+      }
+      fq += 4u;
     }
-    w += nb;
-    nb = 0;
   };
+  uint32_t nb = 0;      // pool mode: ids staged
   uint32_t nchunk = 0;  // chunks of this query handed out so far (pool mode)
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x / kWave;
@@ -195,59 +219,70 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
       nb = 0;
     }
   };
-  uint32_t node = 0, depth = 0, dd = 0, pending = 0, vis = 0;
+  // node has a left (right) son iff node < tl (tr), i.e. 2 node + 1 (2) < n
+  const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
+  uint32_t node = 0, depth = 0, pending = 0, vis = 0;
   bool alive = n != 0;
   while (alive) {
     if (MODE == kStats) ++vis;
-    float x[DIM];
-    uint32_t id;
-    load_rec<DIM, BUF>(src, base, node, x, id);
-    float v = x[0], lo = mn[0], hi = mx[0];
+    float x[DIM], extra;
+    load_rec<DIM, BUF>(src, base + node, x, extra);
+    const uint32_t dd = depth_dim<DIM>(depth);
+    float v = STAGE ? x[0] : extra, lo = mn[0], hi = mx[0];
+    if (LDS_BOX) {
+      const float2 bx = s_box[LDS_BOX ? dd : 0][LDS_BOX ? threadIdx.x : 0];
+      lo = bx.x;
+      hi = bx.y;
+    } else {
 #pragma unroll
-    for (int m = 1; m < DIM; ++m) {
-      const bool sel = dd == (uint32_t)m;
-      v = sel ? x[m] : v;
-      lo = sel ? mn[m] : lo;
-      hi = sel ? mx[m] : hi;
+      for (int m = 1; m < DIM; ++m) {
+        if (STAGE) v = dd == (uint32_t)m ? x[m] : v;
+        lo = dd == (uint32_t)m ? mn[m] : lo;
+        hi = dd == (uint32_t)m ? mx[m] : hi;
+      }
     }
+    const uint32_t id = __float_as_uint(extra);
     bool in = true;
 #pragma unroll
     for (int m = 0; m < DIM; ++m) in = in & (x[m] <= mx[m]) & (x[m] >= mn[m]);
     const bool lower = v < lo;
     const bool inter = (v <= hi) & (v >= lo);
     if (FILL) {
-      if (in) {
-        st[nb++] = id;
-        if (((w + nb) & (kStage - 1)) == 0) flush();
-      }
+      rw[wr & (kRing - 1)] = id;  // the slot is free; it is kept only for a hit
+      wr += in ? 1u : 0u;
+      if (__ballot(wr - fq == (uint32_t)kRing) != 0ull) write_groups();
     } else if (POOL) {
-      if (in) st[nb++] = id;
+      st[nb] = id;
+      nb += in ? 1u : 0u;
       c += in ? 1u : 0u;
       pool_flush(nb == (uint32_t)kStage);
     } else {
       c += in ? 1u : 0u;
     }
-    const uint32_t lc = 2u * node + 1u;
-    const bool go_l = (!lower) & (lc < n);
-    const bool go_r = (lower | inter) & (lc + 1u < n);
-    const bool desc = go_l | go_r;
-    pending |= (go_l & go_r) ? (2u << depth) : 0u;
+    const bool go_l = !lower && node < tl;
+    const bool go_r = (lower || inter) && node < tr;
+    const bool desc = go_l || go_r;
+    pending = (go_l && go_r) ? (pending | (2u << depth)) : pending;
+    // deepest pending depth (bit b set, b >= 1); only used when pending != 0
     const uint32_t b = 31u - (uint32_t)__builtin_clz(pending | 1u);
     const uint32_t up = (node + 1u) >> (depth - b);
-    alive = desc | (pending != 0u);
+    alive = desc || pending != 0u;
+    const uint32_t lc = 2u * node + 1u;
     node = go_l ? lc : (go_r ? lc + 1u : up);
-    pending = desc ? pending : (pending & ~(1u << b));
-    const uint32_t dn = (dd + 1u == (uint32_t)DIM) ? 0u : dd + 1u;
-    dd = desc ? dn : depth_dim<DIM>(b);
+    pending = desc ? pending : __builtin_amdgcn_ubfe(pending, 0u, b);  // clears bit b
     depth = desc ? depth + 1u : b;
   }
-  if (FILL && nb) flush();
+  if (FILL) {
+    write_groups();
+    for (uint32_t j = fq > i0 ? fq : i0; j < wr; ++j)
+      if (wb + j < a.cap) a.out[wb + j] = rw[j & (kRing - 1)];
+  }
   if (POOL) {
     pool_flush(nb != 0u);
     // mark the unused rest of this wave's reservation (by one lane: the
     // wave may be partly empty)
-    const uint32_t first = (uint32_t)__builtin_ctzll(__ballot(true));
-    if (lane == first) {
+    const uint32_t first_lane = (uint32_t)__builtin_ctzll(__ballot(true));
+    if (lane == first_lane) {
       const uint32_t wb = s_wbase[wid], wl = s_wleft[wid];
       for (uint32_t i = 0; i < wl; ++i) {
         const uint32_t k = wb + i;
