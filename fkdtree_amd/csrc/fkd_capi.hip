@@ -130,6 +130,7 @@ struct Tiling {
 struct fkd_tuning {
   uint32_t pool_batch = 32;     // single-pass search: chunks a wave reserves per atomic
   uint32_t query_key_bits = 8;  // search: Morton bits per dimension of the box order
+  uint32_t fill_order = 1;      // fill pass: heaviest search blocks first
 };
 
 struct fkd_tree_s {
@@ -151,6 +152,7 @@ struct fkd_tree_s {
   DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets, q_desc;
   DevBuf pool, pool_meta, pool_ctr;  // single-pass search chunk pool
   DevBuf q_counts_slot, q_soff;
+  DevBuf q_bwork, q_border;    // fill pass: work per search block and dispatch order
   uint32_t pool_chunks = 0;
   uint64_t last_capacity = 0;
   uint64_t* last_offsets = nullptr;
@@ -506,6 +508,16 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
   a.out = out;
   a.cap = cap;
   a.out_aligned = (reinterpret_cast<uintptr_t>(out) & 63u) == 0;
+  const uint32_t nb = (uint32_t)((nq + fkd::kThreads - 1) / fkd::kThreads);
+  if (t->tune.fill_order && nb >= 64) {
+    FKD_TRY(t->q_bwork.ensure((size_t)nb * 4));
+    FKD_TRY(t->q_border.ensure((size_t)nb * 4));
+    fkd::block_work_kernel<<<nb, fkd::kThreads, 0, s>>>(qoff, a.perm, nq, t->q_bwork.as<uint32_t>());
+    FKD_TRY(launch_check("block_work"));
+    fkd::block_order_kernel<<<1, 1024, 0, s>>>(t->q_bwork.as<uint32_t>(), nb, t->q_border.as<uint32_t>());
+    FKD_TRY(launch_check("block_order"));
+    a.block_order = t->q_border.as<uint32_t>();
+  }
   if (t->trace_start) FKD_HIP(hipEventRecord(t->trace_start, s));
   FKD_TRY((launch_search<DIM, fkd::kFill>(t, a, s, "search_fill")));
   if (t->trace_stop) FKD_HIP(hipEventRecord(t->trace_stop, s));
@@ -873,7 +885,8 @@ int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
   FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
                     &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
-                    &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta, &t->q_counts_slot, &t->q_soff})
+                    &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta, &t->q_counts_slot, &t->q_soff,
+                    &t->q_bwork, &t->q_border})
     b->release();
   t->desc.release();
   FKD_TRY(t->desc.ensure(256));
@@ -892,7 +905,7 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
                           &t->planes, &t->F, &t->counts, &t->offsets, &t->desc, &t->ctr, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
                           &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta,
-                          &t->pool_ctr, &t->q_counts_slot, &t->q_soff})
+                          &t->pool_ctr, &t->q_counts_slot, &t->q_soff, &t->q_bwork, &t->q_border})
     b += d->bytes;
   *bytes = b;
   return FKD_OK;
@@ -907,6 +920,9 @@ int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
   } else if (k == "query_key_bits") {
     if (value > 32u / (uint32_t)t->dim) return fail(FKD_ERR_INVALID, "query_key_bits * dim must be <= 32");
     t->tune.query_key_bits = value;
+  } else if (k == "fill_order") {
+    if (value > 1) return fail(FKD_ERR_INVALID, "fill_order must be 0 or 1");
+    t->tune.fill_order = value;
   } else {
     return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   }
@@ -918,6 +934,7 @@ int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
   const std::string k(key);
   if (k == "pool_batch") *value = t->tune.pool_batch;
   else if (k == "query_key_bits") *value = t->tune.query_key_bits;
+  else if (k == "fill_order") *value = t->tune.fill_order;
   else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;
 }

@@ -27,6 +27,7 @@ struct SearchArgs {
   uint32_t n_single;      // node count when off == null
   const uint32_t* qev;    // query -> event, or null (event 0)
   const uint32_t* perm;   // processing slot -> query (null: identity)
+  const uint32_t* block_order;  // dispatch position -> search block (null: identity)
   uint32_t nq;
   const float* mins;      // [DIM][nq], indexed by processing slot
   const float* maxs;
@@ -50,6 +51,18 @@ struct SearchArgs {
 
 constexpr uint32_t kPoolRegions = 256;
 constexpr uint32_t kNoQuery = 0xffffffffu;
+
+// Workgroups are handed to the 8 XCDs in turn.  Workgroup b of a search
+// launch takes position start(b % 8) + b / 8, so that every XCD walks one
+// contiguous range of the spatially ordered queries and its L2 holds that
+// range's part of the tree; `order` (if any) maps positions to blocks.
+__host__ __device__ inline uint32_t xcd_chunk_start(uint32_t x, uint32_t nb) {
+  return x * (nb >> 3) + (x < (nb & 7u) ? x : (nb & 7u));
+}
+__device__ inline uint32_t search_block(uint32_t b, uint32_t nb, const uint32_t* order) {
+  const uint32_t j = xcd_chunk_start(b & 7u, nb) + (b >> 3);
+  return order ? order[j] : j;
+}
 
 // depth % DIM for depth < 32 without a 32-bit multiply: for DIM 3,
 // floor(depth / 3) == (depth * 11) >> 5 on 0..31.
@@ -122,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   __shared__ uint32_t stage[POOL ? kThreads : 1][kStage + 1];
   __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
   __shared__ float2 s_box[LDS_BOX ? DIM : 1][LDS_BOX ? kThreads : 1];
-  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t slot = search_block(blockIdx.x, gridDim.x, a.block_order) * kThreads + threadIdx.x;
   if (slot >= a.nq) return;
   const uint32_t q = a.perm ? a.perm[slot] : slot;
   uint32_t base = 0, n = a.n_single;
@@ -293,6 +306,66 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   if (!FILL) a.counts[q] = c;
   if (POOL) a.counts_slot[slot] = c;
   if (MODE == kStats) a.visited[q] = vis;
+}
+
+// Work estimate of every search block for the fill pass: the largest hit
+// count among its queries (processing order).
+__global__ __launch_bounds__(kThreads) void block_work_kernel(const uint64_t* __restrict__ qoff,
+                                                              const uint32_t* __restrict__ perm, uint32_t nq,
+                                                              uint32_t* __restrict__ work) {
+  __shared__ uint32_t s_max[kThreads / kWave];
+  const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
+  uint32_t h = 0;
+  if (slot < nq) {
+    const uint32_t q = perm ? perm[slot] : slot;
+    const uint64_t d = qoff[q + 1] - qoff[q];
+    h = d > 0xffffffffull ? 0xffffffffu : (uint32_t)d;
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) h = max(h, (uint32_t)__shfl_xor((int)h, o));
+  if (lane_id() == 0) s_max[threadIdx.x / kWave] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / kWave; ++w) m = max(m, s_max[w]);
+    work[blockIdx.x] = m;
+  }
+}
+
+// Dispatch order of the fill pass: within every XCD chunk, blocks by
+// decreasing work, bucketed on log2 with two fraction bits, so that the
+// heaviest blocks start first and the last to finish are light ones.  The
+// order inside a bucket is arbitrary; it only affects time.  One workgroup.
+constexpr uint32_t kWorkBuckets = 128;
+__device__ inline uint32_t work_bucket(uint32_t w) {
+  const uint32_t lg = floor_log2(w);
+  const uint32_t frac = lg >= 2u ? (w >> (lg - 2u)) & 3u : (lg == 1u ? (w & 1u) << 1 : 0u);
+  return (31u - lg) * 4u + (3u - frac);
+}
+__global__ __launch_bounds__(1024) void block_order_kernel(const uint32_t* __restrict__ work, uint32_t nb,
+                                                           uint32_t* __restrict__ order) {
+  __shared__ uint32_t cur[8][kWorkBuckets];
+  for (uint32_t i = threadIdx.x; i < 8u * kWorkBuckets; i += blockDim.x) (&cur[0][0])[i] = 0;
+  __syncthreads();
+  auto chunk_of = [&](uint32_t b) {
+    uint32_t x = 0;
+    while (x < 7u && b >= xcd_chunk_start(x + 1u, nb)) ++x;
+    return x;
+  };
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) atomicAdd(&cur[chunk_of(b)][work_bucket(work[b])], 1u);
+  __syncthreads();
+  if (threadIdx.x < 8u) {
+    uint32_t run = xcd_chunk_start(threadIdx.x, nb);
+    for (uint32_t k = 0; k < kWorkBuckets; ++k) {
+      const uint32_t c = cur[threadIdx.x][k];
+      cur[threadIdx.x][k] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+    order[atomicAdd(&cur[chunk_of(b)][work_bucket(work[b])], 1u)] = b;
 }
 
 // Copies the pooled chunks into the query-ordered CSR: slot t of chunk k

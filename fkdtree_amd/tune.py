@@ -25,7 +25,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10)}
+CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1)}
 POOL_CANDIDATES = {"pool_batch": (8, 32, 128, 512)}
 
 

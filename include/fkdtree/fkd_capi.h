@@ -86,8 +86,8 @@ int fkd_tree_device(fkd_tree_t tree, const float** dev_coords, const uint32_t** 
 int fkd_tree_info(fkd_tree_t tree, int* dim, uint32_t* n_events, uint64_t* n_points);
 
 /* Release the build and search scratch of a built tree (synchronises
- * `stream`): the tree arrays and input points stay, about 48 B per point at
- * dim 3.  Later builds and searches re-create what they need.  For keeping
+ * `stream`): the tree arrays, search records and input points stay, about
+ * 64 B per point at dim 3.  Later builds and searches re-create what they need.  For keeping
  * many trees resident.  No reference counterpart. */
 int fkd_trim(fkd_tree_t tree, fkd_stream_t stream);
 
@@ -96,7 +96,9 @@ int fkd_trim(fkd_tree_t tree, fkd_stream_t stream);
  *   "pool_batch"      chunks of 16 ids a wave of the single-pass search
  *                     reserves per atomic, 1..1024;
  *   "query_key_bits"  Morton bits per dimension of the box order (0 keeps
- *                     the caller's order), key bits * dim <= 32. */
+ *                     the caller's order), key bits * dim <= 32;
+ *   "fill_order"      1: the fill pass starts the search blocks with the
+ *                     most hits first, 0: in box order. */
 int fkd_set_tuning(fkd_tree_t tree, const char* key, uint32_t value);
 int fkd_get_tuning(fkd_tree_t tree, const char* key, uint32_t* value);
 

@@ -44,12 +44,12 @@ def test_tuning_keeps_results(require_gpu):
     t.build()
     dmi = DeviceArray.from_host(boxes.mins)
     dma = DeviceArray.from_host(boxes.maxs)
-    for kb in (0, 1, 5, 10):
-        t.set_tuning(query_key_bits=kb)
-        assert t.get_tuning("query_key_bits") == kb
+    for kb, fo in ((0, 1), (1, 0), (5, 1), (10, 0), (8, 1)):
+        t.set_tuning(query_key_bits=kb, fill_order=fo)
+        assert t.get_tuning("query_key_bits") == kb and t.get_tuning("fill_order") == fo
         o, ids = _two_pass(t, dmi, dma, boxes.q)
-        assert np.array_equal(o, ooff), f"key bits {kb}: counts differ"
-        assert np.array_equal(sorted_sets(o, ids), want), f"key bits {kb}: ids differ"
+        assert np.array_equal(o, ooff), f"key bits {kb}, fill order {fo}: counts differ"
+        assert np.array_equal(sorted_sets(o, ids), want), f"key bits {kb}, fill order {fo}: ids differ"
     t.set_tuning(query_key_bits=8)
     for pb in (1, 8, 1024):
         t.set_tuning(pool_batch=pb)
@@ -63,7 +63,8 @@ def test_tuning_keeps_results(require_gpu):
 def test_tuning_rejects_bad_values(require_gpu):
     p = adversarial_points("mixed", 100)
     t = FKDTree(coords=p.coords, ids=p.ids)
-    for k, v in (("pool_batch", 0), ("pool_batch", 2000), ("query_key_bits", 11), ("no_such_key", 1)):
+    for k, v in (("pool_batch", 0), ("pool_batch", 2000), ("query_key_bits", 11), ("fill_order", 2),
+                 ("no_such_key", 1)):
         with pytest.raises(RuntimeError):
             t.set_tuning(**{k: v})
     t.close()
