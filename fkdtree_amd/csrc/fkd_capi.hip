@@ -148,7 +148,7 @@ struct fkd_tree_s {
   DevBuf Tc, Tid, nodes, nodes_id;  // outputs and search records
   DevBuf A, K, R, planes, F;   // build scratch
   DevBuf counts, offsets, desc, ctr, err, bbox;
-  DevBuf q_counts, q_off, q_hits, q_boxes, q_ev;  // search scratch
+  DevBuf q_counts, q_off, q_hits, q_boxes, q_ev, q_aos;  // search scratch
   DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets, q_desc;
   DevBuf pool, pool_meta, pool_ctr;  // single-pass search chunk pool
   DevBuf q_counts_slot, q_soff;
@@ -158,6 +158,7 @@ struct fkd_tree_s {
   uint64_t* last_offsets = nullptr;
   hipEvent_t trace_start = nullptr, trace_stop = nullptr;  // around the traversal kernel
   uint32_t q_plan_nq = 0;  // queries whose processing order q_perm holds
+  uint32_t q_perm_half = 0;  // which half of q_perm holds it
   uint64_t last_nq = 0, last_total = 0;
 };
 
@@ -349,6 +350,11 @@ fkd::SearchArgs search_args(fkd_tree_s* t, uint32_t nq, const uint32_t* qev) {
 
 // Processing order of the queries (spatial sort of the box centres) and the
 // boxes permuted into it.  Kept in the tree for the fill pass that follows.
+// Processing slot -> query of the current plan.
+uint32_t* plan_perm(fkd_tree_s* t) {
+  return t->q_perm.as<uint32_t>() + (uint64_t)t->q_perm_half * std::max<uint32_t>(t->q_plan_nq, 1);
+}
+
 template <int DIM>
 int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, hipStream_t s) {
   const size_t n = std::max<uint32_t>(nq, 1);
@@ -364,7 +370,9 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   uint32_t* perm = t->q_perm.as<uint32_t>();
   const uint32_t g = grid_for(nq);
   const uint32_t kb = t->tune.query_key_bits;
-  fkd::query_keys_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), kb, keys, perm);
+  FKD_TRY(t->q_aos.ensure(n * 32));
+  fkd::query_keys_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), kb, keys, perm,
+                                                t->q_aos.as<float4>());
   FKD_TRY(launch_check("query_keys"));
   fkd::TileMap tm{};
   tm.tile_ev = nullptr;
@@ -380,14 +388,11 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
                        nullptr, 8u * pass, !lastp, false, s));
     c = 1 - c;
   }
-  if (c != 0) {
-    // leave the final order in perm[0..nq)
-    fkd::copy_u32_kernel<<<g, 256, 0, s>>>(perm, perm + n, nq);
-    FKD_TRY(launch_check("perm_copy"));
-  }
+  t->q_perm_half = (uint32_t)c;
   float* smins = t->q_sbox.as<float>();
   float* smaxs = smins + (uint64_t)DIM * n;
-  fkd::permute_boxes_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, perm, nq, smins, smaxs);
+  fkd::permute_boxes_kernel<DIM><<<g, 256, 0, s>>>(t->q_aos.as<float4>(), perm + (uint64_t)c * n, nq, smins,
+                                                   smaxs);
   FKD_TRY(launch_check("permute_boxes"));
   t->q_plan_nq = nq;
   return FKD_OK;
@@ -401,7 +406,7 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
     FKD_TRY(plan_queries<DIM>(t, nq, mins, maxs, s));
     fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
     const size_t n = std::max<uint32_t>(nq, 1);
-    a.perm = t->q_perm.as<uint32_t>();
+    a.perm = plan_perm(t);
     a.mins = t->q_sbox.as<float>();
     a.maxs = a.mins + (uint64_t)DIM * n;
     a.counts = t->q_counts.as<uint32_t>();
@@ -444,7 +449,7 @@ int search_single_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const floa
     FKD_TRY(plan_queries<DIM>(t, nq, mins, maxs, s));
     fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
     const size_t n = std::max<uint32_t>(nq, 1);
-    a.perm = t->q_perm.as<uint32_t>();
+    a.perm = plan_perm(t);
     a.mins = t->q_sbox.as<float>();
     a.maxs = a.mins + (uint64_t)DIM * n;
     a.counts = t->q_counts.as<uint32_t>();
@@ -466,7 +471,7 @@ int search_single_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const floa
       t->q_counts.as<uint32_t>(), qoff, nq, t->desc.as<uint64_t>(), t->ctr.as<uint32_t>());
   FKD_TRY(launch_check("search_scan"));
   if (nq > 0) {
-    fkd::gather_offsets_kernel<<<grid_for(nq), 256, 0, s>>>(qoff, t->q_perm.as<uint32_t>(), nq,
+    fkd::gather_offsets_kernel<<<grid_for(nq), 256, 0, s>>>(qoff, plan_perm(t), nq,
                                                              t->q_soff.as<uint64_t>());
     FKD_TRY(launch_check("gather_offsets"));
     fkd::compact_kernel<<<4096, fkd::kThreads, 0, s>>>(t->pool.as<uint32_t>(), t->pool_meta.as<uint2>(),
@@ -501,7 +506,7 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
     return fail(FKD_ERR_STATE, "fkd_search_fill must follow fkd_search_count with the same queries");
   fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
   const size_t n = std::max<uint32_t>(nq, 1);
-  a.perm = t->q_perm.as<uint32_t>();
+  a.perm = plan_perm(t);
   a.mins = t->q_sbox.as<float>();
   a.maxs = a.mins + (uint64_t)DIM * n;
   a.qoff = qoff;
@@ -886,7 +891,7 @@ int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
   for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
                     &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
                     &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta, &t->q_counts_slot, &t->q_soff,
-                    &t->q_bwork, &t->q_border})
+                    &t->q_bwork, &t->q_border, &t->q_aos})
     b->release();
   t->desc.release();
   FKD_TRY(t->desc.ensure(256));
@@ -905,7 +910,7 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
                           &t->planes, &t->F, &t->counts, &t->offsets, &t->desc, &t->ctr, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
                           &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta,
-                          &t->pool_ctr, &t->q_counts_slot, &t->q_soff, &t->q_bwork, &t->q_border})
+                          &t->pool_ctr, &t->q_counts_slot, &t->q_soff, &t->q_bwork, &t->q_border, &t->q_aos})
     b += d->bytes;
   *bytes = b;
   return FKD_OK;

@@ -429,12 +429,16 @@ __global__ void gather_offsets_kernel(const uint64_t* __restrict__ qoff, const u
 }
 
 // Spatial sort key of every query: Morton interleave of its box centre,
-// quantised to kb bits per dimension inside the tree's bounding box.
+// quantised to kb bits per dimension inside the tree's bounding box.  Also
+// writes the boxes as 32-byte records (mins, then maxs, each padded to 4), so
+// that the permutation gathers one record per box instead of 2 DIM floats.
 // Any key order gives the same results; this one only groups similar work.
 template <int DIM>
 __global__ void query_keys_kernel(const float* __restrict__ mins, const float* __restrict__ maxs,
                                   uint32_t nq, const uint32_t* __restrict__ bbox, uint32_t kb,
-                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                  float4* __restrict__ aos) {
+  static_assert(DIM <= 4, "box records hold 4 floats per bound");
   float lo[DIM], sc[DIM];
   const float cells = (float)(1u << kb);
 #pragma unroll
@@ -448,9 +452,12 @@ __global__ void query_keys_kernel(const float* __restrict__ mins, const float* _
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
     uint32_t key = 0;
     uint32_t u[DIM];
+    float b[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int m = 0; m < DIM; ++m) {
-      const float c = 0.5f * mins[(uint64_t)m * nq + i] + 0.5f * maxs[(uint64_t)m * nq + i];
+      b[m] = mins[(uint64_t)m * nq + i];
+      b[4 + m] = maxs[(uint64_t)m * nq + i];
+      const float c = 0.5f * b[m] + 0.5f * b[4 + m];
       float f = (c - lo[m]) * sc[m];
       f = (f >= 0.f) ? f : 0.f;  // also maps NaN to 0
       f = f < cells - 1.f ? f : cells - 1.f;
@@ -461,20 +468,23 @@ __global__ void query_keys_kernel(const float* __restrict__ mins, const float* _
       for (int m = 0; m < DIM; ++m) key |= ((u[m] >> b) & 1u) << (b * DIM + m);
     keys[i] = key;
     vals[i] = i;
+    aos[2 * (uint64_t)i] = make_float4(b[0], b[1], b[2], b[3]);
+    aos[2 * (uint64_t)i + 1] = make_float4(b[4], b[5], b[6], b[7]);
   }
 }
 
 // Boxes in processing order: smin[d][i] = mins[d][perm[i]].
 template <int DIM>
-__global__ void permute_boxes_kernel(const float* __restrict__ mins, const float* __restrict__ maxs,
-                                     const uint32_t* __restrict__ perm, uint32_t nq,
-                                     float* __restrict__ smins, float* __restrict__ smaxs) {
+__global__ void permute_boxes_kernel(const float4* __restrict__ aos, const uint32_t* __restrict__ perm,
+                                     uint32_t nq, float* __restrict__ smins, float* __restrict__ smaxs) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
     const uint32_t q = perm[i];
+    const float4 lo = aos[2 * (uint64_t)q], hi = aos[2 * (uint64_t)q + 1];
+    const float l[4] = {lo.x, lo.y, lo.z, lo.w}, h[4] = {hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
     for (int m = 0; m < DIM; ++m) {
-      smins[(uint64_t)m * nq + i] = mins[(uint64_t)m * nq + q];
-      smaxs[(uint64_t)m * nq + i] = maxs[(uint64_t)m * nq + q];
+      smins[(uint64_t)m * nq + i] = l[m];
+      smaxs[(uint64_t)m * nq + i] = h[m];
     }
   }
 }
