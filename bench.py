@@ -7,8 +7,7 @@ grid; 1,048,576 closed boxes of half-width 0.01 centred on random points.
 
 A timed step is one box search of all queries against the resident tree:
 spatial ordering of the boxes, count pass, exclusive scan, fill pass, writing
-every hit id (CSR output).  The single-pass form (one traversal into a chunk
-pool + compaction) is timed beside it.
+every hit id (CSR output), all enqueued without a host round trip.
 ``value`` is box queries per second over the timed steps.  Builds are timed
 in their own loop (``builds_per_s``).  Inputs and outputs stay in HBM; timing
 uses HIP events on the stream the kernels run on.
@@ -120,9 +119,6 @@ def main():
     hits = d_cnt.download(stream).astype(np.int64)
     assert int(hits.sum()) == total
 
-    def single():
-        tree.search_single_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
-
     def two_pass():
         tree.search_device(q, d_mins.ptr, d_maxs.ptr, d_off.ptr, d_out.ptr, total, stream=stream)
 
@@ -144,11 +140,7 @@ def main():
         kms = sum(t.ms() for t in ktimers) / args.steps if traced else None
         return step_ms, wall, kms
 
-    single_ms, _, pool_kernel_ms = timed(single, True)
-    total2, ok2 = tree.search_status(stream)
-    assert ok2 and total2 == total
     search_ms, wall_ms, fill_ms = timed(two_pass, True)
-    two_pass_ms = search_ms
 
     V = int(visited.sum())
     H = total
@@ -183,10 +175,8 @@ def main():
         "build_ms": build_ms,
         "search_ms": search_ms,
         "search_wall_ms": wall_ms,
-        "search_mode": "two-pass (count + scan + fill)",
-        "single_pass_search_ms": single_ms,
+        "search_mode": "two-pass (plan + count + scan + fill, one enqueue)",
         "fill_kernel_ms": fill_ms,
-        "single_pass_traversal_kernel_ms": pool_kernel_ms,
         "hits": H,
         "visited_nodes": V,
         "search_algorithmic_GBs": search_bytes / (search_ms * 1e-3) / 1e9,

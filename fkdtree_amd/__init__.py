@@ -261,12 +261,13 @@ class _TreeBase:
 
     def search_single_device(self, nq: int, d_mins: int, d_maxs: int, d_offsets: int, d_out: Optional[int],
                              capacity: int, d_qev: Optional[int] = None, stream=None) -> None:
-        """Enqueue the single-pass search (fkd_search) on device buffers."""
+        """Enqueue the whole search (fkd_search: plan, count, scan, fill) on
+        device buffers; `search_status` tells whether `capacity` sufficed."""
         s = _sh(stream if stream is not None else self.stream)
         check(lib().fkd_search(self._h, nq, d_mins, d_maxs, d_qev, d_offsets, d_out, capacity, s))
 
     def search_status(self, stream=None) -> Tuple[int, bool]:
-        """(total hits, complete) of the last single-pass search (synchronises)."""
+        """(total hits, complete) of the last one-call search (synchronises)."""
         tot = C.c_uint64()
         ok = C.c_int()
         s = _sh(stream if stream is not None else self.stream)
@@ -274,7 +275,7 @@ class _TreeBase:
         return tot.value, bool(ok.value)
 
     def set_tuning(self, **params) -> None:
-        """Launch parameters (fkd_set_tuning): query_key_bits, pool_batch."""
+        """Launch parameters (fkd_set_tuning): query_key_bits, fill_order."""
         for k, v in params.items():
             check(lib().fkd_set_tuning(self._h, k.encode(), int(v)))
 
@@ -373,7 +374,7 @@ class FKDEvents(_TreeBase):
 
 
 class Graph:
-    """Build + single-pass search of fixed device boxes captured as one HIP
+    """Build + one-call search of fixed device boxes captured as one HIP
     graph (fkd_graph_create); launch() replays the whole pipeline."""
 
     def __init__(self, tree: _TreeBase, nq: int, d_mins: int, d_maxs: int, d_offsets: int, d_out: int,

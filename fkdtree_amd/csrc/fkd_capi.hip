@@ -128,7 +128,6 @@ struct Tiling {
 }  // namespace
 
 struct fkd_tuning {
-  uint32_t pool_batch = 32;     // single-pass search: chunks a wave reserves per atomic
   uint32_t query_key_bits = 8;  // search: Morton bits per dimension of the box order
   uint32_t fill_order = 1;      // fill pass: heaviest search blocks first
 };
@@ -150,10 +149,7 @@ struct fkd_tree_s {
   DevBuf counts, offsets, desc, ctr, err, bbox;
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev, q_aos;  // search scratch
   DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets, q_desc;
-  DevBuf pool, pool_meta, pool_ctr;  // single-pass search chunk pool
-  DevBuf q_counts_slot, q_soff;
   DevBuf q_bwork, q_border;    // fill pass: work per search block and dispatch order
-  uint32_t pool_chunks = 0;
   uint64_t last_capacity = 0;
   uint64_t* last_offsets = nullptr;
   hipEvent_t trace_start = nullptr, trace_stop = nullptr;  // around the traversal kernel
@@ -421,71 +417,6 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
   return launch_check("search_scan");
 }
 
-// pool_chunks counts the chunks of ONE region (the pool has kPoolRegions).
-int grow_pool(fkd_tree_s* t, uint64_t region_chunks) {
-  region_chunks = std::min<uint64_t>(region_chunks, (0xffffffffull / fkd::kStage) / fkd::kPoolRegions);
-  if (region_chunks <= t->pool_chunks && t->pool.p) return FKD_OK;
-  const uint64_t chunks = region_chunks * fkd::kPoolRegions;
-  FKD_TRY(t->pool.ensure((size_t)chunks * fkd::kStage * 4));
-  FKD_TRY(t->pool_meta.ensure((size_t)chunks * 8));
-  FKD_TRY(t->pool_ctr.ensure(fkd::kPoolRegions * 4));
-  t->pool_chunks = (uint32_t)region_chunks;
-  return FKD_OK;
-}
-
-// Single pass: traversal into the chunk pool, scan of the counts, compaction.
-template <int DIM>
-int search_single_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
-                       uint64_t* qoff, uint32_t* out, uint64_t cap, hipStream_t s) {
-  FKD_TRY(t->q_counts.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
-  FKD_TRY(t->q_counts_slot.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
-  FKD_TRY(t->q_soff.ensure((size_t)std::max<uint32_t>(nq, 1) * 8));
-  // region size: at least twice the boxes' share plus two reservations
-  const uint64_t want = std::max<uint64_t>(((uint64_t)nq * 2) / fkd::kPoolRegions + 2 * t->tune.pool_batch,
-                                           t->pool_chunks);
-  FKD_TRY(grow_pool(t, want));
-  FKD_TRY(fill_u32(t->pool_ctr.as<uint32_t>(), 0, fkd::kPoolRegions, s));
-  if (nq > 0) {
-    FKD_TRY(plan_queries<DIM>(t, nq, mins, maxs, s));
-    fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
-    const size_t n = std::max<uint32_t>(nq, 1);
-    a.perm = plan_perm(t);
-    a.mins = t->q_sbox.as<float>();
-    a.maxs = a.mins + (uint64_t)DIM * n;
-    a.counts = t->q_counts.as<uint32_t>();
-    a.pool = t->pool.as<uint32_t>();
-    a.chunk_meta = t->pool_meta.as<uint2>();
-    a.pool_ctr = t->pool_ctr.as<uint32_t>();
-    a.region_chunks = t->pool_chunks;
-    a.pool_batch = t->tune.pool_batch;
-    a.counts_slot = t->q_counts_slot.as<uint32_t>();
-    if (t->trace_start) FKD_HIP(hipEventRecord(t->trace_start, s));
-    FKD_TRY((launch_search<DIM, fkd::kPool>(t, a, s, "search_pool")));
-    if (t->trace_stop) FKD_HIP(hipEventRecord(t->trace_stop, s));
-  }
-  const uint64_t st = fkd::scan_tiles(nq);
-  FKD_TRY(t->desc.ensure(st * 8));
-  FKD_TRY(fill_u64(t->desc.as<uint64_t>(), 0, st, s));
-  FKD_TRY(fill_u32(t->ctr.as<uint32_t>(), 0, 1, s));
-  fkd::scan_exclusive_kernel<uint64_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
-      t->q_counts.as<uint32_t>(), qoff, nq, t->desc.as<uint64_t>(), t->ctr.as<uint32_t>());
-  FKD_TRY(launch_check("search_scan"));
-  if (nq > 0) {
-    fkd::gather_offsets_kernel<<<grid_for(nq), 256, 0, s>>>(qoff, plan_perm(t), nq,
-                                                             t->q_soff.as<uint64_t>());
-    FKD_TRY(launch_check("gather_offsets"));
-    fkd::compact_kernel<<<4096, fkd::kThreads, 0, s>>>(t->pool.as<uint32_t>(), t->pool_meta.as<uint2>(),
-                                                      t->pool_ctr.as<uint32_t>(), t->pool_chunks, nq,
-                                                      t->q_counts_slot.as<uint32_t>(), t->q_soff.as<uint64_t>(),
-                                                      out, cap);
-    FKD_TRY(launch_check("compact"));
-  }
-  t->last_nq = nq;
-  t->last_capacity = cap;
-  t->last_offsets = qoff;
-  return FKD_OK;
-}
-
 template <int DIM>
 int search_stats_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                       uint32_t* counts, uint32_t* visited, hipStream_t s) {
@@ -526,6 +457,20 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
   if (t->trace_start) FKD_HIP(hipEventRecord(t->trace_start, s));
   FKD_TRY((launch_search<DIM, fkd::kFill>(t, a, s, "search_fill")));
   if (t->trace_stop) FKD_HIP(hipEventRecord(t->trace_stop, s));
+  return FKD_OK;
+}
+
+// fkd_search: count pass, scan and fill pass back to back on the stream (the
+// fill pass reads the offsets on the device, no host round trip); ids beyond
+// `cap` are dropped and reported by fkd_search_status.
+template <int DIM>
+int search_single_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                       uint64_t* qoff, uint32_t* out, uint64_t cap, hipStream_t s) {
+  FKD_TRY(search_count_impl<DIM>(t, nq, mins, maxs, qev, qoff, s));
+  FKD_TRY(search_fill_impl<DIM>(t, nq, mins, maxs, qev, qoff, out, cap, s));
+  t->last_nq = nq;
+  t->last_capacity = cap;
+  t->last_offsets = qoff;
   return FKD_OK;
 }
 
@@ -821,15 +766,10 @@ int fkd_search_status(fkd_tree_t t, fkd_stream_t stream, uint64_t* total, int* c
   if (!t->last_offsets) return fail(FKD_ERR_STATE, "no fkd_search to report on");
   hipStream_t s = static_cast<hipStream_t>(stream);
   uint64_t tot = 0;
-  std::vector<uint32_t> used(fkd::kPoolRegions, 0);
   FKD_HIP(hipMemcpyAsync(&tot, t->last_offsets + t->last_nq, 8, hipMemcpyDeviceToHost, s));
-  FKD_HIP(hipMemcpyAsync(used.data(), t->pool_ctr.p, fkd::kPoolRegions * 4, hipMemcpyDeviceToHost, s));
   FKD_HIP(hipStreamSynchronize(s));
-  const uint32_t umax = t->last_nq ? *std::max_element(used.begin(), used.end()) : 0u;
-  const bool pool_ok = umax <= t->pool_chunks;
-  if (!pool_ok) FKD_TRY(grow_pool(t, (uint64_t)umax + umax / 4 + 2 * t->tune.pool_batch));
   if (total) *total = tot;
-  if (complete) *complete = (pool_ok && tot <= t->last_capacity) ? 1 : 0;
+  if (complete) *complete = tot <= t->last_capacity ? 1 : 0;
   return FKD_OK;
 }
 
@@ -856,7 +796,7 @@ int fkd_graph_create(fkd_tree_t t, uint32_t nq, const float* mins, const float* 
       return fail(FKD_ERR_CAPACITY, "capacity " + std::to_string(capacity) + " < " + std::to_string(total) + " hits");
     }
   }
-  if (!complete) return fail(FKD_ERR_CAPACITY, "search pool did not settle");
+  if (!complete) return fail(FKD_ERR_CAPACITY, "search did not complete");
   fkd_graph_s* g = new fkd_graph_s();
   int rc = FKD_OK;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -890,12 +830,10 @@ int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
   FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
                     &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
-                    &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta, &t->q_counts_slot, &t->q_soff,
-                    &t->q_bwork, &t->q_border, &t->q_aos})
+                    &t->q_roffsets, &t->q_desc, &t->q_bwork, &t->q_border, &t->q_aos})
     b->release();
   t->desc.release();
   FKD_TRY(t->desc.ensure(256));
-  t->pool_chunks = 0;
   t->q_plan_nq = 0;
   t->last_offsets = nullptr;
   t->last_total = 0;
@@ -909,8 +847,8 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
                           &t->level_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R,
                           &t->planes, &t->F, &t->counts, &t->offsets, &t->desc, &t->ctr, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
-                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_desc, &t->pool, &t->pool_meta,
-                          &t->pool_ctr, &t->q_counts_slot, &t->q_soff, &t->q_bwork, &t->q_border, &t->q_aos})
+                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_desc, &t->q_bwork, &t->q_border,
+                          &t->q_aos})
     b += d->bytes;
   *bytes = b;
   return FKD_OK;
@@ -919,10 +857,7 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
 int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
   if (!t || !key) return fail(FKD_ERR_INVALID, "tree / key is NULL");
   const std::string k(key);
-  if (k == "pool_batch") {
-    if (value < 1 || value > 1024) return fail(FKD_ERR_INVALID, "pool_batch must be 1..1024");
-    t->tune.pool_batch = value;
-  } else if (k == "query_key_bits") {
+  if (k == "query_key_bits") {
     if (value > 32u / (uint32_t)t->dim) return fail(FKD_ERR_INVALID, "query_key_bits * dim must be <= 32");
     t->tune.query_key_bits = value;
   } else if (k == "fill_order") {
@@ -937,8 +872,7 @@ int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
 int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
   if (!t || !key || !value) return fail(FKD_ERR_INVALID, "NULL argument");
   const std::string k(key);
-  if (k == "pool_batch") *value = t->tune.pool_batch;
-  else if (k == "query_key_bits") *value = t->tune.query_key_bits;
+  if (k == "query_key_bits") *value = t->tune.query_key_bits;
   else if (k == "fill_order") *value = t->tune.fill_order;
   else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;

@@ -32,25 +32,13 @@ struct SearchArgs {
   const float* mins;      // [DIM][nq], indexed by processing slot
   const float* maxs;
   uint32_t* counts;       // count pass output (query order)
-  uint32_t* counts_slot;  // pool mode: counts in processing order
   uint32_t* visited;      // stats pass: nodes visited per query
   const uint64_t* qoff;   // fill pass: first output slot of every query
   uint32_t* out;          // fill pass output
   uint64_t cap;           // capacity of out
   uint32_t out_aligned;   // out is 64-byte aligned
   uint32_t node_bytes;    // size of the node records (buffer resource range)
-  // single-pass mode: hits go to 16-id chunks of a pool split into
-  // kPoolRegions regions of region_chunks chunks; block b allocates in
-  // region b % kPoolRegions, a wave reserves pool_batch chunks at a time.
-  uint32_t* pool;         // kPoolRegions * region_chunks * 16 ids
-  uint2* chunk_meta;      // per chunk: (processing slot, index of the chunk in its query)
-  uint32_t* pool_ctr;     // per region: chunks reserved (may exceed region_chunks)
-  uint32_t region_chunks;
-  uint32_t pool_batch;    // chunks a wave reserves per atomic
 };
-
-constexpr uint32_t kPoolRegions = 256;
-constexpr uint32_t kNoQuery = 0xffffffffu;
 
 // Workgroups are handed to the 8 XCDs in turn.  Workgroup b of a search
 // launch takes position start(b % 8) + b / 8, so that every XCD walks one
@@ -107,9 +95,7 @@ __device__ inline void load_rec(const NodeSource& src, uint32_t rec, float (&c)[
   }
 }
 
-enum SearchMode { kCount = 0, kFill = 1, kStats = 2, kPool = 3 };
-
-constexpr int kStage = 16;  // ids staged per lane: one 64-byte line
+enum SearchMode { kCount = 0, kFill = 1, kStats = 2 };
 
 // Fill staging: hits go to a ring of kRing ids per lane in LDS, ring index
 // i holding the id for output slot wb + i (wb = the query's first output slot
@@ -118,21 +104,18 @@ constexpr int kStage = 16;  // ids staged per lane: one 64-byte line
 // stores, so that stores are issued a few times per wave instead of once per
 // lane and line.
 constexpr int kRing = 16;
-constexpr int kRingRow = 20;  // words per lane: 16-byte aligned rows, spread over the banks
+constexpr int kRingRow = 16;  // words per lane (16-byte aligned rows)
 
 // The loop body is branch free except for writing out staged ids: the next
 // node is selected among (left son, right son, pending right son of the
 // deepest pending depth) with conditional moves.  Counting reads the split
 // value from the record and the box bounds on the split dimension from LDS
-// (DIM >= 3), so no per-lane select on the dimension is needed; emitting
-// passes read the id from the record and select the split value.
+// (DIM >= 3), so no per-lane select on the dimension is needed; the fill
+// pass reads the id from the record and selects the split value.
 template <int DIM, int MODE, bool BUF>
 __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   constexpr bool FILL = MODE == kFill;
-  constexpr bool POOL = MODE == kPool;
-  constexpr bool STAGE = FILL || POOL;
-  constexpr bool LDS_BOX = DIM >= 3 && !STAGE;
-  __shared__ uint32_t stage[POOL ? kThreads : 1][kStage + 1];
+  constexpr bool LDS_BOX = DIM >= 3;
   __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
   __shared__ float2 s_box[LDS_BOX ? DIM : 1][LDS_BOX ? kThreads : 1];
   const uint32_t slot = search_block(blockIdx.x, gridDim.x, a.block_order) * kThreads + threadIdx.x;
@@ -151,18 +134,18 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     mx[m] = a.maxs[(uint64_t)m * a.nq + slot];
     if (LDS_BOX) s_box[LDS_BOX ? m : 0][LDS_BOX ? threadIdx.x : 0] = make_float2(mn[m], mx[m]);
   }
-  const float* recs = STAGE ? a.nodes_id : a.nodes;
+  const float* recs = FILL ? a.nodes_id : a.nodes;
   NodeSource src;
   src.flat = recs;
   if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)a.node_bytes, 0x00020000);
   uint32_t c = 0;
-  uint32_t* st = stage[POOL ? threadIdx.x : 0];
   uint32_t* rw = &ring[FILL ? threadIdx.x * kRingRow : 0];
   const uint64_t w0 = FILL ? a.qoff[q] : 0;
   const uint64_t wb = w0 & ~(uint64_t)3;
   const uint32_t i0 = (uint32_t)w0 & 3u;  // first ring index of this query
   uint32_t wr = i0;                       // next ring index
   uint32_t fq = 0;                        // first ring index not written out (multiple of 4)
+  uint32_t wlim = kRing;                  // ring index at which the ring is full
   // Writes the complete 4-id groups; the query's first group is partial when
   // its first slot is not 16-byte aligned.
   auto write_groups = [&]() {
@@ -180,58 +163,6 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
       fq += 4u;
     }
   };
-  uint32_t nb = 0;      // pool mode: ids staged
-  uint32_t nchunk = 0;  // chunks of this query handed out so far (pool mode)
-  const uint32_t lane = lane_id();
-  const uint32_t wid = threadIdx.x / kWave;
-  __shared__ uint32_t s_wbase[POOL ? kThreads / kWave : 1], s_wleft[POOL ? kThreads / kWave : 1];
-  const uint32_t region = blockIdx.x % kPoolRegions;
-  if (POOL && lane == 0) {
-    s_wbase[wid] = 0;
-    s_wleft[wid] = 0;
-  }
-  // Chunk allocation for the lanes with `need` set; called by every active
-  // lane of the wave.  The wave's current reservation lives in LDS so that
-  // lanes that already left the traversal loop see it too.
-  auto pool_flush = [&](bool need) {
-    const uint64_t m = __ballot(need);
-    if (m == 0ull) return;
-    const uint32_t cnt = (uint32_t)__popcll(m);
-    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-    uint32_t wbase = s_wbase[wid], wleft = s_wleft[wid];
-    if (cnt > wleft) {
-      const uint32_t grab = cnt > a.pool_batch ? cnt : a.pool_batch;
-      uint32_t base = 0;
-      if (lane == leader) {
-        base = atomicAdd(&a.pool_ctr[region], grab);
-        // mark the unused rest of the old reservation
-        for (uint32_t i = 0; i < wleft; ++i) {
-          const uint32_t k = wbase + i;
-          if (k < a.region_chunks)
-            a.chunk_meta[(uint64_t)region * a.region_chunks + k] = make_uint2(kNoQuery, 0u);
-        }
-      }
-      wbase = __shfl(base, (int)leader);
-      wleft = grab;
-    }
-    if (lane == leader) {
-      s_wbase[wid] = wbase + cnt;
-      s_wleft[wid] = wleft - cnt;
-    }
-    if (need) {
-      const uint32_t k = wbase + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-      if (k < a.region_chunks) {
-        const uint64_t ch = (uint64_t)region * a.region_chunks + k;
-        uint4* dst = reinterpret_cast<uint4*>(a.pool + ch * kStage);
-#pragma unroll
-        for (int i = 0; i < kStage / 4; ++i)
-          dst[i] = make_uint4(st[4 * i], st[4 * i + 1], st[4 * i + 2], st[4 * i + 3]);
-        a.chunk_meta[ch] = make_uint2(slot, nchunk);
-      }
-      ++nchunk;
-      nb = 0;
-    }
-  };
   // node has a left (right) son iff node < tl (tr), i.e. 2 node + 1 (2) < n
   const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
   uint32_t node = 0, depth = 0, pending = 0, vis = 0;
@@ -241,15 +172,18 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     float x[DIM], extra;
     load_rec<DIM, BUF>(src, base + node, x, extra);
     const uint32_t dd = depth_dim<DIM>(depth);
-    float v = STAGE ? x[0] : extra, lo = mn[0], hi = mx[0];
+    float v = FILL ? x[0] : extra, lo = mn[0], hi = mx[0];
     if (LDS_BOX) {
       const float2 bx = s_box[LDS_BOX ? dd : 0][LDS_BOX ? threadIdx.x : 0];
       lo = bx.x;
       hi = bx.y;
+#pragma unroll
+      for (int m = 1; m < DIM; ++m)
+        if (FILL) v = dd == (uint32_t)m ? x[m] : v;
     } else {
 #pragma unroll
       for (int m = 1; m < DIM; ++m) {
-        if (STAGE) v = dd == (uint32_t)m ? x[m] : v;
+        if (FILL) v = dd == (uint32_t)m ? x[m] : v;
         lo = dd == (uint32_t)m ? mn[m] : lo;
         hi = dd == (uint32_t)m ? mx[m] : hi;
       }
@@ -263,12 +197,10 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     if (FILL) {
       rw[wr & (kRing - 1)] = id;  // the slot is free; it is kept only for a hit
       wr += in ? 1u : 0u;
-      if (__ballot(wr - fq == (uint32_t)kRing) != 0ull) write_groups();
-    } else if (POOL) {
-      st[nb] = id;
-      nb += in ? 1u : 0u;
-      c += in ? 1u : 0u;
-      pool_flush(nb == (uint32_t)kStage);
+      if (__ballot(wr == wlim) != 0ull) {
+        write_groups();
+        wlim = fq + (uint32_t)kRing;
+      }
     } else {
       c += in ? 1u : 0u;
     }
@@ -290,21 +222,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     for (uint32_t j = fq > i0 ? fq : i0; j < wr; ++j)
       if (wb + j < a.cap) a.out[wb + j] = rw[j & (kRing - 1)];
   }
-  if (POOL) {
-    pool_flush(nb != 0u);
-    // mark the unused rest of this wave's reservation (by one lane: the
-    // wave may be partly empty)
-    const uint32_t first_lane = (uint32_t)__builtin_ctzll(__ballot(true));
-    if (lane == first_lane) {
-      const uint32_t wb = s_wbase[wid], wl = s_wleft[wid];
-      for (uint32_t i = 0; i < wl; ++i) {
-        const uint32_t k = wb + i;
-        if (k < a.region_chunks) a.chunk_meta[(uint64_t)region * a.region_chunks + k] = make_uint2(kNoQuery, 0u);
-      }
-    }
-  }
   if (!FILL) a.counts[q] = c;
-  if (POOL) a.counts_slot[slot] = c;
   if (MODE == kStats) a.visited[q] = vis;
 }
 
@@ -366,66 +284,6 @@ __global__ __launch_bounds__(1024) void block_order_kernel(const uint32_t* __res
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
     order[atomicAdd(&cur[chunk_of(b)][work_bucket(work[b])], 1u)] = b;
-}
-
-// Copies the pooled chunks into the query-ordered CSR: slot t of chunk k
-// (processing slot s, k-th chunk j of its query) goes to
-// out[soff[s] + 16 j + t] when 16 j + t < counts[s]; soff / counts are in
-// processing order, so neighbouring chunks read neighbouring entries.  Region r's used chunks are its first
-// min(pool_ctr[r], region_chunks); reserved but unused ones carry kNoQuery.
-// Every block forms the prefix sum of the used chunks per region in LDS and
-// walks only used chunks, 16 threads per chunk.
-__global__ __launch_bounds__(kThreads) void compact_kernel(const uint32_t* __restrict__ pool,
-                                                           const uint2* __restrict__ meta,
-                                                           const uint32_t* __restrict__ pool_ctr,
-                                                           uint32_t region_chunks, uint32_t nq,
-                                                           const uint32_t* __restrict__ counts,
-                                                           const uint64_t* __restrict__ soff,
-                                                           uint32_t* __restrict__ out, uint64_t cap) {
-  static_assert(kPoolRegions == kThreads, "one thread per region");
-  __shared__ uint32_t pre[kPoolRegions + 1];
-  __shared__ uint32_t s_wave[kThreads / kWave];
-  const uint32_t used = min(pool_ctr[threadIdx.x], region_chunks);
-  const uint32_t incl = wave_incl_sum(used);
-  const int wave = threadIdx.x / kWave;
-  if (lane_id() == kWave - 1) s_wave[wave] = incl;
-  __syncthreads();
-  uint32_t woff = 0;
-#pragma unroll
-  for (int w = 0; w < kThreads / kWave; ++w) woff += (w < wave) ? s_wave[w] : 0u;
-  pre[threadIdx.x + 1] = woff + incl;
-  if (threadIdx.x == 0) pre[0] = 0;
-  __syncthreads();
-  const uint32_t chunks = pre[kPoolRegions];
-  for (uint32_t kc = blockIdx.x * kThreads + threadIdx.x; kc < chunks; kc += gridDim.x * kThreads) {
-    uint32_t lo = 0, hi = kPoolRegions;  // largest r with pre[r] <= kc
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (pre[mid] <= kc) lo = mid; else hi = mid;
-    }
-    const uint64_t k = (uint64_t)lo * region_chunks + (kc - pre[lo]);
-    const uint2 mq = meta[k];
-    const uint4* src = reinterpret_cast<const uint4*>(pool + k * kStage);
-    uint4 v[kStage / 4];
-#pragma unroll
-    for (int i = 0; i < kStage / 4; ++i) v[i] = src[i];
-    if (mq.x >= nq) continue;  // kNoQuery: reserved but unused
-    const uint32_t pos = mq.y * kStage;
-    const uint32_t cnt = counts[mq.x];
-    const uint64_t dst = soff[mq.x] + pos;
-    const uint32_t m = cnt > pos ? min(cnt - pos, (uint32_t)kStage) : 0u;
-    const uint32_t* ids = reinterpret_cast<const uint32_t*>(v);
-#pragma unroll
-    for (int i = 0; i < kStage; ++i)
-      if ((uint32_t)i < m && dst + i < cap) out[dst + i] = ids[i];
-  }
-}
-
-// Offsets in processing order: soff[i] = qoff[perm[i]].
-__global__ void gather_offsets_kernel(const uint64_t* __restrict__ qoff, const uint32_t* __restrict__ perm,
-                                      uint32_t nq, uint64_t* __restrict__ soff) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
-    soff[i] = qoff[perm[i]];
 }
 
 // Spatial sort key of every query: Morton interleave of its box centre,

@@ -12,7 +12,8 @@ The server drives the GPU from one host thread:
   batches run on the GPU join the next batch: this is continuous batching,
   not a single queue drained batch by batch.
 * A batch is one FKDEvents handle: one build launch per depth for all its
-  events, then one single-pass search of all its boxes, on its own stream.
+  events, then one search of all its boxes (count, scan and fill enqueued
+  together), on its own stream.
   Several batches overlap on the device.
 * When a batch completes, its results are checked for completeness (the
   output is regrown and the search re-run if needed) and its latency is
@@ -92,7 +93,7 @@ class ServeReport:
 def batch_bytes(points: int, hits_per_point: float) -> int:
     """Estimated device footprint of one batch during build + search:
     build scratch and tree (about 300 B per point at dim 3), boxes and search
-    plan (about 100 B per point), result ids and their staging pool."""
+    plan (about 100 B per point), and the result ids."""
     return int(points * (400 + 8 * hits_per_point)) + (64 << 20)
 
 

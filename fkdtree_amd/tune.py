@@ -3,9 +3,9 @@
     python -m fkdtree_amd.tune [--workload mixed_1m] [--reps 3]
 
 Sweeps `query_key_bits` (Morton bits per dimension of the box order; 0
-keeps the caller's order) for the two-pass search and `pool_batch` (chunks a
-wave reserves per atomic) for the single-pass search on a workload, timing
-each with HIP events, and stores the
+keeps the caller's order) and `fill_order` (heaviest search blocks first in
+the fill pass) on a workload, timing the two-pass search with HIP events,
+and stores the
 fastest setting in `tuning.json` next to this file, keyed by GPU
 architecture, dimension and log2 of the point count.  Trees created by this
 package apply the nearest stored entry (`apply`).  Results never depend on
@@ -26,7 +26,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
 CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1)}
-POOL_CANDIDATES = {"pool_batch": (8, 32, 128, 512)}
 
 
 def _arch() -> str:
@@ -68,15 +67,18 @@ def apply(tree, arch: Optional[str] = None) -> Optional[Dict[str, int]]:
         params = lookup(arch or _arch(), tree.dim, max(tree.n // max(tree.n_events, 1), 1))
     except Exception:
         return None
-    if params:
-        tree.set_tuning(**params)
-    return params
+    applied = {}
+    for k, v in (params or {}).items():
+        try:            # keys a newer or older library does not know are skipped
+            tree.set_tuning(**{k: v})
+            applied[k] = v
+        except Exception:
+            pass
+    return applied or None
 
 
-def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=None, d_qev: Optional[int] = None,
-          single: bool = False):
-    """Time the two-pass (or single-pass) search for every candidate;
-    [(params, ms)] sorted."""
+def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=None, d_qev: Optional[int] = None):
+    """Time the two-pass search for every candidate; [(params, ms)] sorted."""
     from . import DeviceArray, Stream, Timer
 
     cands = candidates or CANDIDATES
@@ -93,15 +95,9 @@ def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=Non
         tree.set_tuning(**params)
 
         def run():
-            if single:
-                tree.search_single_device(nq, d_mins, d_maxs, d_off.ptr, d_out.ptr, total, d_qev=d_qev, stream=s)
-            else:
-                tree.search_device(nq, d_mins, d_maxs, d_off.ptr, d_out.ptr, total, d_qev=d_qev, stream=s)
+            tree.search_device(nq, d_mins, d_maxs, d_off.ptr, d_out.ptr, total, d_qev=d_qev, stream=s)
 
         run()
-        if single:
-            tree.search_status(s)   # lets the pool settle
-            run()
         s.sync()
         tm.start(s)
         for _ in range(reps):
@@ -125,14 +121,10 @@ def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path
     tree = FKDTree.from_device(pts.dim, pts.n, d_c.ptr, d_i.ptr, s)
     tree.build()
     res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps)
-    tree.set_tuning(**res[0][0])
-    res_pool = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, candidates=POOL_CANDIDATES, single=True)
     tree.close()
     arch = _arch()
-    params = dict(res[0][0], **res_pool[0][0])
-    entry = {"params": params, "ms": res[0][1], "single_pass_ms": res_pool[0][1], "workload": workload,
-             "sweep": [{"params": p, "ms": ms} for p, ms in res],
-             "sweep_single_pass": [{"params": p, "ms": ms} for p, ms in res_pool]}
+    entry = {"params": res[0][0], "ms": res[0][1], "workload": workload,
+             "sweep": [{"params": p, "ms": ms} for p, ms in res]}
     if write:
         table = load_table(path)
         table.setdefault(arch, {})[_key(pts.dim, pts.n)] = entry
@@ -150,7 +142,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     r = autotune(a.workload, a.reps, write=not a.no_write, path=a.out)
     print(json.dumps({k: r[k] for k in ("arch", "key", "params", "ms")}))
-    for row in r["sweep"] + r["sweep_single_pass"]:
+    for row in r["sweep"]:
         print(json.dumps(row))
 
 

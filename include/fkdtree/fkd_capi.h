@@ -93,8 +93,6 @@ int fkd_trim(fkd_tree_t tree, fkd_stream_t stream);
 
 /* Launch parameters of the handle (chosen by fkdtree_amd/tune.py; results
  * never depend on them):
- *   "pool_batch"      chunks of 16 ids a wave of the single-pass search
- *                     reserves per atomic, 1..1024;
  *   "query_key_bits"  Morton bits per dimension of the box order (0 keeps
  *                     the caller's order), key bits * dim <= 32;
  *   "fill_order"      1: the fill pass starts the search blocks with the
@@ -117,13 +115,13 @@ int fkd_search_count(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const 
                      const uint32_t* dev_query_events, uint64_t* dev_offsets,
                      fkd_stream_t stream);
 
-/* Single-pass search: ids of the points inside box q go to
+/* Whole search in one call: ids of the points inside box q go to
  * dev_ids_out[dev_offsets[q] .. dev_offsets[q+1]), dev_offsets[nq] = total.
- * One traversal per box (hits are staged in a library-owned pool, then
- * compacted into query order).  Enqueues only: fkd_search_status reports
- * whether the results are complete (the internal pool and `capacity` were
- * large enough); when not, it grows the pool and the same call must be
- * repeated (with a capacity of at least the reported total).
+ * Enqueues the count pass, the scan and the fill pass back to back (the fill
+ * pass reads the offsets on the device: no host round trip, so the call can
+ * be captured in a HIP graph).  Ids beyond `capacity` are dropped:
+ * fkd_search_status reports whether the results are complete; when not, the
+ * call must be repeated with a capacity of at least the reported total.
  * Replaces: search_in_the_box (FKDTree_cpu.h:123-169) for nq boxes at once. */
 int fkd_search(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
                const uint32_t* dev_query_events, uint64_t* dev_offsets, uint32_t* dev_ids_out,
@@ -133,7 +131,7 @@ int fkd_search(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float*
  * *complete = 1 when every id was written. */
 int fkd_search_status(fkd_tree_t tree, fkd_stream_t stream, uint64_t* total, int* complete);
 
-/* Build + single-pass search of the same boxes captured as one HIP graph.
+/* Build + one-call search (fkd_search) of the same boxes captured as one HIP graph.
  * Runs the sequence eagerly first (so every internal buffer has its final
  * size; FKD_ERR_CAPACITY when `capacity` is below the hits), then captures
  * it on `stream` (a created stream).  fkd_graph_launch replays it: a full

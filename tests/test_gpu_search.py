@@ -98,24 +98,24 @@ def _device_search(t, boxes, single: bool, capacity=None):
     return off, out, total, True, (dmi, dma)
 
 
-def test_single_pass_regrows_and_matches_two_pass(require_gpu):
-    """The single-pass search reports incomplete output (capacity 0, then an
-    internal pool too small for ~100 hits per box), grows, and then matches
-    the two-pass search and the oracle exactly."""
+def test_one_call_search_capacity_and_parity(require_gpu):
+    """fkd_search (plan, count, scan and fill in one enqueue) reports
+    incomplete output for too small a capacity, writes exactly the first
+    `capacity` ids of the full result, and with enough capacity matches the
+    two-pass calls and the oracle exactly."""
     p = adversarial_points("mixed", 60_000)
     boxes = self_boxes(p, 0.03)
-    print(f"[search] single-pass regrow n={p.n} q={boxes.q}", flush=True)
+    print(f"[search] one-call capacity n={p.n} q={boxes.q}", flush=True)
     t = FKDTree(coords=p.coords, ids=p.ids)
     t.build()
     off2, out2, total2, _, _ = _device_search(t, boxes, single=False)
     _, _, total, ok, _ = _device_search(t, boxes, single=True, capacity=0)
     assert total == total2 and not ok
-    attempts = 0
-    while True:
-        off1, out1, total, ok, keep = _device_search(t, boxes, single=True, capacity=total)
-        attempts += 1
-        if ok or attempts > 3:
-            break
+    cut = total // 3
+    offc, outc, total, ok, keep = _device_search(t, boxes, single=True, capacity=cut)
+    assert total == total2 and not ok
+    assert np.array_equal(outc.download(count=cut), out2.download(count=cut))
+    off1, out1, total, ok, keep = _device_search(t, boxes, single=True, capacity=total)
     assert ok and total == total2
     o1, o2 = off1.download(), off2.download()
     assert np.array_equal(o1, o2)

@@ -1,6 +1,6 @@
-"""Launch parameters change time only: the two-pass and single-pass searches
-return the oracle's id sets (FKDTree_cpu.h:123-169) for every query order
-(`query_key_bits`, 0 = the caller's order) and pool batch size."""
+"""Launch parameters change time only: the two-pass search and the one-call
+search return the oracle's id sets (FKDTree_cpu.h:123-169) for every query
+order (`query_key_bits`, 0 = the caller's order) and fill dispatch order."""
 import numpy as np
 import pytest
 
@@ -20,7 +20,7 @@ def _two_pass(t, dmi, dma, q):
     return o, out.download(count=int(o[-1]))
 
 
-def _single_pass(t, dmi, dma, q, cap):
+def _one_call(t, dmi, dma, q, cap):
     off = DeviceArray(q + 1, np.uint64)
     out = DeviceArray(max(cap, 1), np.uint32)
     for _ in range(3):
@@ -50,21 +50,18 @@ def test_tuning_keeps_results(require_gpu):
         o, ids = _two_pass(t, dmi, dma, boxes.q)
         assert np.array_equal(o, ooff), f"key bits {kb}, fill order {fo}: counts differ"
         assert np.array_equal(sorted_sets(o, ids), want), f"key bits {kb}, fill order {fo}: ids differ"
-    t.set_tuning(query_key_bits=8)
-    for pb in (1, 8, 1024):
-        t.set_tuning(pool_batch=pb)
-        assert t.get_tuning("pool_batch") == pb
-        o, ids = _single_pass(t, dmi, dma, boxes.q, int(ooff[-1]))
-        assert np.array_equal(o, ooff), f"pool batch {pb}: counts differ"
-        assert np.array_equal(sorted_sets(o, ids), want), f"pool batch {pb}: ids differ"
+    for fo in (0, 1):
+        t.set_tuning(fill_order=fo)
+        o, ids = _one_call(t, dmi, dma, boxes.q, int(ooff[-1]))
+        assert np.array_equal(o, ooff), f"single call, fill order {fo}: counts differ"
+        assert np.array_equal(sorted_sets(o, ids), want), f"single call, fill order {fo}: ids differ"
     t.close()
 
 
 def test_tuning_rejects_bad_values(require_gpu):
     p = adversarial_points("mixed", 100)
     t = FKDTree(coords=p.coords, ids=p.ids)
-    for k, v in (("pool_batch", 0), ("pool_batch", 2000), ("query_key_bits", 11), ("fill_order", 2),
-                 ("no_such_key", 1)):
+    for k, v in (("pool_batch", 32), ("query_key_bits", 11), ("fill_order", 2), ("no_such_key", 1)):
         with pytest.raises(RuntimeError):
             t.set_tuning(**{k: v})
     t.close()

@@ -7,9 +7,9 @@ from fkdtree_amd import tune
 
 def test_lookup_nearest_bucket(tmp_path):
     table = {"gfx950": {"dim3_log2n10": {"params": {"query_key_bits": 5}},
-                        "dim3_log2n20": {"params": {"query_key_bits": 8, "pool_batch": 32}},
+                        "dim3_log2n20": {"params": {"query_key_bits": 8, "fill_order": 1}},
                         "dim2_log2n20": {"params": {"query_key_bits": 10}}}}
-    assert tune.lookup("gfx950", 3, 1 << 19, table) == {"query_key_bits": 8, "pool_batch": 32}
+    assert tune.lookup("gfx950", 3, 1 << 19, table) == {"query_key_bits": 8, "fill_order": 1}
     assert tune.lookup("gfx950", 3, 900, table) == {"query_key_bits": 5}
     assert tune.lookup("gfx950", 2, 10, table) == {"query_key_bits": 10}
     assert tune.lookup("gfx950", 4, 1 << 20, table) is None
@@ -26,6 +26,27 @@ def test_keys():
 
 
 def test_candidates_are_valid():
-    # query_key_bits * dim must fit a 32-bit key at dim 3; pool batch 1..1024
+    # query_key_bits * dim must fit a 32-bit key at dim 3; fill_order is 0/1
     assert all(0 <= kb <= 10 for kb in tune.CANDIDATES["query_key_bits"])
-    assert all(1 <= b <= 1024 for b in tune.POOL_CANDIDATES["pool_batch"])
+    assert set(tune.CANDIDATES["fill_order"]) <= {0, 1}
+
+
+class _FakeTree:
+    dim, n, n_events = 3, 1 << 20, 1
+
+    def __init__(self):
+        self.set = {}
+
+    def set_tuning(self, **kw):
+        for k, v in kw.items():
+            if k != "query_key_bits":
+                raise RuntimeError("unknown tuning key " + k)
+            self.set[k] = v
+
+
+def test_apply_skips_unknown_keys(tmp_path, monkeypatch):
+    table = {"gfx950": {"dim3_log2n20": {"params": {"query_key_bits": 7, "retired_key": 3}}}}
+    monkeypatch.setattr(tune, "load_table", lambda path=tune.TABLE: table)
+    t = _FakeTree()
+    assert tune.apply(t, arch="gfx950") == {"query_key_bits": 7}
+    assert t.set == {"query_key_bits": 7}
