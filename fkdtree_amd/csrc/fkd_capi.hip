@@ -146,15 +146,17 @@ struct fkd_tree_s {
   DevBuf X, ids;               // input copy
   DevBuf Tc, Tid, nodes, nodes_id;  // outputs and search records
   DevBuf A, K, R, planes, F;   // build scratch
-  DevBuf counts, offsets, desc, ctr, err, bbox;
+  DevBuf counts, offsets, err, bbox;
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev, q_aos;  // search scratch
-  DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets, q_desc;
+  DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets;
   DevBuf q_bwork, q_border;    // fill pass: work per search block and dispatch order
+  DevBuf zarena;               // look-back descriptors and block counters, zeroed per build / search
   uint64_t last_capacity = 0;
   uint64_t* last_offsets = nullptr;
   hipEvent_t trace_start = nullptr, trace_stop = nullptr;  // around the traversal kernel
   uint32_t q_plan_nq = 0;  // queries whose processing order q_perm holds
   uint32_t q_perm_half = 0;  // which half of q_perm holds it
+  uint64_t* q_scan_desc = nullptr;  // zeroed descriptors for the scan of the count pass
   uint64_t last_nq = 0, last_total = 0;
 };
 
@@ -171,7 +173,25 @@ namespace {
 
 int ensure_build_scratch(fkd_tree_s* t);
 
-// One stable LSD digit pass over M key arrays (count, scan, scatter).
+// Zeroed words for the decoupled look-back scans of one build or search:
+// sized up front, cleared by a single launch, handed out in slices.
+struct ZeroArena {
+  uint64_t* base = nullptr;
+  uint64_t used = 0;
+  uint64_t* take(uint64_t words) {
+    uint64_t* p = base + used;
+    used += words;
+    return p;
+  }
+};
+
+int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za);
+
+// Words one radix pass takes from the arena: scan descriptors + counter.
+uint64_t radix_pass_words(uint32_t M, uint32_t T) { return fkd::scan_tiles((uint64_t)M * 256u * T) + 1; }
+
+// One stable LSD digit pass over M key arrays (count, scan, scatter); desc
+// (scan_tiles of the digit counts) and ctr must be zero.
 int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts, uint32_t* offsets,
                uint64_t* desc, uint32_t* ctr, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
                uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift, bool write_keys,
@@ -181,8 +201,6 @@ int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts,
   fkd::radix_count_kernel<<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts);
   FKD_TRY(launch_check("radix_count"));
   const uint64_t st = fkd::scan_tiles(nc);
-  FKD_TRY(fill_u64(desc, 0, st, s));
-  FKD_TRY(fill_u32(ctr, 0, 1, s));
   fkd::scan_exclusive_kernel<uint32_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(counts, offsets, nc, desc, ctr);
   FKD_TRY(launch_check("radix_scan"));
   if (write_keys && !write_ranks)
@@ -201,12 +219,14 @@ int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts,
 }
 
 template <int DIM>
-int run_radix_pass(fkd_tree_s* t, uint32_t M, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
-                   uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift, bool write_keys,
-                   bool write_ranks, hipStream_t s) {
+int run_radix_pass(fkd_tree_s* t, ZeroArena& za, uint32_t M, const uint32_t* kin, uint32_t* kout,
+                   const uint32_t* vin, uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift,
+                   bool write_keys, bool write_ranks, hipStream_t s) {
   const fkd::TileMap tm = t->radix_tiles.map(t->off.as<uint32_t>());
-  return radix_pass(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), t->desc.as<uint64_t>(),
-                    t->ctr.as<uint32_t>(), kin, kout, vin, vout, stride, ranks, shift, write_keys, write_ranks, s);
+  uint64_t* desc = za.take(radix_pass_words(M, tm.T));
+  uint32_t* ctr = reinterpret_cast<uint32_t*>(desc + fkd::scan_tiles((uint64_t)M * 256u * tm.T));
+  return radix_pass(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, kin, kout, vin,
+                    vout, stride, ranks, shift, write_keys, write_ranks, s);
 }
 
 template <int DIM>
@@ -217,6 +237,15 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     return FKD_OK;
   }
   FKD_TRY(ensure_build_scratch(t));
+  const uint32_t levels = std::max<uint32_t>(1, t->Dmax);
+  const fkd::TileMap tm = t->level_tiles.map(t->off.as<uint32_t>());
+  constexpr uint32_t M = DIM - 1;
+  const uint32_t RT = t->radix_tiles.T;
+  ZeroArena za;
+  FKD_TRY(zero_arena(t,
+                     (t->ids_in_order ? 0 : 4 * radix_pass_words(1, RT)) + 4 * radix_pass_words(DIM, RT) +
+                         (uint64_t)levels * ((uint64_t)M * tm.T + 1),
+                     s, za));
   const uint64_t S = P;  // stride of one array
   uint32_t* A = t->A.as<uint32_t>();  // A[m][2][P]
   uint32_t* K = t->K.as<uint32_t>();  // K[m][2][P]
@@ -235,7 +264,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     int c = 0;
     for (int pass = 0; pass < 4; ++pass) {
       const bool lastp = pass == 3;
-      FKD_TRY(run_radix_pass<DIM>(t, 1, Kbuf(0, c), Kbuf(0, 1 - c), Abuf(0, c), lastp ? R : Abuf(0, 1 - c),
+      FKD_TRY(run_radix_pass<DIM>(t, za, 1, Kbuf(0, c), Kbuf(0, 1 - c), Abuf(0, c), lastp ? R : Abuf(0, 1 - c),
                                   S, nullptr, 8u * pass, !lastp, false, s));
       c = 1 - c;
     }
@@ -250,7 +279,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     int c = 0;
     for (int pass = 0; pass < 4; ++pass) {
       const bool lastp = pass == 3;
-      FKD_TRY(run_radix_pass<DIM>(t, DIM, Kbuf(0, c), Kbuf(0, 1 - c), Abuf(0, c), Abuf(0, 1 - c), 2 * S,
+      FKD_TRY(run_radix_pass<DIM>(t, za, DIM, Kbuf(0, c), Kbuf(0, 1 - c), Abuf(0, c), Abuf(0, 1 - c), 2 * S,
                                   lastp ? R : nullptr, 8u * pass, !lastp, lastp, s));
       c = 1 - c;
     }
@@ -268,9 +297,6 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(launch_check("planes"));
   // 5. One partition launch per depth.
   int cur[4] = {0, 0, 0, 0};
-  const uint32_t levels = std::max<uint32_t>(1, t->Dmax);
-  const fkd::TileMap tm = t->level_tiles.map(t->off.as<uint32_t>());
-  constexpr uint32_t M = DIM - 1;
   for (uint32_t L = 0; L < levels; ++L) {
     fkd::LevelArgs<DIM> a;
     a.L = L;
@@ -299,11 +325,9 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     a.Tid = t->Tid.as<uint32_t>();
     a.nodes = t->nodes.as<float>();
     a.nodes_id = t->nodes_id.as<float>();
-    a.desc = t->desc.as<uint64_t>();
-    a.ctr = t->ctr.as<uint32_t>();
+    a.desc = za.take((uint64_t)M * tm.T + 1);
+    a.ctr = reinterpret_cast<uint32_t*>(a.desc + (uint64_t)M * tm.T);
     const uint32_t blocks = (M ? M : 1u) * tm.T;
-    if (M) FKD_TRY(fill_u64(a.desc, 0, blocks, s));
-    FKD_TRY(fill_u32(a.ctr, 0, 1, s));
     fkd::level_kernel<DIM><<<blocks, fkd::kThreads, 0, s>>>(a);
     FKD_TRY(launch_check("level"));
     for (uint32_t i = 0; i < M; ++i) cur[moved[i]] ^= 1;
@@ -361,7 +385,6 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   const uint64_t nc = 256ull * std::max<uint32_t>(T, 1);
   FKD_TRY(t->q_rcounts.ensure(nc * 4));
   FKD_TRY(t->q_roffsets.ensure((nc + 1) * 4));
-  FKD_TRY(t->q_desc.ensure(std::max<uint64_t>(fkd::scan_tiles(nc), fkd::scan_tiles(nq)) * 8));
   uint32_t* keys = t->q_keys.as<uint32_t>();
   uint32_t* perm = t->q_perm.as<uint32_t>();
   const uint32_t g = grid_for(nq);
@@ -375,16 +398,22 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   tm.T = T;
   tm.single_n = nq;
   const int passes = (int)(kb * DIM + 7) / 8;
+  // one clear for the digit passes and the scan of the count pass
+  ZeroArena za;
+  FKD_TRY(zero_arena(t, passes * radix_pass_words(1, T) + fkd::scan_tiles(nq) + 1, s, za));
   int c = 0;
   for (int pass = 0; pass < passes; ++pass) {
     const bool lastp = pass == passes - 1;
-    FKD_TRY(radix_pass(tm, nq, 1, t->q_rcounts.as<uint32_t>(), t->q_roffsets.as<uint32_t>(),
-                       t->q_desc.as<uint64_t>(), t->ctr.as<uint32_t>(), keys + (uint64_t)c * n,
+    uint64_t* desc = za.take(radix_pass_words(1, T));
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(desc + fkd::scan_tiles(nc));
+    FKD_TRY(radix_pass(tm, nq, 1, t->q_rcounts.as<uint32_t>(), t->q_roffsets.as<uint32_t>(), desc, ctr,
+                       keys + (uint64_t)c * n,
                        keys + (uint64_t)(1 - c) * n, perm + (uint64_t)c * n, perm + (uint64_t)(1 - c) * n, n,
                        nullptr, 8u * pass, !lastp, false, s));
     c = 1 - c;
   }
   t->q_perm_half = (uint32_t)c;
+  t->q_scan_desc = za.take(fkd::scan_tiles(nq) + 1);
   float* smins = t->q_sbox.as<float>();
   float* smaxs = smins + (uint64_t)DIM * n;
   fkd::permute_boxes_kernel<DIM><<<g, 256, 0, s>>>(t->q_aos.as<float4>(), perm + (uint64_t)c * n, nq, smins,
@@ -409,11 +438,14 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
     FKD_TRY((launch_search<DIM, fkd::kCount>(t, a, s, "search_count")));
   }
   const uint64_t st = fkd::scan_tiles(nq);
-  FKD_TRY(t->desc.ensure(st * 8));
-  FKD_TRY(fill_u64(t->desc.as<uint64_t>(), 0, st, s));
-  FKD_TRY(fill_u32(t->ctr.as<uint32_t>(), 0, 1, s));
+  uint64_t* desc = t->q_scan_desc;
+  if (nq == 0) {
+    ZeroArena za;
+    FKD_TRY(zero_arena(t, st + 1, s, za));
+    desc = za.take(st + 1);
+  }
   fkd::scan_exclusive_kernel<uint64_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
-      t->q_counts.as<uint32_t>(), qoff, nq, t->desc.as<uint64_t>(), t->ctr.as<uint32_t>());
+      t->q_counts.as<uint32_t>(), qoff, nq, desc, reinterpret_cast<uint32_t*>(desc + st));
   return launch_check("search_scan");
 }
 
@@ -512,6 +544,13 @@ int validate_offsets(uint32_t n_events, const uint64_t* event_offsets, uint64_t*
 
 // Allocates everything, given the input already in t->X / t->ids; validates.
 // Scratch of the build (released by fkd_trim, re-created on the next build).
+int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za) {
+  FKD_TRY(t->zarena.ensure(std::max<uint64_t>(words, 1) * 8));
+  za.base = t->zarena.as<uint64_t>();
+  za.used = 0;
+  return fill_u64(za.base, 0, words, s);
+}
+
 int ensure_build_scratch(fkd_tree_s* t) {
   const size_t S = std::max<uint32_t>(t->P, 1);
   const int dim = t->dim;
@@ -523,8 +562,6 @@ int ensure_build_scratch(fkd_tree_s* t) {
   const uint64_t nc = (uint64_t)dim * 256u * std::max<uint32_t>(t->radix_tiles.T, 1);
   FKD_TRY(t->counts.ensure(nc * 4));
   FKD_TRY(t->offsets.ensure((nc + 1) * 4));
-  const uint64_t ndesc = std::max<uint64_t>(fkd::scan_tiles(nc), (uint64_t)dim * t->level_tiles.T + 1);
-  FKD_TRY(t->desc.ensure(ndesc * 8));
   return FKD_OK;
 }
 
@@ -549,7 +586,6 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(t->nodes.ensure(S * rf * 4));
   FKD_TRY(t->nodes_id.ensure(S * rf * 4));
   FKD_TRY(ensure_build_scratch(t));
-  FKD_TRY(t->ctr.ensure(64));
   FKD_TRY(t->err.ensure(64));
   FKD_TRY(t->bbox.ensure(64));
   FKD_TRY(fill_u32(t->err.as<uint32_t>(), 0, 1, s));
@@ -830,10 +866,8 @@ int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
   FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
                     &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
-                    &t->q_roffsets, &t->q_desc, &t->q_bwork, &t->q_border, &t->q_aos})
+                    &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_aos, &t->zarena})
     b->release();
-  t->desc.release();
-  FKD_TRY(t->desc.ensure(256));
   t->q_plan_nq = 0;
   t->last_offsets = nullptr;
   t->last_total = 0;
@@ -845,10 +879,10 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
   uint64_t b = 0;
   for (const DevBuf* d : {&t->off, &t->radix_tiles.tile_ev, &t->radix_tiles.tb, &t->level_tiles.tile_ev,
                           &t->level_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R,
-                          &t->planes, &t->F, &t->counts, &t->offsets, &t->desc, &t->ctr, &t->err, &t->bbox,
+                          &t->planes, &t->F, &t->counts, &t->offsets, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
-                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_desc, &t->q_bwork, &t->q_border,
-                          &t->q_aos})
+                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_bwork, &t->q_border,
+                          &t->q_aos, &t->zarena})
     b += d->bytes;
   *bytes = b;
   return FKD_OK;
