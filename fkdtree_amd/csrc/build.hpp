@@ -104,6 +104,15 @@ __global__ void id_keys_kernel(const uint32_t* __restrict__ ids, uint32_t P, uin
 constexpr int kLevelItems = 4;
 constexpr int kLevelTile = kThreads * kLevelItems;
 
+// Depths whose subtrees have at most kFinCap points are built by one
+// workgroup per subtree in LDS (finish_kernel); the global level loop stops
+// above them.  In an event of depth D every subtree of depth L has at most
+// 2^(D-L+1) - 1 points, so the finisher starts at depth fin_depth(D).
+constexpr uint32_t kFinLog = 10;
+constexpr uint32_t kFinCap = (1u << kFinLog) - 1u;
+static_assert(kFinCap <= (uint32_t)kLevelTile, "a finisher subtree fits one pass of the workgroup");
+__host__ __device__ inline uint32_t fin_depth(uint32_t D) { return D + 1u > kFinLog ? D + 1u - kFinLog : 0u; }
+
 template <int DIM>
 struct LevelArgs {
   uint32_t L;                 // depth whose intervals are split
@@ -217,7 +226,7 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   const uint32_t o = a.tm.off[e];
   const uint32_t n = a.tm.off[e + 1] - o;
   const Shape sh = shape_of(n);
-  const bool split = a.L < sh.D;
+  const bool split = a.L < fin_depth(sh.D);  // deeper levels: finish_kernel
   const bool root_only = (n == 1 && a.L == 0);
   const uint64_t P = a.P;
   if (!split) {
@@ -375,6 +384,180 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   }
 }
 
+// Block-wide exclusive segmented scan of the threads' aggregates (every
+// thread calls it); returns the thread's exclusive prefix.  s_wave holds
+// kThreads / kWave entries; the caller syncs before reusing it.
+__device__ inline SegCount block_seg_excl(SegCount agg, SegCount* s_wave) {
+  SegCount incl = agg;
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int ofs = 1; ofs < kWave; ofs <<= 1) {
+    SegCount u;
+    u.f = __shfl_up(incl.f, ofs);
+    u.l = __shfl_up(incl.l, ofs);
+    u.r = __shfl_up(incl.r, ofs);
+    if ((int)lane >= ofs) incl = seg_op(u, incl);
+  }
+  SegCount excl;
+  excl.f = __shfl_up(incl.f, 1);
+  excl.l = __shfl_up(incl.l, 1);
+  excl.r = __shfl_up(incl.r, 1);
+  if (lane == 0) excl = SegCount{0u, 0u, 0u};
+  const int wave = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_wave[wave] = incl;
+  __syncthreads();
+  SegCount wcarry{0u, 0u, 0u};
+#pragma unroll
+  for (int w = 0; w < kThreads / kWave; ++w)
+    if (w < wave) wcarry = seg_op(wcarry, s_wave[w]);
+  return seg_op(wcarry, excl);
+}
+
+// Level launches that moved array m of an event before depth L0: every depth
+// L < L0 whose split dimension L % dim is not m.  Array m of that event sits
+// in buffer (moves & 1) when the finisher starts; events with fewer global
+// depths than the launch count were skipped by the later launches.
+__host__ __device__ inline uint32_t moves_before(uint32_t L0, uint32_t m, uint32_t dim) {
+  const uint32_t same = L0 > m ? (L0 - 1u - m) / dim + 1u : 0u;
+  return L0 - same;
+}
+
+template <int DIM>
+struct FinArgs {
+  const uint32_t* planes[4][2];  // array m, buffer c: plane k (rank in dimension k) at planes[m][c] + k * P
+  uint32_t* F;                // in-order position -> rank in dimension 0 of its node
+  TileMap fm;                 // workgroup -> event; local index = subtree of depth fin_depth(D)
+  uint32_t P;
+};
+
+// The levels fin_depth(D) .. D-1 of one subtree in LDS: the same medians and
+// stable partitions as level_kernel (FKDTree_cpu.h:229-268), with the
+// subtree's points named by their index in array 0 (sorted by dimension 0)
+// and their ranks kept in a table, so only 16-bit names move.  Writes the
+// in-order node table F of the subtree, leaves included.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
+  __shared__ uint32_t s_rank[DIM][kFinCap];
+  __shared__ uint16_t s_arr[DIM][2][kFinCap + 1];
+  __shared__ SegCount s_wave[kThreads / kWave];
+  const uint32_t e = a.fm.tile_ev[blockIdx.x];
+  const uint32_t t0 = blockIdx.x - a.fm.tb[e];
+  const uint32_t o = a.fm.off[e];
+  const uint32_t n = a.fm.off[e + 1] - o;
+  const Shape sh = shape_of(n);
+  const uint32_t L0 = fin_depth(sh.D);
+  uint32_t start0, size0;
+  subtree_range(sh, L0, t0, start0, size0);
+  const uint64_t P = a.P;
+  const uint32_t* pl[DIM];
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) pl[m] = a.planes[m][moves_before(L0, (uint32_t)m, (uint32_t)DIM) & 1u];
+  for (uint32_t j = threadIdx.x; j < size0; j += kThreads) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) s_rank[k][j] = pl[0][(uint64_t)k * P + o + start0 + j];
+    s_arr[0][0][j] = (uint16_t)j;
+  }
+  if (sh.D == 0) {  // a single point
+    if (threadIdx.x == 0 && n == 1) a.F[o] = pl[0][o];
+    return;
+  }
+  __syncthreads();
+  // names of the other arrays' points: index of their dimension-0 rank in array 0
+#pragma unroll
+  for (int m = 1; m < DIM; ++m)
+    for (uint32_t j = threadIdx.x; j < size0; j += kThreads) {
+      const uint32_t r0 = pl[m][o + start0 + j];
+      uint32_t lo = 0, hi = size0;  // first index with s_rank[0][idx] >= r0
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_rank[0][mid] < r0) lo = mid + 1u; else hi = mid;
+      }
+      s_arr[m][0][j] = (uint16_t)(lo < size0 ? lo : size0 - 1u);  // always found; kept in range
+    }
+  __syncthreads();
+  uint32_t cur[DIM];
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) cur[m] = 0;
+  const uint32_t first = threadIdx.x * (uint32_t)kLevelItems;
+  for (uint32_t L = L0; L < sh.D; ++L) {
+    const uint32_t d = L % (uint32_t)DIM;
+    const bool last = L + 1u == sh.D;
+    uint32_t st[kLevelItems], mp[kLevelItems];
+    bool inside[kLevelItems];
+#pragma unroll
+    for (int i = 0; i < kLevelItems; ++i) {
+      const uint32_t j = first + i;
+      inside[i] = false;
+      st[i] = mp[i] = 0;
+      if (j < size0) {
+        const uint32_t jl = start0 + j;
+        uint32_t sz;
+        subtree_range(sh, L, subtree_at(sh, L, jl), st[i], sz);
+        inside[i] = jl - st[i] < sz;
+        mp[i] = st[i] + left_size(sz);
+        // node of depth L, or (last split) a leaf
+        if (inside[i] && (jl == mp[i] || last)) a.F[o + jl] = s_rank[0][s_arr[d][cur[d]][j]];
+      }
+    }
+    if (last) break;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      if ((uint32_t)m == d) continue;
+      const uint16_t* src = s_arr[m][cur[m]];
+      uint16_t* dst = s_arr[m][cur[m] ^ 1u];
+      uint32_t cls[kLevelItems], dest0[kLevelItems], who[kLevelItems];
+      SegCount agg{0u, 0u, 0u};
+#pragma unroll
+      for (int i = 0; i < kLevelItems; ++i) {
+        const uint32_t j = first + i;
+        cls[i] = kNone;
+        dest0[i] = 0;
+        who[i] = 0;
+        if (!inside[i]) continue;
+        const uint32_t jl = start0 + j;
+        who[i] = src[j];
+        const uint32_t r = s_rank[d][who[i]];
+        const uint32_t rm = s_rank[d][s_arr[d][cur[d]][mp[i] - start0]];
+        if (jl == st[i]) agg = SegCount{1u, 0u, 0u};
+        uint32_t c;
+        if (r < rm) {
+          c = kLeft;
+          dest0[i] = st[i];
+          agg.l++;
+        } else if (r > rm) {
+          c = kRight;
+          dest0[i] = mp[i] + 1u;
+          agg.r++;
+        } else {
+          c = kMed;
+          dest0[i] = mp[i];
+        }
+        if (jl == st[i]) c |= 4u;
+        cls[i] = c;
+      }
+      const SegCount run0 = block_seg_excl(agg, s_wave);
+      uint32_t nl = run0.l, nr = run0.r;
+#pragma unroll
+      for (int i = 0; i < kLevelItems; ++i) {
+        const uint32_t c = cls[i];
+        if (c & 4u) {
+          nl = 0;
+          nr = 0;
+        }
+        uint32_t dest = dest0[i];
+        switch (c & 3u) {
+          case kLeft: dest += nl; ++nl; break;
+          case kRight: dest += nr; ++nr; break;
+          default: break;
+        }
+        if ((c & 3u) != kNone) dst[dest - start0] = (uint16_t)who[i];
+      }
+      __syncthreads();
+      cur[m] ^= 1u;
+    }
+  }
+}
+
 // Tree arrays from the in-order node table: heap node v of every event sits
 // at in-order position start + left_size(size) of its subtree; its point is
 // S0[F[that position]].  One thread per node, writes coalesced.
@@ -393,7 +576,8 @@ __global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a) {
     uint32_t start, size;
     subtree_range(sh, L, ts, start, size);
     const uint32_t j = start + left_size(size);
-    write_node(a, o, v, L, a.S0[a.F[o + j]]);
+    const uint32_t f = a.F[o + j];
+    if (f < a.P) write_node(a, o, v, L, a.S0[f]);  // F is complete; the test keeps reads in range
   }
 }
 

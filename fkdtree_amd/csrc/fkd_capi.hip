@@ -115,6 +115,26 @@ struct Tiling {
     FKD_HIP(hipStreamSynchronize(s));
     return FKD_OK;
   }
+  // Tiles given per event.
+  int init_counts(const std::vector<uint32_t>& per_event, hipStream_t s) {
+    const size_t E = per_event.size();
+    std::vector<uint32_t> htb(E + 1), hev;
+    uint32_t t = 0;
+    for (size_t e = 0; e < E; ++e) {
+      htb[e] = t;
+      for (uint32_t i = 0; i < per_event[e]; ++i) hev.push_back((uint32_t)e);
+      t += per_event[e];
+    }
+    htb[E] = t;
+    T = t;
+    FKD_TRY(tb.ensure(htb.size() * 4));
+    FKD_TRY(tile_ev.ensure(std::max<size_t>(hev.size(), 1) * 4));
+    FKD_HIP(hipMemcpyAsync(tb.p, htb.data(), htb.size() * 4, hipMemcpyHostToDevice, s));
+    if (!hev.empty())
+      FKD_HIP(hipMemcpyAsync(tile_ev.p, hev.data(), hev.size() * 4, hipMemcpyHostToDevice, s));
+    FKD_HIP(hipStreamSynchronize(s));
+    return FKD_OK;
+  }
   fkd::TileMap map(const uint32_t* off) const {
     fkd::TileMap m;
     m.tile_ev = tile_ev.as<uint32_t>();
@@ -138,11 +158,12 @@ struct fkd_tree_s {
   uint32_t n_events = 0;
   uint32_t P = 0;
   uint32_t Dmax = 0;
+  uint32_t global_levels = 0;  // depths built by level_kernel; the rest by finish_kernel
   bool ids_in_order = true;
   bool built = false;
   std::vector<uint64_t> h_off;
   DevBuf off;                  // uint32 event offsets (n_events + 1)
-  Tiling radix_tiles, level_tiles;
+  Tiling radix_tiles, level_tiles, fin_tiles;
   DevBuf X, ids;               // input copy
   DevBuf Tc, Tid, nodes, nodes_id;  // outputs and search records
   DevBuf A, K, R, planes, F;   // build scratch
@@ -237,7 +258,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     return FKD_OK;
   }
   FKD_TRY(ensure_build_scratch(t));
-  const uint32_t levels = std::max<uint32_t>(1, t->Dmax);
+  const uint32_t levels = t->global_levels;
   const fkd::TileMap tm = t->level_tiles.map(t->off.as<uint32_t>());
   constexpr uint32_t M = DIM - 1;
   const uint32_t RT = t->radix_tiles.T;
@@ -331,11 +352,35 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     fkd::level_kernel<DIM><<<blocks, fkd::kThreads, 0, s>>>(a);
     FKD_TRY(launch_check("level"));
     for (uint32_t i = 0; i < M; ++i) cur[moved[i]] ^= 1;
-    if (L + 1 == levels) {
-      // 6. Tree arrays from the in-order node table.
-      fkd::output_kernel<DIM><<<tm.T, fkd::kThreads, 0, s>>>(a);
-      FKD_TRY(launch_check("output"));
+  }
+  // 6. The depths whose subtrees fit a workgroup, one subtree per workgroup.
+  if (t->fin_tiles.T) {
+    fkd::FinArgs<DIM> f{};
+    for (int m = 0; m < DIM; ++m) {
+      f.planes[m][0] = Pbuf(m, 0);
+      f.planes[m][1] = Pbuf(m, 1);
     }
+    f.F = t->F.as<uint32_t>();
+    f.fm = t->fin_tiles.map(t->off.as<uint32_t>());
+    f.P = P;
+    fkd::finish_kernel<DIM><<<t->fin_tiles.T, fkd::kThreads, 0, s>>>(f);
+    FKD_TRY(launch_check("finish"));
+  }
+  // 7. Tree arrays from the in-order node table.
+  {
+    fkd::LevelArgs<DIM> a{};
+    a.tm = tm;
+    a.S0 = Abuf(0, 0);
+    a.F = t->F.as<uint32_t>();
+    a.X = X;
+    a.ids = t->ids.as<uint32_t>();
+    a.P = P;
+    a.Tc = t->Tc.as<float>();
+    a.Tid = t->Tid.as<uint32_t>();
+    a.nodes = t->nodes.as<float>();
+    a.nodes_id = t->nodes_id.as<float>();
+    fkd::output_kernel<DIM><<<tm.T, fkd::kThreads, 0, s>>>(a);
+    FKD_TRY(launch_check("output"));
   }
   t->built = true;
   t->q_plan_nq = 0;
@@ -571,14 +616,21 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   std::vector<uint32_t> off32(E + 1);
   for (uint32_t e = 0; e <= E; ++e) off32[e] = (uint32_t)t->h_off[e];
   t->Dmax = 0;
+  t->global_levels = 0;
+  std::vector<uint32_t> fin_count(E, 0);
   for (uint32_t e = 0; e < E; ++e) {
     const uint64_t n = t->h_off[e + 1] - t->h_off[e];
-    if (n) t->Dmax = std::max(t->Dmax, fkd::floor_log2((uint32_t)n));
+    if (!n) continue;
+    const uint32_t D = fkd::floor_log2((uint32_t)n);
+    t->Dmax = std::max(t->Dmax, D);
+    t->global_levels = std::max(t->global_levels, fkd::fin_depth(D));
+    fin_count[e] = 1u << fkd::fin_depth(D);  // subtrees the finisher builds
   }
   FKD_TRY(t->off.ensure((E + 1) * 4));
   FKD_HIP(hipMemcpyAsync(t->off.p, off32.data(), (E + 1) * 4, hipMemcpyHostToDevice, s));
   FKD_TRY(t->radix_tiles.init(t->h_off, fkd::kRadixTile, s));
   FKD_TRY(t->level_tiles.init(t->h_off, fkd::kLevelTile, s));
+  FKD_TRY(t->fin_tiles.init_counts(fin_count, s));
   const size_t S = std::max<uint32_t>(P, 1);
   const int rf = dim <= 3 ? 4 : 8;
   FKD_TRY(t->Tc.ensure(S * dim * 4));
@@ -878,7 +930,7 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
   if (!t || !bytes) return fail(FKD_ERR_INVALID, "tree / bytes is NULL");
   uint64_t b = 0;
   for (const DevBuf* d : {&t->off, &t->radix_tiles.tile_ev, &t->radix_tiles.tb, &t->level_tiles.tile_ev,
-                          &t->level_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R,
+                          &t->level_tiles.tb, &t->fin_tiles.tile_ev, &t->fin_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R,
                           &t->planes, &t->F, &t->counts, &t->offsets, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
                           &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_bwork, &t->q_border,
