@@ -77,17 +77,21 @@ __global__ void bbox_kernel(const float* __restrict__ X, const uint32_t* __restr
 }
 
 // Rank planes of the level arrays: array m at position j holds the ranks of
-// its point in every dimension k: (k == m) ? j : R[k][A_m[j]].
+// its point in every dimension k: (k == m) ? j : R[A_m[j]][k], where the
+// ranks of a point form one 16-byte record (one gather per array element).
 template <int DIM>
-__global__ void planes_kernel(const uint32_t* __restrict__ A, uint64_t astride, const uint32_t* __restrict__ R,
+__global__ void planes_kernel(const uint32_t* __restrict__ A, uint64_t astride, const uint4* __restrict__ R,
                               uint32_t P, uint32_t* __restrict__ planes, uint64_t pstride) {
+  static_assert(DIM <= 4, "rank records hold 4 dimensions");
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
 #pragma unroll
     for (int m = 0; m < DIM; ++m) {
       const uint32_t p = A[m * astride + j];
+      const uint4 r = R[p];
+      const uint32_t rk[4] = {r.x, r.y, r.z, r.w};
       uint32_t* dst = planes + m * pstride;
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * P + j] = (k == m) ? j : R[(uint64_t)k * P + p];
+      for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * P + j] = (k == m) ? j : rk[k];
     }
   }
 }
@@ -101,7 +105,7 @@ __global__ void id_keys_kernel(const uint32_t* __restrict__ ids, uint32_t P, uin
   }
 }
 
-constexpr int kLevelItems = 4;
+constexpr int kLevelItems = 8;
 constexpr int kLevelTile = kThreads * kLevelItems;
 
 // Depths whose subtrees have at most kFinCap points are built by one
@@ -110,7 +114,8 @@ constexpr int kLevelTile = kThreads * kLevelItems;
 // 2^(D-L+1) - 1 points, so the finisher starts at depth fin_depth(D).
 constexpr uint32_t kFinLog = 10;
 constexpr uint32_t kFinCap = (1u << kFinLog) - 1u;
-static_assert(kFinCap <= (uint32_t)kLevelTile, "a finisher subtree fits one pass of the workgroup");
+constexpr int kFinItems = 4;  // consecutive positions per thread in the finisher
+static_assert(kFinCap <= (uint32_t)(kThreads * kFinItems), "a finisher subtree fits one pass of the workgroup");
 __host__ __device__ inline uint32_t fin_depth(uint32_t D) { return D + 1u > kFinLog ? D + 1u - kFinLog : 0u; }
 
 template <int DIM>
@@ -121,7 +126,7 @@ struct LevelArgs {
   const uint32_t* src[3];     // rank planes of the DIM-1 other arrays ...
   uint32_t* dst[3];           // ... and where their partitioned copies go
   const uint32_t* S0;         // rank in dimension 0 -> point index
-  uint32_t* F;                // in-order position -> rank in dimension 0 of its node
+  uint32_t* F;                // heap index -> rank in dimension 0 of the node's point
   TileMap tm;                 // tiles of kLevelTile points, event aligned
   const float* X;             // input points
   const uint32_t* ids;
@@ -135,18 +140,29 @@ struct LevelArgs {
 };
 
 template <int DIM>
-__device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t node, uint32_t depth, uint32_t p) {
+__device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t node, uint32_t depth, uint32_t p,
+                                  const float4* prec) {
   constexpr int RF = NodeRec<DIM>::kFloats;
   float c[DIM];
   float split = 0.f;
+  uint32_t id;
   const uint32_t dd = depth % (uint32_t)DIM;
+  if (DIM <= 3) {
+    const float4 r = prec[p];
+    const float rc[3] = {r.x, r.y, r.z};
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) c[m] = rc[m < 3 ? m : 0];
+    id = __float_as_uint(r.w);
+  } else {
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) c[m] = a.X[(uint64_t)m * a.P + p];
+    id = a.ids[p];
+  }
 #pragma unroll
   for (int m = 0; m < DIM; ++m) {
-    c[m] = a.X[(uint64_t)m * a.P + p];
     a.Tc[(uint64_t)m * a.P + o + node] = c[m];
     split = dd == (uint32_t)m ? c[m] : split;
   }
-  const uint32_t id = a.ids[p];
   a.Tid[o + node] = id;
   float* rec = a.nodes + (uint64_t)(o + node) * RF;
   float* rid = a.nodes_id + (uint64_t)(o + node) * RF;
@@ -210,8 +226,8 @@ enum : uint32_t { kNone = 0, kLeft = 1, kMed = 2, kRight = 3 };
 // median) and (start+k, start+size) (rank above).  Elements carry their
 // ranks in every dimension, so the partition reads and writes only
 // coalesced planes.  Blocks of array 0 also store the depth-L nodes
-// (add_at_position, FKDTree_cpu.h:253-255) and, on the last split depth,
-// the leaves (FKDTree_cpu.h:271-276).
+// (add_at_position, FKDTree_cpu.h:253-255).  Depths from fin_depth(D) on,
+// the last split and the leaves included, are finish_kernel's.
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   constexpr uint32_t M = DIM - 1;
@@ -235,20 +251,20 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
     return;
   }
   const uint32_t first = lt * (uint32_t)kLevelTile + threadIdx.x * kLevelItems;
-  const bool last = (a.L + 1 == sh.D);
   const uint32_t* Ad_d = a.Ad + (uint64_t)a.d * P;  // plane d of array d
   const uint32_t* S = M ? a.src[arr] : nullptr;
 
   // Phase 1: interval of every item and all loads (no stores in between so
   // the loads of all items can be in flight together).
   uint32_t cls[kLevelItems], dest0[kLevelItems], rk[kLevelItems][DIM], rm[kLevelItems], fr[kLevelItems];
-  uint32_t st0[kLevelItems], mp[kLevelItems];
+  uint32_t st0[kLevelItems], mp[kLevelItems], hn[kLevelItems];
 #pragma unroll
   for (int i = 0; i < kLevelItems; ++i) {
     const uint32_t jl = first + i;
     cls[i] = kNone;
     st0[i] = 0;
     mp[i] = 0;
+    hn[i] = 0;
     rm[i] = 0;
     fr[i] = 0;
 #pragma unroll
@@ -261,13 +277,14 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
         const uint32_t mpos = start + left_size(size);
         st0[i] = start;
         mp[i] = mpos;
+        hn[i] = (1u << a.L) - 1u + ts;  // heap index of the interval's node
         cls[i] = 8u;  // in an interval
         if (M) {
           rm[i] = Ad_d[o + mpos];
 #pragma unroll
           for (int k = 0; k < DIM; ++k) rk[i][k] = S[(uint64_t)k * P + o + jl];
         }
-        if (arr == 0 && (jl == mpos || last)) fr[i] = a.Ad[o + jl];  // plane 0 of array d
+        if (arr == 0 && jl == mpos) fr[i] = a.Ad[o + jl];  // plane 0 of array d
       }
     }
   }
@@ -277,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   for (int i = 0; i < kLevelItems; ++i) {
     const uint32_t jl = first + i;
     if (!(cls[i] & 8u)) continue;
-    if (arr == 0 && (jl == mp[i] || last)) a.F[o + jl] = fr[i];  // node of depth L, or a leaf
+    if (arr == 0 && jl == mp[i]) a.F[o + hn[i]] = fr[i];  // node of depth L
     if (!M) continue;
     uint32_t r = rk[i][0];
 #pragma unroll
@@ -425,7 +442,7 @@ __host__ __device__ inline uint32_t moves_before(uint32_t L0, uint32_t m, uint32
 template <int DIM>
 struct FinArgs {
   const uint32_t* planes[4][2];  // array m, buffer c: plane k (rank in dimension k) at planes[m][c] + k * P
-  uint32_t* F;                // in-order position -> rank in dimension 0 of its node
+  uint32_t* F;                // heap index -> rank in dimension 0 of the node's point
   TileMap fm;                 // workgroup -> event; local index = subtree of depth fin_depth(D)
   uint32_t P;
 };
@@ -433,8 +450,8 @@ struct FinArgs {
 // The levels fin_depth(D) .. D-1 of one subtree in LDS: the same medians and
 // stable partitions as level_kernel (FKDTree_cpu.h:229-268), with the
 // subtree's points named by their index in array 0 (sorted by dimension 0)
-// and their ranks kept in a table, so only 16-bit names move.  Writes the
-// in-order node table F of the subtree, leaves included.
+// and their ranks kept in a table, so only 16-bit names move.  Writes F
+// (heap index -> rank) for the subtree's nodes, leaves included.
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
   __shared__ uint32_t s_rank[DIM][kFinCap];
@@ -478,25 +495,29 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
   uint32_t cur[DIM];
 #pragma unroll
   for (int m = 0; m < DIM; ++m) cur[m] = 0;
-  const uint32_t first = threadIdx.x * (uint32_t)kLevelItems;
+  const uint32_t first = threadIdx.x * (uint32_t)kFinItems;
   for (uint32_t L = L0; L < sh.D; ++L) {
     const uint32_t d = L % (uint32_t)DIM;
     const bool last = L + 1u == sh.D;
-    uint32_t st[kLevelItems], mp[kLevelItems];
-    bool inside[kLevelItems];
+    uint32_t st[kFinItems], mp[kFinItems];
+    bool inside[kFinItems];
 #pragma unroll
-    for (int i = 0; i < kLevelItems; ++i) {
+    for (int i = 0; i < kFinItems; ++i) {
       const uint32_t j = first + i;
       inside[i] = false;
       st[i] = mp[i] = 0;
       if (j < size0) {
         const uint32_t jl = start0 + j;
+        const uint32_t ts = subtree_at(sh, L, jl);
         uint32_t sz;
-        subtree_range(sh, L, subtree_at(sh, L, jl), st[i], sz);
+        subtree_range(sh, L, ts, st[i], sz);
         inside[i] = jl - st[i] < sz;
         mp[i] = st[i] + left_size(sz);
-        // node of depth L, or (last split) a leaf
-        if (inside[i] && (jl == mp[i] || last)) a.F[o + jl] = s_rank[0][s_arr[d][cur[d]][j]];
+        // node of depth L, or (last split) a leaf of depth D
+        if (inside[i] && (jl == mp[i] || last)) {
+          const uint32_t h = jl == mp[i] ? (1u << L) - 1u + ts : (1u << sh.D) - 1u + subtree_at(sh, sh.D, jl);
+          a.F[o + h] = s_rank[0][s_arr[d][cur[d]][j]];
+        }
       }
     }
     if (last) break;
@@ -505,10 +526,10 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
       if ((uint32_t)m == d) continue;
       const uint16_t* src = s_arr[m][cur[m]];
       uint16_t* dst = s_arr[m][cur[m] ^ 1u];
-      uint32_t cls[kLevelItems], dest0[kLevelItems], who[kLevelItems];
+      uint32_t cls[kFinItems], dest0[kFinItems], who[kFinItems];
       SegCount agg{0u, 0u, 0u};
 #pragma unroll
-      for (int i = 0; i < kLevelItems; ++i) {
+      for (int i = 0; i < kFinItems; ++i) {
         const uint32_t j = first + i;
         cls[i] = kNone;
         dest0[i] = 0;
@@ -538,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
       const SegCount run0 = block_seg_excl(agg, s_wave);
       uint32_t nl = run0.l, nr = run0.r;
 #pragma unroll
-      for (int i = 0; i < kLevelItems; ++i) {
+      for (int i = 0; i < kFinItems; ++i) {
         const uint32_t c = cls[i];
         if (c & 4u) {
           nl = 0;
@@ -558,26 +579,36 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
   }
 }
 
-// Tree arrays from the in-order node table: heap node v of every event sits
-// at in-order position start + left_size(size) of its subtree; its point is
-// S0[F[that position]].  One thread per node, writes coalesced.
+// Point records (coordinates, id) in input order, one 16-byte gather per
+// node for the output (DIM <= 3).
 template <int DIM>
-__global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a) {
+__global__ void records_kernel(const float* __restrict__ X, const uint32_t* __restrict__ ids, uint32_t P,
+                               float4* __restrict__ rec) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+    float4 r;
+    r.x = X[p];
+    r.y = DIM > 1 ? X[(uint64_t)(DIM > 1 ? 1 : 0) * P + p] : 0.f;
+    r.z = DIM > 2 ? X[(uint64_t)(DIM > 2 ? 2 : 0) * P + p] : 0.f;
+    r.w = __uint_as_float(ids[p]);
+    rec[p] = r;
+  }
+}
+
+// Tree arrays and search records: heap node v of every event holds the
+// point S0[F[v]] (S0: rank in dimension 0 -> point).  One thread per node,
+// writes coalesced.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a, const float4* __restrict__ rec) {
   const uint32_t t = blockIdx.x;
   const uint32_t e = a.tm.tile_ev[t];
   const uint32_t lt = t - a.tm.tb[e];
   const uint32_t o = a.tm.off[e];
   const uint32_t n = a.tm.off[e + 1] - o;
-  const Shape sh = shape_of(n);
   for (uint32_t v = lt * (uint32_t)kLevelTile + threadIdx.x; v < min(n, (lt + 1) * (uint32_t)kLevelTile);
        v += kThreads) {
-    const uint32_t L = floor_log2(v + 1u);
-    const uint32_t ts = v - ((1u << L) - 1u);
-    uint32_t start, size;
-    subtree_range(sh, L, ts, start, size);
-    const uint32_t j = start + left_size(size);
-    const uint32_t f = a.F[o + j];
-    if (f < a.P) write_node(a, o, v, L, a.S0[f]);  // F is complete; the test keeps reads in range
+    const uint32_t f = a.F[o + v];
+    if (f >= a.P) continue;  // F is complete; the test keeps reads in range
+    write_node(a, o, v, floor_log2(v + 1u), a.S0[f], rec);
   }
 }
 

@@ -166,7 +166,7 @@ struct fkd_tree_s {
   Tiling radix_tiles, level_tiles, fin_tiles;
   DevBuf X, ids;               // input copy
   DevBuf Tc, Tid, nodes, nodes_id;  // outputs and search records
-  DevBuf A, K, R, planes, F;   // build scratch
+  DevBuf A, K, R, planes, F, Xr;   // build scratch
   DevBuf counts, offsets, err, bbox;
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev, q_aos;  // search scratch
   DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets;
@@ -270,7 +270,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   const uint64_t S = P;  // stride of one array
   uint32_t* A = t->A.as<uint32_t>();  // A[m][2][P]
   uint32_t* K = t->K.as<uint32_t>();  // K[m][2][P]
-  uint32_t* R = t->R.as<uint32_t>();  // R[m][P]
+  uint32_t* R = t->R.as<uint32_t>();  // R[p][4]: ranks of point p (first the id order, see 1.)
   auto Abuf = [&](int m, int c) { return A + ((uint64_t)m * 2 + c) * S; };
   auto Kbuf = [&](int m, int c) { return K + ((uint64_t)m * 2 + c) * S; };
   const float* X = t->X.as<float>();
@@ -289,13 +289,13 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
                                   S, nullptr, 8u * pass, !lastp, false, s));
       c = 1 - c;
     }
-    order = R;  // R[0] holds the (event, id) order until the key pass rewrites it
+    order = R;  // R holds the (event, id) order until the last key pass writes the ranks
   }
   // 2. Keys of every coordinate, in tie order.
   fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>());
   FKD_TRY(launch_check("keys"));
   // 3. Stable LSD sort of every coordinate inside every event; the last pass
-  //    stores each point's rank R[m][p] (the values end in buffer 0).
+  //    stores each point's rank R[p][m] (the values end in buffer 0).
   {
     int c = 0;
     for (int pass = 0; pass < 4; ++pass) {
@@ -314,7 +314,8 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   // 4. Rank planes: every array element carries its ranks in all dimensions.
   uint32_t* PL = t->planes.as<uint32_t>();  // PL[m][c][k][P]
   auto Pbuf = [&](int m, int c) { return PL + (((uint64_t)m * 2 + c) * DIM) * S; };
-  fkd::planes_kernel<DIM><<<grid, 256, 0, s>>>(Abuf(0, 0), 2 * S, R, P, Pbuf(0, 0), 2 * (uint64_t)DIM * S);
+  fkd::planes_kernel<DIM><<<grid, 256, 0, s>>>(Abuf(0, 0), 2 * S, reinterpret_cast<const uint4*>(R), P, Pbuf(0, 0),
+                                               2 * (uint64_t)DIM * S);
   FKD_TRY(launch_check("planes"));
   // 5. One partition launch per depth.
   int cur[4] = {0, 0, 0, 0};
@@ -366,8 +367,12 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     fkd::finish_kernel<DIM><<<t->fin_tiles.T, fkd::kThreads, 0, s>>>(f);
     FKD_TRY(launch_check("finish"));
   }
-  // 7. Tree arrays from the in-order node table.
+  // 7. Tree arrays and search records from F.
   {
+    if (DIM <= 3) {
+      fkd::records_kernel<DIM><<<grid, 256, 0, s>>>(X, t->ids.as<uint32_t>(), P, t->Xr.as<float4>());
+      FKD_TRY(launch_check("records"));
+    }
     fkd::LevelArgs<DIM> a{};
     a.tm = tm;
     a.S0 = Abuf(0, 0);
@@ -379,7 +384,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     a.Tid = t->Tid.as<uint32_t>();
     a.nodes = t->nodes.as<float>();
     a.nodes_id = t->nodes_id.as<float>();
-    fkd::output_kernel<DIM><<<tm.T, fkd::kThreads, 0, s>>>(a);
+    fkd::output_kernel<DIM><<<tm.T, fkd::kThreads, 0, s>>>(a, t->Xr.as<float4>());
     FKD_TRY(launch_check("output"));
   }
   t->built = true;
@@ -601,9 +606,10 @@ int ensure_build_scratch(fkd_tree_s* t) {
   const int dim = t->dim;
   FKD_TRY(t->A.ensure(S * dim * 2 * 4));
   FKD_TRY(t->K.ensure(S * dim * 2 * 4));
-  FKD_TRY(t->R.ensure(S * dim * 4));
+  FKD_TRY(t->R.ensure(S * 4 * 4));
   FKD_TRY(t->planes.ensure(S * dim * dim * 2 * 4));
   FKD_TRY(t->F.ensure(S * 4));
+  FKD_TRY(t->Xr.ensure(S * 16));
   const uint64_t nc = (uint64_t)dim * 256u * std::max<uint32_t>(t->radix_tiles.T, 1);
   FKD_TRY(t->counts.ensure(nc * 4));
   FKD_TRY(t->offsets.ensure((nc + 1) * 4));
@@ -916,7 +922,7 @@ void fkd_graph_destroy(fkd_graph_t g) { delete g; }
 int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
-  for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
+  for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->Xr, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
                     &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
                     &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_aos, &t->zarena})
     b->release();
@@ -930,7 +936,7 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
   if (!t || !bytes) return fail(FKD_ERR_INVALID, "tree / bytes is NULL");
   uint64_t b = 0;
   for (const DevBuf* d : {&t->off, &t->radix_tiles.tile_ev, &t->radix_tiles.tb, &t->level_tiles.tile_ev,
-                          &t->level_tiles.tb, &t->fin_tiles.tile_ev, &t->fin_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R,
+                          &t->level_tiles.tb, &t->fin_tiles.tile_ev, &t->fin_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R, &t->Xr,
                           &t->planes, &t->F, &t->counts, &t->offsets, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
                           &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_bwork, &t->q_border,

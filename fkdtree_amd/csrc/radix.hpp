@@ -70,7 +70,8 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
 
 // Stable scatter of one digit pass.  offsets = exclusive scan of the counts of
 // all arrays (array m's entries start at m * P).  Keys are written only when
-// WRITE_KEYS; when WRITE_RANKS, ranks[m][val] = final slot of val.
+// WRITE_KEYS; when WRITE_RANKS, ranks[4 val + m] = final slot of val (the
+// ranks of a point in all dimensions form one 16-byte record).
 template <bool WRITE_KEYS, bool WRITE_RANKS>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, uint32_t* __restrict__ keys_out,
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       const uint32_t dest = cnt[wave][dig] + rank[r];
       Vo[dest] = val[r];
       if (WRITE_KEYS) Ko[dest] = key[r];
-      if (WRITE_RANKS) ranks[m * (uint64_t)P + val[r]] = dest;
+      if (WRITE_RANKS) ranks[4 * (uint64_t)val[r] + m] = dest;
     }
   }
 }
