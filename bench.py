@@ -193,7 +193,7 @@ def main():
     if args.serve_events:
         res["serving"] = bench_serving(args.serve_events, [float(r) for r in args.serve_rates.split(",")])
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads)
+        res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads, args.events)
     print(json.dumps(res), flush=True)
     return 0
 
@@ -232,6 +232,17 @@ def measured_traffic(workload: str, kernel: str):
     return k.get("fetch_bytes", 0.0) + k.get("write_bytes", 0.0)
 
 
+def events_batch(n_events: int):
+    """The first n_events events of the 2,048-event workload and their boxes."""
+    from fkdtree_amd import gen
+
+    seed = gen.workload_seed("events_2048")
+    sizes = gen.event_sizes(2048, seed, 1000, gen.LOG2_200)[:n_events]
+    evs = [gen.mixed_event(int(sz), seed, e) for e, sz in enumerate(sizes)]
+    batch = gen.concat_events(evs)
+    return batch, gen.event_self_boxes(batch, 0.01)
+
+
 def bench_events(n_events: int, steps: int) -> dict:
     """Events per second of build + self-search (every point of an event is
     the centre of one box of half-width 0.01 searched in its own tree), for
@@ -244,11 +255,7 @@ def bench_events(n_events: int, steps: int) -> dict:
     from fkdtree_amd import DeviceArray, Stream, Timer, gen
     from fkdtree_amd._lib import check, lib
 
-    seed = gen.workload_seed("events_2048")
-    sizes = gen.event_sizes(2048, seed, 1000, gen.LOG2_200)[:n_events]
-    evs = [gen.mixed_event(int(sz), seed, e) for e, sz in enumerate(sizes)]
-    batch = gen.concat_events(evs)
-    boxes = gen.event_self_boxes(batch, 0.01)
+    batch, boxes = events_batch(n_events)
     P = batch.points.n
     qev = np.repeat(np.arange(n_events, dtype=np.uint32), np.diff(batch.offsets))
     s = Stream()
@@ -344,13 +351,24 @@ def bench_events(n_events: int, steps: int) -> dict:
             "batched_over_plain": plain_ms / batched_ms}
 
 
-def cpu_baseline(pts, boxes, threads):
-    """Reference CPU tree (oracle/_ref, compiled from the reference sources)
-    when present, else the oracle restatement: build on 1 thread
-    (main.cpp:201-205), search_in_the_box of every box on all host cores
-    (OpenMP in place of tbb::parallel_for, main.cpp:372)."""
+def cpu_baseline(pts, boxes, threads, n_events=0):
+    """CPU baselines, timed on this host.
+
+    * Headline: the reference CPU tree (oracle/_ref, compiled from the
+      reference sources) when present, else the oracle restatement: build on
+      1 thread (main.cpp:201-205), search_in_the_box of every box on all host
+      cores (OpenMP in place of tbb::parallel_for, main.cpp:372).
+    * The same search on 1 thread over the first 131,072 boxes.
+    * Side comparator: the reference's copy of the CMSSW KD-tree, built and
+      searched box after box on 1 thread as main.cpp:560-629 does (first
+      131,072 boxes).
+    * Batched events: the oracle restatement, events built in parallel over
+      the host cores and every point's box searched in parallel (the same
+      events as the GPU events section)."""
     import oracle
     cores = threads or os.cpu_count() or 1
+    sample = min(boxes.q, 131072)
+    mins1, maxs1 = np.ascontiguousarray(boxes.mins[:, :sample]), np.ascontiguousarray(boxes.maxs[:, :sample])
     if oracle.ref_available():
         kind = "reference"
         rt = oracle.RefTree(pts.coords, pts.ids)
@@ -360,6 +378,9 @@ def cpu_baseline(pts, boxes, threads):
         t0 = time.perf_counter()
         off, _ = rt.search(boxes.mins, boxes.maxs, variant=0, threads=cores)
         search_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rt.search(mins1, maxs1, variant=0, threads=1)
+        search1_s = time.perf_counter() - t0
         rt.close()
     else:
         kind = "port"
@@ -369,9 +390,31 @@ def cpu_baseline(pts, boxes, threads):
         t0 = time.perf_counter()
         off, _ = oracle.search(tc, ti, boxes.mins, boxes.maxs, threads=cores)
         search_s = time.perf_counter() - t0
-    return {"value": boxes.q / search_s, "unit": "box queries/s", "cores": cores, "kind": kind,
-            "sample": f"all {boxes.q} boxes of the workload, one build of all {pts.n} points on 1 thread",
-            "build_ms_1thread": build_s * 1e3, "hits": int(off[-1])}
+        t0 = time.perf_counter()
+        oracle.search(tc, ti, mins1, maxs1, threads=1)
+        search1_s = time.perf_counter() - t0
+    res = {"value": boxes.q / search_s, "unit": "box queries/s", "cores": cores, "kind": kind,
+           "sample": f"all {boxes.q} boxes of the workload, one build of all {pts.n} points on 1 thread",
+           "build_ms_1thread": build_s * 1e3, "hits": int(off[-1]),
+           "search_1thread": {"value": sample / search1_s, "unit": "box queries/s", "cores": 1,
+                              "sample": f"first {sample} boxes"}}
+    if oracle.cmssw_available() and pts.dim == 3:
+        b_ms, s_ms, found = oracle.cmssw_build_search(pts.coords, mins1, maxs1)
+        res["side_comparator"] = {"kind": "CMSSW KDTreeLinkerAlgo (reference copy, KDTreeLinkerAlgoT.h)",
+                                  "value": sample / (s_ms * 1e-3), "unit": "box queries/s", "cores": 1,
+                                  "build_ms_1thread": b_ms, "found": found,
+                                  "sample": f"build of all {pts.n} points, first {sample} boxes"}
+    if n_events:
+        batch, eb = events_batch(n_events)
+        qev = np.repeat(np.arange(n_events, dtype=np.uint32), np.diff(batch.offsets))
+        t0 = time.perf_counter()
+        tc, ti = oracle.build_events(batch.points.coords, batch.points.ids, batch.offsets, threads=cores)
+        qoff, _ = oracle.search_events(tc, ti, batch.offsets, qev, eb.mins, eb.maxs, threads=cores)
+        ev_s = time.perf_counter() - t0
+        res["events"] = {"value": n_events / ev_s, "unit": "events/s", "cores": cores, "kind": "port",
+                         "hits": int(qoff[-1]),
+                         "sample": f"{n_events} events, {batch.points.n} points, every point's box"}
+    return res
 
 
 if __name__ == "__main__":

@@ -12,6 +12,9 @@ Two native libraries are wrapped:
 * ``_ref/libfkdref.so`` (``ref_shim.cpp``): the reference's own
   ``FKDTree_cpu.h`` compiled from its sources.  Present only where it was built
   (it travels with the tree as a built library).
+* ``_ref/libcmsswref.so`` (``cmssw_shim.cpp``): the reference's copy of the
+  CMSSW KD-tree (``KDTreeLinkerAlgoT.h``), a side comparator for the CPU
+  timing table (``main.cpp:560-629``).
 """
 from __future__ import annotations
 
@@ -24,6 +27,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libfkdref.so")
+CMSSW_SO = os.path.join(HERE, "_ref", "libcmsswref.so")
 
 _P = C.c_void_p
 _U64 = C.c_uint64
@@ -79,6 +83,24 @@ def ref():
         L.ref_destroy.argtypes = [_P]
         _ref = L
     return _ref
+
+
+def cmssw_available() -> bool:
+    return os.path.exists(CMSSW_SO)
+
+
+def cmssw_build_search(coords: np.ndarray, mins: np.ndarray, maxs: np.ndarray):
+    """CMSSW KD-tree (reference copy) on one thread: build, then every box in
+    turn, as main.cpp:560-629.  Returns (build_ms, search_ms, found)."""
+    L = C.CDLL(CMSSW_SO)
+    L.cms_build_search.argtypes = [_U64, _P, _U64, _P, _P, _P, _P, _P]
+    c = np.ascontiguousarray(coords, dtype=np.float32)
+    lo = np.ascontiguousarray(mins, dtype=np.float32)
+    hi = np.ascontiguousarray(maxs, dtype=np.float32)
+    assert c.shape[0] == 3 and lo.shape[0] == 3, "the CMSSW tree is instantiated for 3 dimensions"
+    b, s, f = C.c_double(), C.c_double(), C.c_uint64()
+    L.cms_build_search(c.shape[1], _ptr(c), lo.shape[1], _ptr(lo), _ptr(hi), C.byref(b), C.byref(s), C.byref(f))
+    return b.value, s.value, f.value
 
 
 def _ptr(a: np.ndarray):
