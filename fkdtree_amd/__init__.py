@@ -227,6 +227,23 @@ class _TreeBase:
             self._mirror = (c, i)
         return self._mirror
 
+    def serialize(self) -> bytes:
+        """The wire format of the built tree (fkd_serialize): header, event
+        offsets, heap-ordered coordinates and ids."""
+        size = C.c_uint64()
+        check(lib().fkd_serialize(self._h, None, 0, C.byref(size), _sh(self.stream)))
+        buf = np.empty(size.value, dtype=np.uint8)
+        check(lib().fkd_serialize(self._h, buf.ctypes.data, size.value, C.byref(size), _sh(self.stream)))
+        return buf.tobytes()
+
+    def save(self, path: str) -> None:
+        """Write the wire format to `path` (spill a resident tree)."""
+        data = self.serialize()
+        tmp = path + ".tmp"
+        with open(tmp, "wb") as f:
+            f.write(data)
+        os.replace(tmp, path)
+
     def device_tree(self) -> Tuple[int, int]:
         c, i = C.c_void_p(), C.c_void_p()
         check(lib().fkd_tree_device(self._h, C.byref(c), C.byref(i)))
@@ -371,6 +388,31 @@ class FKDEvents(_TreeBase):
     def search(self, mins, maxs, query_events) -> Tuple[np.ndarray, np.ndarray]:
         """Query q searches the tree of event query_events[q]: (offsets, ids)."""
         return self._search_host(mins, maxs, query_events)
+
+
+def deserialize(data: bytes, verify: bool = False, stream=None):
+    """A built tree from the wire format (fkd_deserialize): an FKDTree for one
+    event, else an FKDEvents.  verify=True rebuilds from the points and checks
+    that every array comes out identical."""
+    buf = np.frombuffer(data, dtype=np.uint8)
+    h = C.c_void_p()
+    check(lib().fkd_deserialize(buf.ctypes.data, buf.size, 1 if verify else 0, _sh(stream), C.byref(h)))
+    e = C.c_uint32()
+    check(lib().fkd_tree_info(h.value, None, C.byref(e), None))
+    cls = FKDTree if e.value == 1 else FKDEvents
+    self = cls.__new__(cls)
+    self._init_handle(h, stream)
+    if cls is FKDEvents:
+        hdr = np.frombuffer(data, dtype=np.uint8, count=24)
+        E = int(hdr[12:16].view(np.uint32)[0])
+        self.offsets = np.frombuffer(data, dtype=np.uint64, count=E + 1, offset=24).copy()
+    return self
+
+
+def load(path: str, verify: bool = False, stream=None):
+    """A built tree from a file written by `save`."""
+    with open(path, "rb") as f:
+        return deserialize(f.read(), verify=verify, stream=stream)
 
 
 class Graph:

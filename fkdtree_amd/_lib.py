@@ -28,6 +28,8 @@ _SIGS = {
     "fkd_tree_device": (I32, [P, C.POINTER(P), C.POINTER(P)]),
     "fkd_tree_info": (I32, [P, C.POINTER(I32), C.POINTER(U32), C.POINTER(U64)]),
     "fkd_trim": (I32, [P, P]),
+    "fkd_serialize": (I32, [P, P, U64, C.POINTER(U64), P]),
+    "fkd_deserialize": (I32, [P, U64, I32, P, C.POINTER(P)]),
     "fkd_set_tuning": (I32, [P, C.c_char_p, U32]),
     "fkd_get_tuning": (I32, [P, C.c_char_p, C.POINTER(U32)]),
     "fkd_device_bytes": (I32, [P, C.POINTER(U64)]),

@@ -579,6 +579,47 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
   }
 }
 
+// Bounding box of all points in fkey order (trees loaded without a build):
+// per-wave reductions, one atomic per wave and bound.
+template <int DIM>
+__global__ void bbox_points_kernel(const float* __restrict__ X, uint32_t P, uint32_t* bbox) {
+  uint32_t lo[DIM], hi[DIM];
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+    lo[m] = 0xffffffffu;
+    hi[m] = 0u;
+  }
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      const uint32_t k = fkey(X[(uint64_t)m * P + p]);
+      lo[m] = min(lo[m], k);
+      hi[m] = max(hi[m], k);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      lo[m] = min(lo[m], (uint32_t)__shfl_xor((int)lo[m], o));
+      hi[m] = max(hi[m], (uint32_t)__shfl_xor((int)hi[m], o));
+    }
+    if (lane_id() == 0 && lo[m] <= hi[m]) {
+      atomicMin(&bbox[m], lo[m]);
+      atomicMax(&bbox[DIM + m], hi[m]);
+    }
+  }
+}
+
+// 1 in *err if the rebuilt tree arrays differ from the loaded ones.
+__global__ void compare_kernel(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint64_t n,
+                               uint32_t* err) {
+  bool diff = false;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    diff |= a[i] != b[i];
+  if (__ballot(diff) != 0ull && lane_id() == 0) atomicOr(err, 1u);
+}
+
 // Point records (coordinates, id) in input order, one 16-byte gather per
 // node for the output (DIM <= 3).
 template <int DIM>
@@ -595,8 +636,9 @@ __global__ void records_kernel(const float* __restrict__ X, const uint32_t* __re
 }
 
 // Tree arrays and search records: heap node v of every event holds the
-// point S0[F[v]] (S0: rank in dimension 0 -> point).  One thread per node,
-// writes coalesced.
+// point S0[F[v]] (S0: rank in dimension 0 -> point), or point o + v when S0
+// is null (a tree loaded from its arrays).  One thread per node, writes
+// coalesced.
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a, const float4* __restrict__ rec) {
   const uint32_t t = blockIdx.x;
@@ -606,9 +648,13 @@ __global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a, cons
   const uint32_t n = a.tm.off[e + 1] - o;
   for (uint32_t v = lt * (uint32_t)kLevelTile + threadIdx.x; v < min(n, (lt + 1) * (uint32_t)kLevelTile);
        v += kThreads) {
-    const uint32_t f = a.F[o + v];
-    if (f >= a.P) continue;  // F is complete; the test keeps reads in range
-    write_node(a, o, v, floor_log2(v + 1u), a.S0[f], rec);
+    uint32_t p = o + v;  // S0 == null: the points already are the heap-ordered tree
+    if (a.S0) {
+      const uint32_t f = a.F[o + v];
+      if (f >= a.P) continue;  // F is complete; the test keeps reads in range
+      p = a.S0[f];
+    }
+    write_node(a, o, v, floor_log2(v + 1u), p, rec);
   }
 }
 

@@ -707,9 +707,140 @@ int create_common(int dim, uint32_t n_events, const uint64_t* event_offsets, fkd
   return FKD_OK;
 }
 
+// Tree arrays and search records of a handle whose points already are the
+// heap-ordered tree (loaded from the wire format), without a build.
+template <int DIM>
+int adopt_tree_impl(fkd_tree_s* t, hipStream_t s) {
+  const uint32_t P = t->P;
+  if (P) {
+    const uint32_t grid = grid_for(P);
+    if (DIM <= 3) {
+      fkd::records_kernel<DIM><<<grid, 256, 0, s>>>(t->X.as<float>(), t->ids.as<uint32_t>(), P, t->Xr.as<float4>());
+      FKD_TRY(launch_check("records"));
+    }
+    fkd::LevelArgs<DIM> a{};
+    a.tm = t->level_tiles.map(t->off.as<uint32_t>());
+    a.S0 = nullptr;
+    a.F = nullptr;
+    a.X = t->X.as<float>();
+    a.ids = t->ids.as<uint32_t>();
+    a.P = P;
+    a.Tc = t->Tc.as<float>();
+    a.Tid = t->Tid.as<uint32_t>();
+    a.nodes = t->nodes.as<float>();
+    a.nodes_id = t->nodes_id.as<float>();
+    fkd::output_kernel<DIM><<<a.tm.T, fkd::kThreads, 0, s>>>(a, t->Xr.as<float4>());
+    FKD_TRY(launch_check("output"));
+  }
+  FKD_TRY(fill_u32(t->bbox.as<uint32_t>(), 0xffffffffu, DIM, s));
+  FKD_TRY(fill_u32(t->bbox.as<uint32_t>() + DIM, 0u, DIM, s));
+  if (P) {
+    fkd::bbox_points_kernel<DIM><<<grid_for(P), 256, 0, s>>>(t->X.as<float>(), P, t->bbox.as<uint32_t>());
+    FKD_TRY(launch_check("bbox_points"));
+  }
+  t->built = true;
+  t->q_plan_nq = 0;
+  return FKD_OK;
+}
+
+constexpr char kWireMagic[8] = {'F', 'K', 'D', 'T', 'R', 'E', 'E', '1'};
+constexpr uint64_t kWireHeader = 24;  // magic, uint32 dim, uint32 n_events, uint64 n_points
+
+uint64_t wire_bytes(int dim, uint32_t n_events, uint64_t P) {
+  return kWireHeader + 8ull * (n_events + 1) + 4ull * P * (uint64_t)(dim + 1);
+}
+
 }  // namespace
 
 extern "C" {
+
+int fkd_serialize(fkd_tree_t t, void* buf, uint64_t capacity, uint64_t* size, fkd_stream_t stream) {
+  if (!t || !size) return fail(FKD_ERR_INVALID, "tree / size is NULL");
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  const uint64_t bytes = wire_bytes(t->dim, t->n_events, t->P);
+  *size = bytes;
+  if (!buf) return FKD_OK;
+  if (capacity < bytes) return fail(FKD_ERR_CAPACITY, "buffer holds " + std::to_string(capacity) + " of " +
+                                                          std::to_string(bytes) + " bytes");
+  char* b = static_cast<char*>(buf);
+  const uint32_t dim = (uint32_t)t->dim, E = t->n_events;
+  const uint64_t P = t->P;
+  std::memcpy(b, kWireMagic, 8);
+  std::memcpy(b + 8, &dim, 4);
+  std::memcpy(b + 12, &E, 4);
+  std::memcpy(b + 16, &P, 8);
+  std::memcpy(b + kWireHeader, t->h_off.data(), 8ull * (E + 1));
+  char* data = b + kWireHeader + 8ull * (E + 1);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (P) {
+    FKD_HIP(hipMemcpyAsync(data, t->Tc.p, P * dim * 4, hipMemcpyDeviceToHost, s));
+    FKD_HIP(hipMemcpyAsync(data + P * dim * 4, t->Tid.p, P * 4, hipMemcpyDeviceToHost, s));
+  }
+  FKD_HIP(hipStreamSynchronize(s));
+  return FKD_OK;
+}
+
+int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t stream, fkd_tree_t* out) {
+  if (!buf || !out) return fail(FKD_ERR_INVALID, "buf / out is NULL");
+  *out = nullptr;
+  const char* b = static_cast<const char*>(buf);
+  if (size < kWireHeader || std::memcmp(b, kWireMagic, 8) != 0)
+    return fail(FKD_ERR_INVALID, "not an FKDTREE1 buffer");
+  uint32_t dim = 0, E = 0;
+  uint64_t P = 0;
+  std::memcpy(&dim, b + 8, 4);
+  std::memcpy(&E, b + 12, 4);
+  std::memcpy(&P, b + 16, 8);
+  if (dim < 1 || dim > 4 || E < 1 || P > (1ull << 30)) return fail(FKD_ERR_INVALID, "bad FKDTREE1 header");
+  if (size != wire_bytes((int)dim, E, P))
+    return fail(FKD_ERR_INVALID, "FKDTREE1 size " + std::to_string(size) + " != " +
+                                     std::to_string(wire_bytes((int)dim, E, P)));
+  std::vector<uint64_t> off(E + 1);
+  std::memcpy(off.data(), b + kWireHeader, 8ull * (E + 1));
+  if (off[E] != P) return fail(FKD_ERR_INVALID, "FKDTREE1 offsets do not end at n_points");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  fkd_tree_s* t = nullptr;
+  FKD_TRY(create_common((int)dim, E, off.data(), &t));
+  const char* data = b + kWireHeader + 8ull * (E + 1);
+  int r = FKD_OK;
+  if (P) {
+    hipError_t e = hipMemcpyAsync(t->X.p, data, P * dim * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(t->ids.p, data + P * dim * 4, P * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) r = fail(FKD_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+  }
+  if (r == FKD_OK) r = finish_create(t, s);
+  if (r == FKD_OK && verify) {
+    // The tree is a unique function of its points: a valid buffer rebuilds to itself.
+    r = dispatch_dim((int)dim, [&](auto D) { return build_impl<decltype(D)::value>(t, s); });
+    if (r == FKD_OK && P) r = fill_u32(t->err.as<uint32_t>(), 0, 1, s);
+    if (r == FKD_OK && P) {
+      fkd::compare_kernel<<<grid_for(P * dim), 256, 0, s>>>(t->Tc.as<uint32_t>(), t->X.as<uint32_t>(), P * dim,
+                                                             t->err.as<uint32_t>());
+      fkd::compare_kernel<<<grid_for(P), 256, 0, s>>>(t->Tid.as<uint32_t>(), t->ids.as<uint32_t>(), P,
+                                                       t->err.as<uint32_t>());
+      r = launch_check("compare");
+    }
+    uint32_t err = 0;
+    if (r == FKD_OK && P) {
+      hipError_t e = hipMemcpyAsync(&err, t->err.p, 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) r = fail(FKD_ERR_HIP, std::string("verify: ") + hipGetErrorString(e));
+    }
+    if (r == FKD_OK && err) r = fail(FKD_ERR_INVALID, "FKDTREE1 arrays are not the k-d tree of their points");
+  } else if (r == FKD_OK) {
+    r = dispatch_dim((int)dim, [&](auto D) { return adopt_tree_impl<decltype(D)::value>(t, s); });
+    if (r == FKD_OK) {
+      hipError_t e = hipStreamSynchronize(s);
+      if (e != hipSuccess) r = fail(FKD_ERR_HIP, std::string("load: ") + hipGetErrorString(e));
+    }
+  }
+  if (r != FKD_OK) {
+    delete t;
+    return r;
+  }
+  *out = t;
+  return FKD_OK;
+}
 
 const char* fkd_last_error(void) { return g_err.c_str(); }
 

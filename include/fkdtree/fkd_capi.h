@@ -91,6 +91,24 @@ int fkd_tree_info(fkd_tree_t tree, int* dim, uint32_t* n_events, uint64_t* n_poi
  * many trees resident.  No reference counterpart. */
 int fkd_trim(fkd_tree_t tree, fkd_stream_t stream);
 
+/* Wire format of a built handle (little-endian), for spilling resident
+ * trees and reloading them without a build:
+ *   "FKDTREE1" (8 bytes), uint32 dim, uint32 n_events, uint64 n_points,
+ *   uint64 event_offsets[n_events + 1],
+ *   float coords[dim][n_points], uint32 ids[n_points]: the heap-ordered tree
+ *   arrays of every event, events end to end.
+ * fkd_serialize writes it into `buf` (NULL: only *size is set; synchronises
+ * `stream`).  Serialises what getDimensionVector / getIdVector expose
+ * (FKDTree_cpu.h:209-213); the reference has no file format. */
+int fkd_serialize(fkd_tree_t tree, void* buf, uint64_t capacity, uint64_t* size, fkd_stream_t stream);
+
+/* A built handle from the wire format: the arrays are uploaded as they are
+ * and the search records derived from them (no build); the handle's points
+ * are the tree's own, so fkd_build reproduces the same tree.  verify = 1
+ * rebuilds from the points instead and fails with FKD_ERR_INVALID unless
+ * every array comes out identical.  Synchronises `stream`. */
+int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t stream, fkd_tree_t* out);
+
 /* Launch parameters of the handle (chosen by fkdtree_amd/tune.py; results
  * never depend on them):
  *   "query_key_bits"  Morton bits per dimension of the box order (0 keeps

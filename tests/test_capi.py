@@ -53,3 +53,19 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(RuntimeError, match="missing"):
         _lib.lib()
+
+
+def test_deserialize_rejects_bad_buffers():
+    """Header checks run on the host before any device call."""
+    import ctypes as C
+
+    import numpy as np
+
+    from fkdtree_amd._lib import lib
+
+    h = C.c_void_p()
+    for data in (b"", b"NOTATREE" + bytes(40), b"FKDTREE1" + np.array([3, 1], np.uint32).tobytes() +
+                 np.array([10], np.uint64).tobytes() + np.array([0, 10], np.uint64).tobytes()):
+        buf = np.frombuffer(data + b"\0", dtype=np.uint8)
+        rc = lib().fkd_deserialize(buf.ctypes.data, len(data), 0, None, C.byref(h))
+        assert rc == -1 and h.value is None
