@@ -37,8 +37,13 @@ def device_info() -> dict:
     fr = C.c_size_t()
     tot = C.c_size_t()
     check(lib().fkd_device_info(name, 256, C.byref(cus), C.byref(fr), C.byref(tot)))
-    return {"name": name.value.decode(), "compute_units": cus.value, "free_bytes": fr.value,
-            "total_bytes": tot.value}
+    res, used = C.c_uint64(), C.c_uint64()
+    check(lib().fkd_pool_info(C.byref(res), C.byref(used)))
+    cached = max(res.value - used.value, 0)
+    # free_bytes: what the device reports free plus what the allocator's pool
+    # holds unused (reusable by stream-ordered allocations)
+    return {"name": name.value.decode(), "compute_units": cus.value, "free_bytes": fr.value + cached,
+            "device_free_bytes": fr.value, "pool_cached_bytes": cached, "total_bytes": tot.value}
 
 
 class Stream:
@@ -123,14 +128,21 @@ def _sh(stream) -> Optional[int]:
 class DeviceArray:
     """A device allocation holding a numpy-typed array."""
 
-    def __init__(self, shape, dtype, ptr: Optional[int] = None):
+    def __init__(self, shape, dtype, ptr: Optional[int] = None, stream=None):
+        """With `stream`, the memory is allocated and later freed in stream
+        order on it (no device-wide synchronisation); the array keeps the
+        stream alive."""
         self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
         self._own = ptr is None
+        self._stream = stream
         if ptr is None:
             p = C.c_void_p()
-            check(lib().fkd_malloc(C.byref(p), max(self.nbytes, 1)))
+            if stream is None:
+                check(lib().fkd_malloc(C.byref(p), max(self.nbytes, 1)))
+            else:
+                check(lib().fkd_malloc_async(C.byref(p), max(self.nbytes, 1), _sh(stream)))
             ptr = p.value
         self.ptr = ptr
 
@@ -147,6 +159,14 @@ class DeviceArray:
         check(lib().fkd_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, _sh(stream)))
         check(lib().fkd_stream_sync(_sh(stream)))
 
+    def upload_at(self, a: np.ndarray, byte_offset: int, stream=None) -> None:
+        """Enqueue a copy of host array `a` to byte_offset of this array; `a`
+        must stay alive until the stream has passed the copy."""
+        a = np.ascontiguousarray(a)
+        assert byte_offset + a.nbytes <= self.nbytes, (byte_offset, a.nbytes, self.nbytes)
+        if a.nbytes:
+            check(lib().fkd_memcpy_htod(self.ptr + byte_offset, a.ctypes.data, a.nbytes, _sh(stream)))
+
     def download(self, stream=None, count: Optional[int] = None) -> np.ndarray:
         if count is None:
             out = np.empty(self.shape, dtype=self.dtype)
@@ -157,7 +177,10 @@ class DeviceArray:
 
     def free(self):
         if self._own and self.ptr:
-            lib().fkd_free(self.ptr)
+            if self._stream is None:
+                lib().fkd_free(self.ptr)
+            else:
+                lib().fkd_free_async(self.ptr, _sh(self._stream))
         self.ptr = None
 
     def __del__(self):
@@ -226,6 +249,13 @@ class _TreeBase:
             check(lib().fkd_get_tree(self._h, c.ctypes.data, i.ctypes.data, _sh(self.stream)))
             self._mirror = (c, i)
         return self._mirror
+
+    def boxes_around(self, half_width: float, d_mins: int, d_maxs: int, d_qev: Optional[int] = None,
+                     stream=None) -> None:
+        """Enqueue fkd_boxes_around: boxes of half-width `half_width` around
+        every point (input order, SoA [dim][n]) and each box's event."""
+        check(lib().fkd_boxes_around(self._h, float(half_width), d_mins, d_maxs, d_qev,
+                                     _sh(stream if stream is not None else self.stream)))
 
     def serialize(self) -> bytes:
         """The wire format of the built tree (fkd_serialize): header, event
@@ -311,8 +341,14 @@ class _TreeBase:
         return b.value
 
     def close(self):
+        """Free the handle.  With a stream, its memory returns to the
+        allocator in stream order (the handle's work must be on that stream
+        or complete); without one, after a device-wide synchronisation."""
         if self._h:
-            lib().fkd_destroy(self._h)
+            if self.stream is None:
+                lib().fkd_destroy(self._h)
+            else:
+                lib().fkd_destroy_async(self._h, _sh(self.stream))
             self._h = None
 
     def __del__(self):
@@ -353,6 +389,12 @@ class FKDTree(_TreeBase):
         hi = np.asarray(max_point, dtype=np.float32).reshape(self.dim, 1)
         _, ids = self._search_host(lo, hi)
         return ids
+
+    # The reference's other search entry points (FKDTree_cpu.h:25-121,
+    # :171-207): the same id set, so the same search.
+    search_in_the_box_BFS = search_in_the_box
+    search_in_the_box_recursive = search_in_the_box
+    search_in_the_box_branchless = search_in_the_box
 
     def search_in_the_box_multiple(self, mins, maxs) -> np.ndarray:
         off, _ = self._search_host(mins, maxs)

@@ -23,11 +23,13 @@ _SIGS = {
     "fkd_create_events": (I32, [I32, U32, P, P, P, C.POINTER(P)]),
     "fkd_create_device": (I32, [I32, U32, P, P, P, P, C.POINTER(P)]),
     "fkd_destroy": (None, [P]),
+    "fkd_destroy_async": (I32, [P, P]),
     "fkd_build": (I32, [P, P]),
     "fkd_get_tree": (I32, [P, P, P, P]),
     "fkd_tree_device": (I32, [P, C.POINTER(P), C.POINTER(P)]),
     "fkd_tree_info": (I32, [P, C.POINTER(I32), C.POINTER(U32), C.POINTER(U64)]),
     "fkd_trim": (I32, [P, P]),
+    "fkd_boxes_around": (I32, [P, C.c_float, P, P, P, P]),
     "fkd_serialize": (I32, [P, P, U64, C.POINTER(U64), P]),
     "fkd_deserialize": (I32, [P, U64, I32, P, C.POINTER(P)]),
     "fkd_set_tuning": (I32, [P, C.c_char_p, U32]),
@@ -47,6 +49,9 @@ _SIGS = {
     "fkd_device_info": (I32, [C.c_char_p, SZ, C.POINTER(I32), C.POINTER(SZ), C.POINTER(SZ)]),
     "fkd_malloc": (I32, [C.POINTER(P), SZ]),
     "fkd_free": (I32, [P]),
+    "fkd_malloc_async": (I32, [C.POINTER(P), SZ, P]),
+    "fkd_free_async": (I32, [P, P]),
+    "fkd_pool_info": (I32, [C.POINTER(U64), C.POINTER(U64)]),
     "fkd_memcpy_htod": (I32, [P, P, SZ, P]),
     "fkd_memcpy_dtoh": (I32, [P, P, SZ, P]),
     "fkd_copy_u32": (I32, [P, P, U64, P]),

@@ -620,6 +620,31 @@ __global__ void compare_kernel(const uint32_t* __restrict__ a, const uint32_t* _
   if (__ballot(diff) != 0ull && lane_id() == 0) atomicOr(err, 1u);
 }
 
+// Closed boxes [c - hw, c + hw] around every point of a batch, in input
+// order (SoA [DIM][P], one float rounding per bound), and the event of every
+// box: queries that search each point's neighbourhood in its own tree.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void boxes_around_kernel(const float* __restrict__ X, uint32_t P, float hw,
+                                                                TileMap tm, float* __restrict__ mins,
+                                                                float* __restrict__ maxs, uint32_t* __restrict__ qev) {
+  const uint32_t t = blockIdx.x;
+  const uint32_t e = tm.tile_ev[t];
+  const uint32_t lt = t - tm.tb[e];
+  const uint32_t o = tm.off[e];
+  const uint32_t n = tm.off[e + 1] - o;
+  for (uint32_t j = lt * (uint32_t)kLevelTile + threadIdx.x; j < min(n, (lt + 1) * (uint32_t)kLevelTile);
+       j += kThreads) {
+    const uint64_t i = (uint64_t)o + j;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      const float c = X[(uint64_t)m * P + i];
+      mins[(uint64_t)m * P + i] = c - hw;
+      maxs[(uint64_t)m * P + i] = c + hw;
+    }
+    if (qev) qev[i] = e;
+  }
+}
+
 // Point records (coordinates, id) in input order, one 16-byte gather per
 // node for the output (DIM <= 3).
 template <int DIM>

@@ -39,21 +39,62 @@ int fail(int code, const std::string& msg) {
 
 constexpr uint32_t kMaxPoints = 1u << 30;
 
+// Stream on which the current call allocates and frees its buffers
+// (stream-ordered allocator), set by every entry point that enqueues work;
+// outside such a call (fkd_destroy) buffers are freed synchronously.
+thread_local hipStream_t tl_stream = nullptr;
+thread_local bool tl_ordered = false;
+
+struct StreamScope {
+  hipStream_t prev;
+  bool prev_ordered;
+  explicit StreamScope(hipStream_t s) : prev(tl_stream), prev_ordered(tl_ordered) {
+    tl_stream = s;
+    tl_ordered = true;
+  }
+  ~StreamScope() {
+    tl_stream = prev;
+    tl_ordered = prev_ordered;
+  }
+};
+
+// The device's default pool keeps freed memory for reuse instead of handing
+// it back at every synchronisation.
+void keep_pool_memory() {
+  static bool done[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || done[dev]) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  done[dev] = true;
+}
+
+hipError_t dev_alloc(void** p, size_t bytes) {
+  if (!tl_ordered) return hipMalloc(p, bytes);
+  keep_pool_memory();
+  return hipMallocAsync(p, bytes, tl_stream);
+}
+
+hipError_t dev_free(void* p) { return tl_ordered ? hipFreeAsync(p, tl_stream) : hipFree(p); }
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) (void)dev_free(p);
   }
   int ensure(size_t want) {
     if (want <= bytes) return FKD_OK;
     if (p) {
-      FKD_HIP(hipFree(p));
+      FKD_HIP(dev_free(p));
       p = nullptr;
       bytes = 0;
     }
     want = std::max<size_t>(want, 256);
-    FKD_HIP(hipMalloc(&p, want));
+    FKD_HIP(dev_alloc(&p, want));
     bytes = want;
     return FKD_OK;
   }
@@ -62,7 +103,7 @@ struct DevBuf {
     return static_cast<T*>(p);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) (void)dev_free(p);
     p = nullptr;
     bytes = 0;
   }
@@ -781,6 +822,7 @@ int fkd_serialize(fkd_tree_t t, void* buf, uint64_t capacity, uint64_t* size, fk
 }
 
 int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t stream, fkd_tree_t* out) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!buf || !out) return fail(FKD_ERR_INVALID, "buf / out is NULL");
   *out = nullptr;
   const char* b = static_cast<const char*>(buf);
@@ -846,6 +888,7 @@ const char* fkd_last_error(void) { return g_err.c_str(); }
 
 int fkd_create_events(int dim, uint32_t n_events, const uint64_t* event_offsets, const void* host_points,
                       fkd_stream_t stream, fkd_tree_t* out) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   hipStream_t s = static_cast<hipStream_t>(stream);
   fkd_tree_s* t = nullptr;
   FKD_TRY(create_common(dim, n_events, event_offsets, &t));
@@ -870,6 +913,7 @@ int fkd_create(int dim, uint32_t n, const void* host_points, fkd_stream_t stream
 
 int fkd_create_device(int dim, uint32_t n_events, const uint64_t* event_offsets, const float* dev_coords,
                       const uint32_t* dev_ids, fkd_stream_t stream, fkd_tree_t* out) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   hipStream_t s = static_cast<hipStream_t>(stream);
   fkd_tree_s* t = nullptr;
   FKD_TRY(create_common(dim, n_events, event_offsets, &t));
@@ -895,7 +939,14 @@ int fkd_create_device(int dim, uint32_t n_events, const uint64_t* event_offsets,
 
 void fkd_destroy(fkd_tree_t tree) { delete tree; }
 
+int fkd_destroy_async(fkd_tree_t tree, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
+  delete tree;
+  return launch_check("destroy");
+}
+
 int fkd_build(fkd_tree_t t, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   hipStream_t s = static_cast<hipStream_t>(stream);
   return dispatch_dim(t->dim, [&](auto D) { return build_impl<decltype(D)::value>(t, s); });
@@ -931,6 +982,7 @@ int fkd_tree_info(fkd_tree_t t, int* dim, uint32_t* n_events, uint64_t* n_points
 
 int fkd_search_count(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                      uint64_t* dev_offsets, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
   if (!dev_offsets || (nq && (!mins || !maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
@@ -943,6 +995,7 @@ int fkd_search_count(fkd_tree_t t, uint32_t nq, const float* mins, const float* 
 
 int fkd_search_fill(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                     const uint64_t* dev_offsets, uint32_t* out, uint64_t capacity, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
   if (!dev_offsets || (nq && (!mins || !maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
@@ -956,6 +1009,7 @@ int fkd_search_fill(fkd_tree_t t, uint32_t nq, const float* mins, const float* m
 
 int fkd_search_stats(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                      uint32_t* dev_counts, uint32_t* dev_visited, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
   if (nq && (!mins || !maxs || !dev_counts || !dev_visited)) return fail(FKD_ERR_INVALID, "NULL search buffer");
@@ -968,6 +1022,7 @@ int fkd_search_stats(fkd_tree_t t, uint32_t nq, const float* mins, const float* 
 
 int fkd_search(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                uint64_t* dev_offsets, uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
   if (!dev_offsets || (nq && (!mins || !maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
@@ -976,6 +1031,21 @@ int fkd_search(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, 
   hipStream_t s = static_cast<hipStream_t>(stream);
   return dispatch_dim(t->dim, [&](auto D) {
     return search_single_impl<decltype(D)::value>(t, nq, mins, maxs, qev, dev_offsets, dev_ids_out, capacity, s);
+  });
+}
+
+int fkd_boxes_around(fkd_tree_t t, float half_width, float* dev_mins, float* dev_maxs, uint32_t* dev_query_events,
+                     fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (t->P && (!dev_mins || !dev_maxs)) return fail(FKD_ERR_INVALID, "NULL box buffer");
+  if (!(half_width >= 0.f)) return fail(FKD_ERR_INVALID, "half_width must be >= 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!t->P) return FKD_OK;
+  const fkd::TileMap tm = t->level_tiles.map(t->off.as<uint32_t>());
+  return dispatch_dim(t->dim, [&](auto D) {
+    fkd::boxes_around_kernel<decltype(D)::value><<<tm.T, fkd::kThreads, 0, s>>>(
+        t->X.as<float>(), t->P, half_width, tm, dev_mins, dev_maxs, dev_query_events);
+    return launch_check("boxes_around");
   });
 }
 
@@ -1001,6 +1071,7 @@ int fkd_search_status(fkd_tree_t t, fkd_stream_t stream, uint64_t* total, int* c
 int fkd_graph_create(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                      uint64_t* dev_offsets, uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream,
                      fkd_graph_t* out) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t || !out) return fail(FKD_ERR_INVALID, "tree / out is NULL");
   *out = nullptr;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1051,6 +1122,7 @@ int fkd_graph_launch(fkd_graph_t g, fkd_stream_t stream) {
 void fkd_graph_destroy(fkd_graph_t g) { delete g; }
 
 int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->Xr, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
@@ -1103,6 +1175,7 @@ int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
 
 int fkd_search_host(fkd_tree_t t, uint32_t nq, const float* host_mins, const float* host_maxs,
                     const uint32_t* host_qev, uint64_t* host_offsets, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   if (!host_offsets || (nq && (!host_mins || !host_maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1139,6 +1212,7 @@ int fkd_search_host(fkd_tree_t t, uint32_t nq, const float* host_mins, const flo
 }
 
 int fkd_fetch_results(fkd_tree_t t, uint32_t* host_ids, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (t->last_total) {
@@ -1165,6 +1239,19 @@ int fkd_device_info(char* name, size_t name_len, int* compute_units, size_t* fre
   return FKD_OK;
 }
 
+int fkd_pool_info(uint64_t* reserved, uint64_t* used) {
+  int dev = 0;
+  FKD_HIP(hipGetDevice(&dev));
+  hipMemPool_t pool;
+  FKD_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t r = 0, u = 0;
+  FKD_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &r));
+  FKD_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &u));
+  if (reserved) *reserved = r;
+  if (used) *used = u;
+  return FKD_OK;
+}
+
 int fkd_malloc(void** ptr, size_t bytes) {
   if (!ptr) return fail(FKD_ERR_INVALID, "ptr is NULL");
   FKD_HIP(hipMalloc(ptr, std::max<size_t>(bytes, 256)));
@@ -1172,6 +1259,17 @@ int fkd_malloc(void** ptr, size_t bytes) {
 }
 int fkd_free(void* ptr) {
   if (ptr) FKD_HIP(hipFree(ptr));
+  return FKD_OK;
+}
+int fkd_malloc_async(void** ptr, size_t bytes, fkd_stream_t stream) {
+  if (!ptr) return fail(FKD_ERR_INVALID, "ptr is NULL");
+  StreamScope scope(static_cast<hipStream_t>(stream));
+  FKD_HIP(dev_alloc(ptr, std::max<size_t>(bytes, 256)));
+  return FKD_OK;
+}
+int fkd_free_async(void* ptr, fkd_stream_t stream) {
+  StreamScope scope(static_cast<hipStream_t>(stream));
+  if (ptr) FKD_HIP(dev_free(ptr));
   return FKD_OK;
 }
 int fkd_memcpy_htod(void* dst, const void* src, size_t bytes, fkd_stream_t stream) {

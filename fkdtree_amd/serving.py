@@ -117,27 +117,45 @@ def plan_batch(queue: Sequence[Request], budget_bytes: int, max_points: int, max
 
 
 class _Batch:
+    """One batch on its stream.  Ingestion is device side: each request's
+    coordinates and ids are copied straight into one device array at the
+    event's offset (no host concatenation or transposition), the handle is
+    created from it, and the boxes around every point are made on the device
+    (fkd_boxes_around).  All its device memory is allocated and freed in
+    stream order on the batch's stream."""
+
     def __init__(self, idx: int, members: List[Request], server: "EventServer", stream):
         import fkdtree_amd as F
 
         self.idx = idx
         self.members = members
         self.stream = stream
-        coords = np.concatenate([r.coords for r in members], axis=1)
-        ids = np.concatenate([r.ids for r in members])
+        dim = int(members[0].coords.shape[0])
         offs = np.zeros(len(members) + 1, dtype=np.uint64)
         offs[1:] = np.cumsum([r.n for r in members])
         self.offsets = offs
-        self.P = int(offs[-1])
-        self.handle = F.FKDEvents(coords, ids, offs, stream=stream)
-        boxes = gen.boxes_around(coords, server.half_width)
-        qev = np.repeat(np.arange(len(members), dtype=np.uint32), np.diff(offs).astype(np.int64))
-        self.d_mins = F.DeviceArray.from_host(boxes.mins, stream)
-        self.d_maxs = F.DeviceArray.from_host(boxes.maxs, stream)
-        self.d_qev = F.DeviceArray.from_host(qev, stream)
-        self.d_off = F.DeviceArray(self.P + 1, np.uint64)
-        self.cap = max(1, int(self.P * server.hits_per_point))
-        self.d_out = F.DeviceArray(self.cap, np.uint32)
+        self.P = P = int(offs[-1])
+        d_c = F.DeviceArray((dim, max(P, 1)), np.float32, stream=stream)
+        d_i = F.DeviceArray(max(P, 1), np.uint32, stream=stream)
+        hold = []  # host sources stay alive until the copies have run
+        for r, a in zip(members, offs[:-1]):
+            a = int(a)
+            for m in range(dim):
+                hold.append(np.ascontiguousarray(r.coords[m], dtype=np.float32))
+                d_c.upload_at(hold[-1], 4 * (m * P + a), stream)
+            hold.append(np.ascontiguousarray(r.ids, dtype=np.uint32))
+            d_i.upload_at(hold[-1], 4 * a, stream)
+        self.handle = F.FKDEvents.from_device(dim, offs, d_c.ptr, d_i.ptr, stream)  # synchronises the stream
+        del hold
+        d_c.free()
+        d_i.free()
+        self.d_mins = F.DeviceArray((dim, max(P, 1)), np.float32, stream=stream)
+        self.d_maxs = F.DeviceArray((dim, max(P, 1)), np.float32, stream=stream)
+        self.d_qev = F.DeviceArray(max(P, 1), np.uint32, stream=stream)
+        self.handle.boxes_around(server.half_width, self.d_mins.ptr, self.d_maxs.ptr, self.d_qev.ptr, stream)
+        self.d_off = F.DeviceArray(P + 1, np.uint64, stream=stream)
+        self.cap = max(1, int(P * server.hits_per_point))
+        self.d_out = F.DeviceArray(self.cap, np.uint32, stream=stream)
         self.done_ev = F.Event()
         self.launch()
 
@@ -224,7 +242,7 @@ class EventServer:
                     if total > b.cap:
                         import fkdtree_amd as F
                         b.cap = total
-                        b.d_out = F.DeviceArray(total, np.uint32)
+                        b.d_out = F.DeviceArray(total, np.uint32, stream=b.stream)
                     b.launch(build=False)
                     continue
                 t_done = time.perf_counter() - t0

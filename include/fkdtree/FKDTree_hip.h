@@ -77,6 +77,46 @@ class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
     return counts;
   }
 
+  // The reference's other search entry points (FKDTree_cpu.h:25-121,
+  // :171-207) return the same id set in other orders; here they are one
+  // search.  (_branchless reads past the arrays on even N in the reference,
+  // :58-60; this one does not.)
+  std::vector<unsigned int> search_in_the_box_BFS(const Point& minPoint, const Point& maxPoint) {
+    return search_in_the_box(minPoint, maxPoint);
+  }
+  std::vector<unsigned int> search_in_the_box_recursive(const Point& minPoint, const Point& maxPoint) {
+    return search_in_the_box(minPoint, maxPoint);
+  }
+  std::vector<unsigned int> search_in_the_box_branchless(const Point& minPoint, const Point& maxPoint) {
+    return search_in_the_box(minPoint, maxPoint);
+  }
+
+  // Wire format of the built tree (fkd_serialize) and a tree restored from
+  // it without a build (fkd_deserialize; verify rebuilds and compares).
+  std::vector<unsigned char> serialize() {
+    std::lock_guard<std::mutex> g(mu_);
+    uint64_t size = 0;
+    check(fkd_serialize(tree_, nullptr, 0, &size, stream_));
+    std::vector<unsigned char> buf(size);
+    check(fkd_serialize(tree_, buf.data(), size, &size, stream_));
+    return buf;
+  }
+  FKDTree_HIP(const std::vector<unsigned char>& wire, bool verify, fkd_stream_t stream = nullptr)
+      : FKDTree<TYPE, numberOfDimensions>(std::vector<Point>()), stream_(stream) {
+    check(fkd_deserialize(wire.data(), wire.size(), verify ? 1 : 0, stream_, &tree_));
+    int dim = 0;
+    uint32_t events = 0;
+    uint64_t n = 0;
+    check(fkd_tree_info(tree_, &dim, &events, &n));
+    if (dim != (int)numberOfDimensions || events != 1) {
+      fkd_destroy(tree_);
+      tree_ = nullptr;
+      throw std::invalid_argument("fkdtree: wire format holds another dimension or several events");
+    }
+    n_ = n;
+    built_ = true;
+  }
+
   // Ids of box q are ids[offsets[q] .. offsets[q+1]).
   void search_in_the_box_multiple_csr(const std::vector<Point>& minPoints, const std::vector<Point>& maxPoints,
                                       std::vector<uint64_t>& offsets, std::vector<unsigned int>& ids) {

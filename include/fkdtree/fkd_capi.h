@@ -68,6 +68,11 @@ int fkd_create_device(int dim, uint32_t n_events, const uint64_t* event_offsets,
 
 void fkd_destroy(fkd_tree_t tree);
 
+/* As fkd_destroy, but the device memory returns to the allocator in stream
+ * order on `stream` instead of after a device-wide synchronisation: every
+ * call on the handle must be ordered before it on `stream` (or complete). */
+int fkd_destroy_async(fkd_tree_t tree, fkd_stream_t stream);
+
 /* Build every tree of the handle.  Enqueues only.
  * Replaces: FKDTree_CPU::build (FKDTree_cpu.h:221-277),
  *           FKDTree_OpenCL::build (FKDTree_opencl.h:208-246). */
@@ -145,6 +150,15 @@ int fkd_search(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float*
                const uint32_t* dev_query_events, uint64_t* dev_offsets, uint32_t* dev_ids_out,
                uint64_t capacity, fkd_stream_t stream);
 
+/* Boxes [c - half_width, c + half_width] around every point of the handle,
+ * in the order the points were given (SoA [dim][n_points], one float
+ * rounding per bound), and the event of every box (dev_query_events may be
+ * NULL): the boxes of a self-search of every point in its own tree, made on
+ * the device.  Enqueues only.  No reference counterpart (the drivers form
+ * such boxes on the host around every point, main.cpp:174-187). */
+int fkd_boxes_around(fkd_tree_t tree, float half_width, float* dev_mins, float* dev_maxs,
+                     uint32_t* dev_query_events, fkd_stream_t stream);
+
 /* Synchronises `stream`; *total = hits of the last fkd_search on this tree,
  * *complete = 1 when every id was written. */
 int fkd_search_status(fkd_tree_t tree, fkd_stream_t stream, uint64_t* total, int* complete);
@@ -191,6 +205,14 @@ int fkd_device_info(char* name, size_t name_len, int* compute_units, size_t* fre
                     size_t* total_bytes);
 int fkd_malloc(void** ptr, size_t bytes);
 int fkd_free(void* ptr);
+/* Stream-ordered device allocation (hipMallocAsync / hipFreeAsync on the
+ * device's default pool, which keeps freed memory for reuse). */
+int fkd_malloc_async(void** ptr, size_t bytes, fkd_stream_t stream);
+/* Bytes the default pool holds from the device, and of those in use: the
+ * difference is free for stream-ordered allocations though hipMemGetInfo
+ * counts it as used. */
+int fkd_pool_info(uint64_t* reserved, uint64_t* used);
+int fkd_free_async(void* ptr, fkd_stream_t stream);
 int fkd_memcpy_htod(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
 int fkd_memcpy_dtoh(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
 /* Streaming device copy of n uint32 (a plain kernel, for counter calibration). */

@@ -77,6 +77,39 @@ int main(int argc, char** argv) {
     }
   }
 
+  // the reference's other search names give the same ids
+  for (uint32_t i = 0; i < q && i < 16; ++i) {
+    auto a = tree.search_in_the_box_BFS(mins[i], maxs[i]);
+    auto b = tree.search_in_the_box_recursive(mins[i], maxs[i]);
+    auto c = tree.search_in_the_box_branchless(mins[i], maxs[i]);
+    if (a.size() != res[i].size() || b.size() != res[i].size() || c.size() != res[i].size()) {
+      std::cerr << "variant mismatch at box " << i << "\n";
+      return 1;
+    }
+  }
+
+  // a tree restored from its wire format, unverified and verified, is the same tree
+  const std::vector<unsigned char> wire = tree.serialize();
+  for (int verify = 0; verify < 2; ++verify) {
+    FKDTree_HIP<float, 3> back(wire, verify != 0);
+    for (int d = 0; d < 3; ++d)
+      if (back.getDimensionVector(d) != tree.getDimensionVector(d)) {
+        std::cerr << "restored tree differs in dimension " << d << "\n";
+        return 1;
+      }
+    if (back.getIdVector() != tree.getIdVector()) {
+      std::cerr << "restored ids differ\n";
+      return 1;
+    }
+    std::vector<uint64_t> off2;
+    std::vector<unsigned int> all2;
+    back.search_in_the_box_multiple_csr(mins, maxs, off2, all2);
+    if (off2 != off) {
+      std::cerr << "restored tree searches differently\n";
+      return 1;
+    }
+  }
+
   std::ofstream out(argv[2], std::ios::binary);
   for (int d = 0; d < 3; ++d) out.write(reinterpret_cast<const char*>(base.getDimensionVector(d).data()), n * 4);
   out.write(reinterpret_cast<const char*>(base.getIdVector().data()), n * 4);

@@ -53,6 +53,8 @@ def test_single_box_api(require_gpu):
     got = np.sort(t.search_in_the_box(lo, hi))
     inside = np.all((p.coords >= np.float32(lo)[:, None]) & (p.coords <= np.float32(hi)[:, None]), axis=0)
     assert np.array_equal(got, np.sort(p.ids[inside]))
+    for variant in (t.search_in_the_box_BFS, t.search_in_the_box_recursive, t.search_in_the_box_branchless):
+        assert np.array_equal(np.sort(variant(lo, hi)), got)
 
 
 def test_uniform_100k_all_points(require_gpu):

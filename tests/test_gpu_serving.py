@@ -67,3 +67,26 @@ def test_tiny_budget_forces_single_event_batches_and_eviction(require_gpu):
         got = rep.results[r.event]
         assert np.array_equal(sorted_sets(got.offsets, got.ids), sorted_sets(off, ids))
     srv.close()
+
+
+def test_boxes_around_matches_host(require_gpu):
+    """fkd_boxes_around: the same float32 boxes as the host formula
+    (main.cpp:177-186 forms them around every point) and each box's event."""
+    import fkdtree_amd as F
+    sizes = [3, 0, 1000, 1, 4500]
+    evs = [gen.serving_event(s, 40 + i) if s else gen.PointSet(np.zeros((3, 0), np.float32), np.zeros(0, np.uint32))
+           for i, s in enumerate(sizes)]
+    b = gen.concat_events(evs)
+    s = F.Stream()
+    t = F.FKDEvents(b.points.coords, b.points.ids, b.offsets, stream=s)
+    P = b.points.n
+    dmi = F.DeviceArray((3, P), np.float32, stream=s)
+    dma = F.DeviceArray((3, P), np.float32, stream=s)
+    dq = F.DeviceArray(P, np.uint32, stream=s)
+    t.boxes_around(0.01, dmi.ptr, dma.ptr, dq.ptr)
+    want = gen.boxes_around(b.points.coords, 0.01)
+    assert np.array_equal(dmi.download(s).view(np.uint32), want.mins.view(np.uint32))
+    assert np.array_equal(dma.download(s).view(np.uint32), want.maxs.view(np.uint32))
+    assert np.array_equal(dq.download(s), np.repeat(np.arange(len(sizes), dtype=np.uint32),
+                                                      np.diff(b.offsets).astype(np.int64)))
+    t.close()
