@@ -42,9 +42,9 @@ def parse():
     ap.add_argument("--events", type=int, default=256,
                     help="events of the batched workload (first N of the 2,048-event set; 0 skips)")
     ap.add_argument("--event-steps", type=int, default=3)
-    ap.add_argument("--serve-events", type=int, default=64,
+    ap.add_argument("--serve-events", type=int, default=128,
                     help="events of the serving stream (0 skips)")
-    ap.add_argument("--serve-rates", default="1000,5000", help="Poisson arrival rates (events/s)")
+    ap.add_argument("--serve-rates", default="1000,5000,20000", help="Poisson arrival rates (events/s)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 256 MiB copy kernel (counter calibration) and exit")
     return ap.parse_args()
@@ -201,16 +201,19 @@ def main():
 def bench_serving(n_events: int, rates) -> list:
     """p50 / p99 latency and events/s of the continuous-batching server for
     Poisson streams of n_events events (sizes log-uniform in [1,000, 500,000]
-    points, mixed distribution; every point searched with half-width 0.01)."""
+    points, mixed distribution; every point searched with half-width 0.01),
+    received as FKDPoint records into a page-locked ingest ring."""
     from fkdtree_amd.serving import EventServer, poisson_requests
 
     out = []
     for rate in rates:
-        reqs = poisson_requests(n_events, rate)
-        srv = EventServer(n_streams=3, max_batch_points=4_000_000, max_batch_events=256)
+        # events arrive as FKDPoint records into the page-locked ingest ring
+        reqs = poisson_requests(n_events, rate, records=True)
+        ring = max(16_000_000, 2 * max(r.n for r in reqs))
+        srv = EventServer(n_streams=3, max_batch_points=4_000_000, max_batch_events=256, ring_points=ring)
         srv.run(reqs[: min(4, n_events)])          # warm-up (first allocations)
         srv.close()
-        srv = EventServer(n_streams=3, max_batch_points=4_000_000, max_batch_events=256)
+        srv = EventServer(n_streams=3, max_batch_points=4_000_000, max_batch_events=256, ring_points=ring)
         rep = srv.run(reqs)
         srv.close()
         d = rep.summary()

@@ -190,6 +190,36 @@ class DeviceArray:
             pass
 
 
+class _PinnedBlock:
+    """Owner of one page-locked host allocation (fkd_host_alloc)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check(lib().fkd_host_alloc(C.byref(p), max(int(nbytes), 1)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().fkd_host_free(self.ptr)
+        except Exception:
+            pass
+        self.ptr = None
+
+
+def pinned_empty(shape, dtype) -> np.ndarray:
+    """A numpy array in page-locked host memory; the memory lives as long as
+    the array (or any view of it)."""
+    dt = np.dtype(dtype)
+    shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+    nbytes = int(np.prod(shape, dtype=np.int64)) * dt.itemsize
+    blk = _PinnedBlock(nbytes)
+    raw = (C.c_uint8 * max(nbytes, 1)).from_address(blk.ptr)
+    raw._fkd_block = blk  # the ctypes buffer keeps the allocation alive
+    return np.frombuffer(raw, dtype=np.uint8, count=nbytes).view(dt).reshape(shape)
+
+
 def _as_soa(coords, dim=None) -> np.ndarray:
     c = np.ascontiguousarray(coords, dtype=np.float32)
     if c.ndim != 2:
@@ -415,6 +445,23 @@ class FKDEvents(_TreeBase):
                                       _sh(stream), C.byref(h)))
         self.offsets = off
         self._init_handle(h, stream)
+
+    @classmethod
+    def from_records(cls, records: np.ndarray, offsets, stream=None) -> "FKDEvents":
+        """From FKDPoint records (uint32 [n, dim+1]: coordinate bits, id),
+        events end to end: one copy to the device (asynchronous at full rate
+        from page-locked memory, see pinned_empty), transposed there."""
+        self = cls.__new__(cls)
+        rec = records if records.flags.c_contiguous else np.ascontiguousarray(records)
+        if rec.dtype != np.uint32 or rec.ndim != 2:
+            raise ValueError("records must be uint32 [n, dim+1]")
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().fkd_create_events(rec.shape[1] - 1, off.shape[0] - 1, off.ctypes.data,
+                                      rec.ctypes.data if rec.shape[0] else None, _sh(stream), C.byref(h)))
+        self.offsets = off
+        self._init_handle(h, stream)
+        return self
 
     @classmethod
     def from_device(cls, dim: int, offsets, d_coords: int, d_ids: int, stream=None) -> "FKDEvents":

@@ -52,9 +52,12 @@ _SIGS = {
     "fkd_malloc_async": (I32, [C.POINTER(P), SZ, P]),
     "fkd_free_async": (I32, [P, P]),
     "fkd_pool_info": (I32, [C.POINTER(U64), C.POINTER(U64)]),
+    "fkd_host_alloc": (I32, [C.POINTER(P), SZ]),
+    "fkd_host_free": (I32, [P]),
     "fkd_memcpy_htod": (I32, [P, P, SZ, P]),
     "fkd_memcpy_dtoh": (I32, [P, P, SZ, P]),
     "fkd_copy_u32": (I32, [P, P, U64, P]),
+    "fkd_gather_u64": (I32, [P, P, P, U64, P]),
     "fkd_stream_create": (I32, [C.POINTER(P)]),
     "fkd_stream_destroy": (I32, [P]),
     "fkd_stream_sync": (I32, [P]),

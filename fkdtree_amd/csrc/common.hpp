@@ -124,6 +124,13 @@ __global__ void copy_u32_kernel(uint32_t* __restrict__ dst, const uint32_t* __re
     dst[i] = src[i];
 }
 
+// dst[i] = src[idx[i]]
+__global__ void gather_u64_kernel(uint64_t* __restrict__ dst, const uint64_t* __restrict__ src,
+                                  const uint64_t* __restrict__ idx, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[idx[i]];
+}
+
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t v, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)

@@ -25,6 +25,13 @@ The server drives the GPU from one host thread:
   reports free.  Resident trees are evicted first, then the server waits for
   batches in flight to finish.
 
+* Ingestion: events that arrive as FKDPoint records (uint32 [n, dim+1],
+  `Request.records`) are written into a page-locked ring (`IngestRing`) as
+  they arrive; a batch of consecutive events is then one contiguous range of
+  the ring, handed to fkd_create_events as it lies (one asynchronous copy at
+  full link rate, transposed on the device).  Events given as SoA arrays are
+  copied per event into the batch's device arrays instead.
+
 Latency of an event = completion time of its batch - arrival time (host
 clock).  `ServeReport` holds p50 / p99 latency and events per second.
 """
@@ -43,13 +50,82 @@ from . import gen
 @dataclass
 class Request:
     event: int
-    coords: np.ndarray      # float32 [dim, n]
-    ids: np.ndarray         # uint32 [n]
-    arrival: float = 0.0    # seconds after the start of the stream
+    coords: Optional[np.ndarray]    # float32 [dim, n], or None when `records` is given
+    ids: Optional[np.ndarray]       # uint32 [n]
+    arrival: float = 0.0            # seconds after the start of the stream
+    records: Optional[np.ndarray] = None  # FKDPoint records uint32 [n, dim+1] (coordinate bits, id)
+    ring_start: int = -1            # slot of the records in the ingest ring, once received
 
     @property
     def n(self) -> int:
-        return int(self.ids.shape[0])
+        return int(self.records.shape[0] if self.records is not None else self.ids.shape[0])
+
+    @property
+    def dim(self) -> int:
+        return int(self.records.shape[1] - 1 if self.records is not None else self.coords.shape[0])
+
+
+class IngestRing:
+    """Ring of FKDPoint records (uint32 [capacity, dim+1]: coordinate bits,
+    then the id; FKDPoint.h:13-104) in page-locked host memory, into which
+    arriving events are written in arrival order.  An event occupies one
+    contiguous range; an event that does not fit before the end starts again
+    at slot 0 (the tail end is skipped).  Space is released in the same
+    order.  The reference's OpenCL backend instead unpacks every point on the
+    host (FKDTree_opencl.h:22-33)."""
+
+    def __init__(self, capacity: int, dim: int = 3, buffer: Optional[np.ndarray] = None):
+        if buffer is None:
+            import fkdtree_amd as F
+            buffer = F.pinned_empty((capacity, dim + 1), np.uint32)
+            buffer.fill(0)  # touch every page once, before events arrive
+        assert buffer.shape == (capacity, dim + 1) and buffer.dtype == np.uint32
+        self.buf = buffer
+        self.capacity = capacity
+        self.dim = dim
+        self.tail = 0     # next free slot
+        self.used = 0     # slots held, skipped tail ends included
+        self.spans: "collections.deque[Tuple[int, int]]" = collections.deque()  # (slots held, n)
+
+    def put(self, records: np.ndarray) -> Optional[int]:
+        """Copy an event's records in; its first slot, or None when full."""
+        n = int(records.shape[0])
+        if n > self.capacity:
+            raise ValueError(f"event of {n} points exceeds the ring ({self.capacity})")
+        start, skip = self.tail, 0
+        if start + n > self.capacity:
+            start, skip = 0, self.capacity - self.tail
+        if self.used + skip + n > self.capacity:
+            return None
+        self.buf[start:start + n] = records
+        self.tail = (start + n) % self.capacity
+        self.used += skip + n
+        self.spans.append((skip + n, n))
+        return start
+
+    def release(self, count: int) -> None:
+        """Free the oldest `count` events."""
+        for _ in range(count):
+            held, _ = self.spans.popleft()
+            self.used -= held
+        if not self.spans:
+            self.tail = 0
+            self.used = 0
+
+    def view(self, start: int, n: int) -> np.ndarray:
+        return self.buf[start:start + n]
+
+
+def contiguous_prefix(queue: Sequence[Request], k: int) -> int:
+    """How many of the first k queued events lie end to end in the ring."""
+    if k <= 1:
+        return k
+    nxt = queue[0].ring_start + queue[0].n
+    for i in range(1, k):
+        if queue[i].ring_start < 0 or queue[i].ring_start != nxt:
+            return i
+        nxt += queue[i].n
+    return k
 
 
 @dataclass
@@ -130,25 +206,18 @@ class _Batch:
         self.idx = idx
         self.members = members
         self.stream = stream
-        dim = int(members[0].coords.shape[0])
+        dim = members[0].dim
         offs = np.zeros(len(members) + 1, dtype=np.uint64)
         offs[1:] = np.cumsum([r.n for r in members])
         self.offsets = offs
         self.P = P = int(offs[-1])
-        d_c = F.DeviceArray((dim, max(P, 1)), np.float32, stream=stream)
-        d_i = F.DeviceArray(max(P, 1), np.uint32, stream=stream)
-        hold = []  # host sources stay alive until the copies have run
-        for r, a in zip(members, offs[:-1]):
-            a = int(a)
-            for m in range(dim):
-                hold.append(np.ascontiguousarray(r.coords[m], dtype=np.float32))
-                d_c.upload_at(hold[-1], 4 * (m * P + a), stream)
-            hold.append(np.ascontiguousarray(r.ids, dtype=np.uint32))
-            d_i.upload_at(hold[-1], 4 * a, stream)
-        self.handle = F.FKDEvents.from_device(dim, offs, d_c.ptr, d_i.ptr, stream)  # synchronises the stream
-        del hold
-        d_c.free()
-        d_i.free()
+        if members[0].ring_start >= 0:
+            # one range of the page-locked ring: one copy, transposed on the device
+            recs = server.ring.view(members[0].ring_start, P)
+            self.handle = F.FKDEvents.from_records(recs, offs, stream)  # synchronises the stream
+            server.ring.release(len(members))
+        else:
+            self.handle = self._from_soa(members, offs, dim, stream)
         self.d_mins = F.DeviceArray((dim, max(P, 1)), np.float32, stream=stream)
         self.d_maxs = F.DeviceArray((dim, max(P, 1)), np.float32, stream=stream)
         self.d_qev = F.DeviceArray(max(P, 1), np.uint32, stream=stream)
@@ -159,12 +228,43 @@ class _Batch:
         self.done_ev = F.Event()
         self.launch()
 
+    @staticmethod
+    def _from_soa(members, offs, dim, stream):
+        import fkdtree_amd as F
+
+        P = int(offs[-1])
+        d_c = F.DeviceArray((dim, max(P, 1)), np.float32, stream=stream)
+        d_i = F.DeviceArray(max(P, 1), np.uint32, stream=stream)
+        hold = []  # host sources stay alive until the copies have run
+        for r, a in zip(members, offs[:-1]):
+            a = int(a)
+            for m in range(dim):
+                hold.append(np.ascontiguousarray(r.coords[m], dtype=np.float32))
+                d_c.upload_at(hold[-1], 4 * (m * P + a), stream)
+            hold.append(np.ascontiguousarray(r.ids, dtype=np.uint32))
+            d_i.upload_at(hold[-1], 4 * a, stream)
+        handle = F.FKDEvents.from_device(dim, offs, d_c.ptr, d_i.ptr, stream)  # synchronises the stream
+        del hold
+        d_c.free()
+        d_i.free()
+        return handle
+
     def launch(self, build: bool = True):
         if build:
             self.handle.build_async()
         self.handle.search_single_device(self.P, self.d_mins.ptr, self.d_maxs.ptr, self.d_off.ptr, self.d_out.ptr,
                                          self.cap, d_qev=self.d_qev.ptr, stream=self.stream)
         self.done_ev.record(self.stream)
+
+    def event_offsets(self) -> np.ndarray:
+        """CSR offsets at the event boundaries of the batch (device gather)."""
+        import fkdtree_amd as F
+        from fkdtree_amd._lib import check, lib
+
+        idx = F.DeviceArray.from_host(self.offsets, self.stream)
+        out = F.DeviceArray(self.offsets.shape[0], np.uint64, stream=self.stream)
+        check(lib().fkd_gather_u64(out.ptr, self.d_off.ptr, idx.ptr, self.offsets.shape[0], self.stream.handle))
+        return out.download(self.stream)
 
     def free_search_buffers(self):
         self.d_mins = self.d_maxs = self.d_qev = self.d_off = self.d_out = None
@@ -174,10 +274,12 @@ class EventServer:
     def __init__(self, n_streams: int = 3, max_batch_points: int = 8_000_000, max_batch_events: int = 512,
                  hbm_fraction: float = 0.8, hbm_budget: Optional[int] = None, half_width: float = 0.01,
                  hits_per_point: float = 256.0, keep_results: bool = False, keep_resident: bool = True,
-                 poll_sleep_s: float = 50e-6):
+                 poll_sleep_s: float = 50e-6, ring_points: int = 0, dim: int = 3):
         import fkdtree_amd as F
 
         self.streams = [F.Stream() for _ in range(n_streams)]
+        # page-locked ingest ring for events that arrive as FKDPoint records
+        self.ring = IngestRing(ring_points, dim) if ring_points else None
         self.max_batch_points = max_batch_points
         self.max_batch_events = max_batch_events
         self.hbm_fraction = hbm_fraction
@@ -231,6 +333,14 @@ class EventServer:
         while pending or queue or inflight:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
+                r = pending[0]
+                if r.records is not None:
+                    if self.ring is None:
+                        raise ValueError("events given as records need an ingest ring (ring_points)")
+                    start = self.ring.put(r.records)
+                    if start is None:
+                        break       # ring full: later arrivals wait for batches to take their events
+                    r.ring_start = start
                 queue.append(pending.popleft())
             progressed = False
             # completions
@@ -261,6 +371,8 @@ class EventServer:
                     if inflight:
                         break           # wait for memory held by batches in flight
                     k = 1               # nothing else holds memory: try the head alone
+                if queue[0].ring_start >= 0:
+                    k = contiguous_prefix(queue, k)
                 members = [queue.popleft() for _ in range(k)]
                 s = free_streams.pop(0)
                 b = _Batch(nb, members, self, s)
@@ -279,10 +391,15 @@ class EventServer:
         return ServeReport(results, nb, time.perf_counter() - t0, peak, self.evictions, sizes)
 
     def _finish(self, b: _Batch, results: Dict[int, EventResult], t_done: float):
-        off = b.d_off.download(b.stream)
-        ids = b.d_out.download(b.stream, count=int(off[-1])) if self.keep_results else None
+        if self.keep_results:
+            off = b.d_off.download(b.stream)
+            ids = b.d_out.download(b.stream, count=int(off[-1]))
+            at = b.offsets.astype(np.int64)
+        else:
+            off, ids = b.event_offsets(), None     # only the event boundaries
+            at = np.arange(b.offsets.shape[0])
         for i, r in enumerate(b.members):
-            a, c = int(b.offsets[i]), int(b.offsets[i + 1])
+            a, c = int(at[i]), int(at[i + 1])
             res = EventResult(r.event, int(off[c] - off[a]), r.arrival, t_done, b.idx)
             if self.keep_results:
                 res.offsets = (off[a:c + 1] - off[a]).astype(np.uint64)
@@ -313,13 +430,21 @@ class EventServer:
             pass
 
 
-def poisson_requests(n_events: int, rate_hz: float, size_scale: float = 1.0, seed_offset: int = 0) -> List[Request]:
+def poisson_requests(n_events: int, rate_hz: float, size_scale: float = 1.0, seed_offset: int = 0,
+                     records: bool = False) -> List[Request]:
     """The serving workload: Poisson arrivals at `rate_hz`, sizes log-uniform
-    in [1,000, 500,000] points (scaled by size_scale), mixed distribution."""
+    in [1,000, 500,000] points (scaled by size_scale), mixed distribution.
+    records=True: every event arrives as FKDPoint records (uint32 [n, 4])."""
     times, sizes = gen.serving_arrivals(n_events, rate_hz)
     out = []
     for e in range(n_events):
         n = max(1, int(sizes[e] * size_scale))
         p = gen.serving_event(n, e + seed_offset)
-        out.append(Request(e, p.coords, p.ids, float(times[e])))
+        if records:
+            rec = np.empty((n, p.coords.shape[0] + 1), dtype=np.uint32)
+            rec[:, :-1] = p.coords.T.view(np.uint32)
+            rec[:, -1] = p.ids
+            out.append(Request(e, None, None, float(times[e]), records=rec))
+        else:
+            out.append(Request(e, p.coords, p.ids, float(times[e])))
     return out

@@ -208,6 +208,11 @@ int fkd_free(void* ptr);
 /* Stream-ordered device allocation (hipMallocAsync / hipFreeAsync on the
  * device's default pool, which keeps freed memory for reuse). */
 int fkd_malloc_async(void** ptr, size_t bytes, fkd_stream_t stream);
+/* Page-locked host memory (hipHostMalloc): copies from it to the device run
+ * asynchronously at full link rate.  For receiving FKDPoint records
+ * (FKDPoint.h:13-104) that go to fkd_create_events without a staging copy. */
+int fkd_host_alloc(void** ptr, size_t bytes);
+int fkd_host_free(void* ptr);
 /* Bytes the default pool holds from the device, and of those in use: the
  * difference is free for stream-ordered allocations though hipMemGetInfo
  * counts it as used. */
@@ -217,6 +222,9 @@ int fkd_memcpy_htod(void* dst, const void* src, size_t bytes, fkd_stream_t strea
 int fkd_memcpy_dtoh(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
 /* Streaming device copy of n uint32 (a plain kernel, for counter calibration). */
 int fkd_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, fkd_stream_t stream);
+/* dst[i] = src[idx[i]] for i < n (device arrays; e.g. the CSR offsets at the
+ * event boundaries of a batch).  Enqueues only. */
+int fkd_gather_u64(uint64_t* dst, const uint64_t* src, const uint64_t* idx, uint64_t n, fkd_stream_t stream);
 int fkd_stream_create(fkd_stream_t* stream);
 int fkd_stream_destroy(fkd_stream_t stream);
 int fkd_stream_sync(fkd_stream_t stream);

@@ -90,3 +90,32 @@ def test_boxes_around_matches_host(require_gpu):
     assert np.array_equal(dq.download(s), np.repeat(np.arange(len(sizes), dtype=np.uint32),
                                                       np.diff(b.offsets).astype(np.int64)))
     t.close()
+
+
+def test_ring_ingestion_equals_soa(require_gpu):
+    """Events arriving as FKDPoint records through the page-locked ingest
+    ring (small enough to wrap several times and to fill) give the same
+    per-event results as events given as SoA arrays."""
+    soa = poisson_requests(40, rate_hz=50_000.0, size_scale=0.02)
+    rec = poisson_requests(40, rate_hz=50_000.0, size_scale=0.02, records=True)
+    ring = 3 * max(r.n for r in rec)
+    print(f"[serve] ring events=40 points={sum(r.n for r in rec)} ring={ring}", flush=True)
+    a = EventServer(n_streams=2, max_batch_points=150_000, max_batch_events=16, keep_results=True)
+    ra = a.run(soa)
+    a.close()
+    b = EventServer(n_streams=2, max_batch_points=150_000, max_batch_events=16, keep_results=True,
+                    ring_points=ring)
+    rb = b.run(rec)
+    assert rb.n_batches >= 4
+    for r in soa:
+        x, y = ra.results[r.event], rb.results[r.event]
+        assert np.array_equal(x.offsets, y.offsets), r.event
+        assert np.array_equal(sorted_sets(x.offsets, x.ids), sorted_sets(y.offsets, y.ids)), r.event
+    assert b.ring.used == 0
+    b.close()
+    # without keeping results only the event boundaries come back; totals agree
+    c = EventServer(n_streams=2, max_batch_points=150_000, max_batch_events=16, ring_points=ring)
+    rc = c.run(rec)
+    for r in soa:
+        assert rc.results[r.event].hits == int(ra.results[r.event].offsets[-1])
+    c.close()

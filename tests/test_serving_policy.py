@@ -1,5 +1,6 @@
 """Host-side admission policy of the event server (no GPU)."""
 import numpy as np
+import pytest
 
 from fkdtree_amd.serving import Request, batch_bytes, plan_batch
 
@@ -25,3 +26,43 @@ def test_memory_budget_limits_batch():
 def test_oversized_head_still_admitted_alone_by_points():
     q = reqs([50_000, 10])
     assert plan_batch(q, 1 << 40, max_points=1000, max_events=10, hits_per_point=1) == 1
+
+
+def _rec(n, tag):
+    r = np.zeros((n, 4), dtype=np.uint32)
+    r[:, 3] = tag
+    return r
+
+
+def test_ingest_ring_wraps_fills_and_releases_in_order():
+    from fkdtree_amd.serving import IngestRing
+
+    ring = IngestRing(10, 3, buffer=np.zeros((10, 4), dtype=np.uint32))
+    assert ring.put(_rec(4, 1)) == 0
+    assert ring.put(_rec(4, 2)) == 4
+    assert ring.put(_rec(3, 3)) is None          # 2 slots left at the end, 0 at the start
+    ring.release(1)                               # frees slots 0..3
+    assert ring.put(_rec(3, 3)) == 0              # skips the 2-slot tail end, wraps
+    assert ring.used == 4 + 2 + 3
+    assert np.all(ring.view(0, 3)[:, 3] == 3) and np.all(ring.view(4, 4)[:, 3] == 2)
+    assert ring.put(_rec(2, 4)) is None
+    ring.release(2)
+    assert ring.used == 0 and ring.tail == 0
+    assert ring.put(_rec(10, 5)) == 0
+    with pytest.raises(ValueError):
+        ring.put(_rec(11, 6))
+
+
+def test_contiguous_prefix_stops_at_the_wrap():
+    from fkdtree_amd.serving import Request, contiguous_prefix
+
+    def req(start, n):
+        r = Request(0, None, None, 0.0, records=_rec(n, 0))
+        r.ring_start = start
+        return r
+
+    q = [req(0, 3), req(3, 2), req(5, 4), req(0, 1), req(1, 1)]
+    assert contiguous_prefix(q, 5) == 3
+    assert contiguous_prefix(q, 2) == 2
+    assert contiguous_prefix(q[3:], 2) == 2
+    assert contiguous_prefix(q, 1) == 1
