@@ -135,9 +135,11 @@ uint32_t grid_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<
 struct Tiling {
   uint32_t T = 0;
   DevBuf tile_ev, tb;
+  std::vector<uint32_t> htb, hev;  // host sources of the uploads (kept until the handle goes)
   int init(const std::vector<uint64_t>& off, uint32_t tile, hipStream_t s) {
     const size_t E = off.size() - 1;
-    std::vector<uint32_t> htb(E + 1), hev;
+    htb.assign(E + 1, 0);
+    hev.clear();
     uint32_t t = 0;
     for (size_t e = 0; e < E; ++e) {
       htb[e] = t;
@@ -153,13 +155,13 @@ struct Tiling {
     FKD_HIP(hipMemcpyAsync(tb.p, htb.data(), htb.size() * 4, hipMemcpyHostToDevice, s));
     if (!hev.empty())
       FKD_HIP(hipMemcpyAsync(tile_ev.p, hev.data(), hev.size() * 4, hipMemcpyHostToDevice, s));
-    FKD_HIP(hipStreamSynchronize(s));
     return FKD_OK;
   }
   // Tiles given per event.
   int init_counts(const std::vector<uint32_t>& per_event, hipStream_t s) {
     const size_t E = per_event.size();
-    std::vector<uint32_t> htb(E + 1), hev;
+    htb.assign(E + 1, 0);
+    hev.clear();
     uint32_t t = 0;
     for (size_t e = 0; e < E; ++e) {
       htb[e] = t;
@@ -173,7 +175,6 @@ struct Tiling {
     FKD_HIP(hipMemcpyAsync(tb.p, htb.data(), htb.size() * 4, hipMemcpyHostToDevice, s));
     if (!hev.empty())
       FKD_HIP(hipMemcpyAsync(tile_ev.p, hev.data(), hev.size() * 4, hipMemcpyHostToDevice, s));
-    FKD_HIP(hipStreamSynchronize(s));
     return FKD_OK;
   }
   fkd::TileMap map(const uint32_t* off) const {

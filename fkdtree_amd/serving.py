@@ -17,12 +17,12 @@ The server drives the GPU from one host thread:
   Several batches overlap on the device.
 * When a batch completes, its results are checked for completeness (the
   output is regrown and the search re-run if needed) and its latency is
-  recorded.  Its build / search scratch is trimmed, and its trees stay
-  resident for later queries (`query`), least recently used first out when
-  HBM runs short.
+  recorded.  Its trees stay resident for later queries (`query`).  When HBM
+  runs short, resident trees first give up their build / search scratch
+  (fkd_trim, oldest first), then the least recently used are evicted.
 * Admission is by free HBM: a batch is admitted only when its estimated
   footprint fits the budget, i.e. `hbm_fraction` of what hipMemGetInfo
-  reports free.  Resident trees are evicted first, then the server waits for
+  reports free.  Resident trees are trimmed and then evicted first, then the server waits for
   batches in flight to finish.
 
 * Ingestion: events that arrive as FKDPoint records (uint32 [n, dim+1],
@@ -308,6 +308,20 @@ class EventServer:
             b = min(b, self.hbm_budget - self.resident_bytes - self.inflight_bytes)
         return b
 
+    def _reclaim_one(self) -> bool:
+        """Make room for a batch: release the build / search scratch of the
+        oldest resident tree that still holds it (fkd_trim), else evict the
+        least recently used tree."""
+        for b in self.resident.values():
+            if not b.trimmed:
+                before = b.resident_bytes
+                b.handle.trim()
+                b.trimmed = True
+                b.resident_bytes = b.handle.device_bytes()
+                self.resident_bytes += b.resident_bytes - before
+                return True
+        return self._evict_one()
+
     def _evict_one(self) -> bool:
         if not self.resident:
             return False
@@ -366,7 +380,7 @@ class EventServer:
                 budget = self._budget()
                 k = plan_batch(queue, budget, self.max_batch_points, self.max_batch_events, self.hits_per_point)
                 if k == 0:
-                    if self._evict_one():
+                    if self._reclaim_one():
                         continue
                     if inflight:
                         break           # wait for memory held by batches in flight
@@ -407,7 +421,8 @@ class EventServer:
             results[r.event] = res
         b.free_search_buffers()
         if self.keep_resident:
-            b.handle.trim()
+            # the scratch stays until memory is needed (_reclaim_one)
+            b.trimmed = False
             b.resident_bytes = b.handle.device_bytes()
             self.resident_bytes += b.resident_bytes
             self.resident[b.idx] = b
