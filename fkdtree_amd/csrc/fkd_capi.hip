@@ -213,6 +213,7 @@ struct fkd_tree_s {
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev, q_aos;  // search scratch
   DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets;
   DevBuf q_bwork, q_border;    // fill pass: work per search block and dispatch order
+  DevBuf q_cslot;              // count pass: counts in processing order
   DevBuf zarena;               // look-back descriptors and block counters, zeroed per build / search
   uint64_t last_capacity = 0;
   uint64_t* last_offsets = nullptr;
@@ -527,6 +528,8 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
     a.mins = t->q_sbox.as<float>();
     a.maxs = a.mins + (uint64_t)DIM * n;
     a.counts = t->q_counts.as<uint32_t>();
+    FKD_TRY(t->q_cslot.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
+    a.counts_slot = t->q_cslot.as<uint32_t>();
     FKD_TRY((launch_search<DIM, fkd::kCount>(t, a, s, "search_count")));
   }
   const uint64_t st = fkd::scan_tiles(nq);
@@ -572,7 +575,7 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
   if (t->tune.fill_order && nb >= 64) {
     FKD_TRY(t->q_bwork.ensure((size_t)nb * 4));
     FKD_TRY(t->q_border.ensure((size_t)nb * 4));
-    fkd::block_work_kernel<<<nb, fkd::kThreads, 0, s>>>(qoff, a.perm, nq, t->q_bwork.as<uint32_t>());
+    fkd::block_work_kernel<<<nb, fkd::kThreads, 0, s>>>(t->q_cslot.as<uint32_t>(), nq, t->q_bwork.as<uint32_t>());
     FKD_TRY(launch_check("block_work"));
     fkd::block_order_kernel<<<1, 1024, 0, s>>>(t->q_bwork.as<uint32_t>(), nb, t->q_border.as<uint32_t>());
     FKD_TRY(launch_check("block_order"));
@@ -1128,7 +1131,7 @@ int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
   FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->Xr, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
                     &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
-                    &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_aos, &t->zarena})
+                    &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_cslot, &t->q_aos, &t->zarena})
     b->release();
   t->q_plan_nq = 0;
   t->last_offsets = nullptr;
@@ -1143,7 +1146,7 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
                           &t->level_tiles.tb, &t->fin_tiles.tile_ev, &t->fin_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R, &t->Xr,
                           &t->planes, &t->F, &t->counts, &t->offsets, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
-                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_bwork, &t->q_border,
+                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_cslot,
                           &t->q_aos, &t->zarena})
     b += d->bytes;
   *bytes = b;

@@ -32,6 +32,7 @@ struct SearchArgs {
   const float* mins;      // [DIM][nq], indexed by processing slot
   const float* maxs;
   uint32_t* counts;       // count pass output (query order)
+  uint32_t* counts_slot;  // count pass: the same counts in processing order, or null
   uint32_t* visited;      // stats pass: nodes visited per query
   const uint64_t* qoff;   // fill pass: first output slot of every query
   uint32_t* out;          // fill pass output
@@ -223,22 +224,17 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
       if (wb + j < a.cap) a.out[wb + j] = rw[j & (kRing - 1)];
   }
   if (!FILL) a.counts[q] = c;
+  if (MODE == kCount && a.counts_slot) a.counts_slot[slot] = c;
   if (MODE == kStats) a.visited[q] = vis;
 }
 
 // Work estimate of every search block for the fill pass: the largest hit
-// count among its queries (processing order).
-__global__ __launch_bounds__(kThreads) void block_work_kernel(const uint64_t* __restrict__ qoff,
-                                                              const uint32_t* __restrict__ perm, uint32_t nq,
+// count among its queries (the count pass's counts in processing order).
+__global__ __launch_bounds__(kThreads) void block_work_kernel(const uint32_t* __restrict__ counts_slot, uint32_t nq,
                                                               uint32_t* __restrict__ work) {
   __shared__ uint32_t s_max[kThreads / kWave];
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
-  uint32_t h = 0;
-  if (slot < nq) {
-    const uint32_t q = perm ? perm[slot] : slot;
-    const uint64_t d = qoff[q + 1] - qoff[q];
-    h = d > 0xffffffffull ? 0xffffffffu : (uint32_t)d;
-  }
+  uint32_t h = slot < nq ? counts_slot[slot] : 0u;
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) h = max(h, (uint32_t)__shfl_xor((int)h, o));
   if (lane_id() == 0) s_max[threadIdx.x / kWave] = h;
