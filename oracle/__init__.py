@@ -60,6 +60,8 @@ def lib():
         L.orc_search_events_count.argtypes = [_I, _U64, _P, _P, _P, _U64, _P, _P, _P, _P, _P, _I]
         L.orc_search_events_fill.argtypes = [_I, _U64, _P, _P, _P, _U64, _P, _P, _P, _P, _P, _I]
         L.orc_brute.argtypes = [_I, _U64, _P, _P, _U64, _P, _P, _P, _P, _P, _I]
+        L.orc_sort_segments.argtypes = [_U64, _P, _P, _P, _I]
+        L.orc_sort_segments.restype = None
         _lib = L
     return _lib
 
@@ -199,6 +201,20 @@ def search_events(tree, tids, offsets, qev, mins, maxs, threads: int = 0, with_v
     return (qoff, out, vis) if with_visited else (qoff, out)
 
 
+def search_events_counts(tree, tids, offsets, qev, mins, maxs, threads: int = 0):
+    """counts[nq] of the per-event search (no ids): query q searches event qev[q]."""
+    tree = _soa(tree)
+    tids = np.ascontiguousarray(tids, dtype=np.uint32)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    qev = np.ascontiguousarray(qev, dtype=np.uint32)
+    mins, maxs = _soa(mins), _soa(maxs)
+    nq = mins.shape[1]
+    counts = np.empty(nq, dtype=np.uint32)
+    lib().orc_search_events_count(tree.shape[0], off.shape[0] - 1, _ptr(off), _ptr(tree), _ptr(tids), nq,
+                                  _ptr(qev), _ptr(mins), _ptr(maxs), _ptr(counts), None, threads)
+    return counts
+
+
 def brute(coords, ids, mins, maxs, threads: int = 0):
     """Brute-force closed-box membership: (offsets[nq+1], ids in point order)."""
     coords = _soa(coords)
@@ -219,11 +235,14 @@ def brute(coords, ids, mins, maxs, threads: int = 0):
 
 def sort_segments(off: np.ndarray, ids: np.ndarray) -> np.ndarray:
     """Sort every CSR segment in place order-independently (returns a copy)."""
-    off = np.asarray(off, dtype=np.int64)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
     nq = off.shape[0] - 1
-    seg = np.repeat(np.arange(nq, dtype=np.int64), np.diff(off))
-    order = np.lexsort((ids, seg))
-    return np.ascontiguousarray(ids[order])
+    if nq < 0 or int(off[-1]) != ids.shape[0] or np.any(off[1:] < off[:-1]):
+        raise ValueError("offsets do not describe ids")
+    out = np.empty_like(ids)
+    lib().orc_sort_segments(nq, _ptr(off), _ptr(ids), _ptr(out), 0)
+    return out
 
 
 class RefTree:

@@ -301,6 +301,19 @@ int orc_brute(int dim, uint64_t n, const float* coords, const uint32_t* ids, uin
   return 0;
 }
 
+// Sorts every CSR segment [offsets[q], offsets[q+1]) of `ids` into `out`
+// (ascending), so result sets can be compared independently of hit order.
+void orc_sort_segments(uint64_t nq, const uint64_t* offsets, const uint32_t* ids, uint32_t* out,
+                       int threads) {
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t q = 0; q < (int64_t)nq; ++q) {
+    const uint64_t a = offsets[q], b = offsets[q + 1];
+    std::copy(ids + a, ids + b, out + a);
+    std::sort(out + a, out + b);
+  }
+}
+
 int orc_max_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -244,6 +244,27 @@ def test_oracle_search_vs_brute_force(kind):
         assert np.array_equal(oracle.sort_segments(off, ids), oracle.sort_segments(boff, bids))
 
 
+def test_sort_segments_vs_numpy():
+    """The threaded segment sort equals numpy's per-segment sort on ragged
+    segments (empty ones, one huge one, duplicates) and rejects bad offsets."""
+    rng = np.random.default_rng(7)
+    counts = rng.integers(0, 40, 5000)
+    counts[::17] = 0
+    counts[100] = 70_000
+    off = np.zeros(counts.size + 1, np.uint64)
+    np.cumsum(counts, out=off[1:])
+    ids = rng.integers(0, 1000, int(off[-1])).astype(np.uint32)
+    want = np.concatenate([np.sort(ids[int(a):int(b)]) for a, b in zip(off[:-1], off[1:])])
+    got = oracle.sort_segments(off, ids)
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got, ids)
+    assert oracle.sort_segments(np.zeros(1, np.uint64), np.zeros(0, np.uint32)).size == 0
+    with pytest.raises(ValueError):
+        oracle.sort_segments(off, ids[:-1])
+    with pytest.raises(ValueError):
+        oracle.sort_segments(np.array([0, 5, 3, 8], np.uint64), ids[:8])
+
+
 def test_oracle_visited_counts_vs_level_sweep():
     p = adversarial_points("mixed", 3000)
     c, i = oracle.build(p.coords, p.ids)
