@@ -17,16 +17,16 @@
 
 namespace fkd {
 
-// Record stride (floats) of a packed node: coordinates then the id bits.
-// Search records of a node, two arrays of kFloats floats per node: the DIM
-// coordinates, then at index kSplit either the coordinate on the node's
-// split dimension (depth % DIM) once more (`nodes`, for counting: the
-// traversal reads the split value without selecting by dimension) or the id
-// (`nodes_id`, for emitting hits without a second load).
+// Search record of a node (kFloats floats): its coordinates rotated so that
+// the split dimension comes first -- slot j holds coordinate (dd + j) % DIM,
+// dd = depth % DIM -- then at index kId the id bits.  The traversal reads the
+// split value from slot 0 and compares slot j with the box rotated the same
+// way, so it never selects by dimension; hits are emitted without a second
+// load.
 template <int DIM>
 struct NodeRec {
   static constexpr int kFloats = DIM <= 3 ? 4 : 8;
-  static constexpr int kSplit = DIM <= 3 ? 3 : 4;
+  static constexpr int kId = DIM <= 3 ? 3 : 4;
 };
 
 // AoS FKDPoint<float, DIM> records (DIM floats then a uint32 id) -> SoA.
@@ -133,8 +133,7 @@ struct LevelArgs {
   uint32_t P;                 // plane stride
   float* Tc;                  // tree outputs
   uint32_t* Tid;
-  float* nodes;               // search records (split value / id)
-  float* nodes_id;
+  float* recs;                // search records (rotated coordinates, id)
   uint64_t* desc;             // look-back descriptors, zero at launch
   uint32_t* ctr;              // dynamic block counter, zero at launch
 };
@@ -144,7 +143,6 @@ __device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t 
                                   const float4* prec) {
   constexpr int RF = NodeRec<DIM>::kFloats;
   float c[DIM];
-  float split = 0.f;
   uint32_t id;
   const uint32_t dd = depth % (uint32_t)DIM;
   if (DIM <= 3) {
@@ -158,36 +156,21 @@ __device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t 
     for (int m = 0; m < DIM; ++m) c[m] = a.X[(uint64_t)m * a.P + p];
     id = a.ids[p];
   }
+  float r[4] = {0.f, 0.f, 0.f, 0.f};  // rotated coordinates
 #pragma unroll
   for (int m = 0; m < DIM; ++m) {
     a.Tc[(uint64_t)m * a.P + o + node] = c[m];
-    split = dd == (uint32_t)m ? c[m] : split;
+#pragma unroll
+    for (int j = 0; j < DIM; ++j)  // slot j holds coordinate (dd + j) % DIM
+      if ((dd + (uint32_t)j) % (uint32_t)DIM == (uint32_t)m) r[j < 4 ? j : 0] = c[m];
   }
   a.Tid[o + node] = id;
-  float* rec = a.nodes + (uint64_t)(o + node) * RF;
-  float* rid = a.nodes_id + (uint64_t)(o + node) * RF;
+  float* rec = a.recs + (uint64_t)(o + node) * RF;
   if (RF == 4) {
-    float4 v;
-    v.x = c[0];
-    v.y = DIM > 1 ? c[DIM > 1 ? 1 : 0] : 0.f;
-    v.z = DIM > 2 ? c[DIM > 2 ? 2 : 0] : 0.f;
-    v.w = split;
-    *reinterpret_cast<float4*>(rec) = v;
-    v.w = __uint_as_float(id);
-    *reinterpret_cast<float4*>(rid) = v;
+    *reinterpret_cast<float4*>(rec) = make_float4(r[0], r[1], r[2], __uint_as_float(id));
   } else {
-    float4 v0, v1;
-    v0.x = c[0];
-    v0.y = c[DIM > 1 ? 1 : 0];
-    v0.z = c[DIM > 2 ? 2 : 0];
-    v0.w = c[DIM > 3 ? 3 : 0];
-    v1.x = split;
-    v1.y = v1.z = v1.w = 0.f;
-    reinterpret_cast<float4*>(rec)[0] = v0;
-    reinterpret_cast<float4*>(rec)[1] = v1;
-    v1.x = __uint_as_float(id);
-    reinterpret_cast<float4*>(rid)[0] = v0;
-    reinterpret_cast<float4*>(rid)[1] = v1;
+    reinterpret_cast<float4*>(rec)[0] = make_float4(r[0], r[1], r[2], r[3]);
+    reinterpret_cast<float4*>(rec)[1] = make_float4(__uint_as_float(id), 0.f, 0.f, 0.f);
   }
 }
 

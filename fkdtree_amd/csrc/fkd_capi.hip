@@ -192,6 +192,7 @@ struct Tiling {
 struct fkd_tuning {
   uint32_t query_key_bits = 8;  // search: Morton bits per dimension of the box order
   uint32_t fill_order = 1;      // fill pass: heaviest search blocks first
+  uint32_t fill_window = 1024;  // fill pass: boxes reordered by work inside windows of this many (0: off)
 };
 
 struct fkd_tree_s {
@@ -207,13 +208,14 @@ struct fkd_tree_s {
   DevBuf off;                  // uint32 event offsets (n_events + 1)
   Tiling radix_tiles, level_tiles, fin_tiles;
   DevBuf X, ids;               // input copy
-  DevBuf Tc, Tid, nodes, nodes_id;  // outputs and search records
+  DevBuf Tc, Tid, recs;        // outputs and search records
   DevBuf A, K, R, planes, F, Xr;   // build scratch
   DevBuf counts, offsets, err, bbox;
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev, q_aos;  // search scratch
   DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets;
   DevBuf q_bwork, q_border;    // fill pass: work per search block and dispatch order
-  DevBuf q_cslot;              // count pass: counts in processing order
+  DevBuf q_wslot;              // count pass: visits in processing order
+  DevBuf q_sorder;             // fill pass: dispatch slot -> processing slot
   DevBuf zarena;               // look-back descriptors and block counters, zeroed per build / search
   uint64_t last_capacity = 0;
   uint64_t* last_offsets = nullptr;
@@ -388,8 +390,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     a.P = P;
     a.Tc = t->Tc.as<float>();
     a.Tid = t->Tid.as<uint32_t>();
-    a.nodes = t->nodes.as<float>();
-    a.nodes_id = t->nodes_id.as<float>();
+    a.recs = t->recs.as<float>();
     a.desc = za.take((uint64_t)M * tm.T + 1);
     a.ctr = reinterpret_cast<uint32_t*>(a.desc + (uint64_t)M * tm.T);
     const uint32_t blocks = (M ? M : 1u) * tm.T;
@@ -425,8 +426,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     a.P = P;
     a.Tc = t->Tc.as<float>();
     a.Tid = t->Tid.as<uint32_t>();
-    a.nodes = t->nodes.as<float>();
-    a.nodes_id = t->nodes_id.as<float>();
+    a.recs = t->recs.as<float>();
     fkd::output_kernel<DIM><<<tm.T, fkd::kThreads, 0, s>>>(a, t->Xr.as<float4>());
     FKD_TRY(launch_check("output"));
   }
@@ -452,8 +452,7 @@ int launch_search(fkd_tree_s* t, const fkd::SearchArgs& a, hipStream_t s, const 
 template <int DIM>
 fkd::SearchArgs search_args(fkd_tree_s* t, uint32_t nq, const uint32_t* qev) {
   fkd::SearchArgs a{};
-  a.nodes = t->nodes.as<float>();
-  a.nodes_id = t->nodes_id.as<float>();
+  a.recs = t->recs.as<float>();
   a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
   a.n_single = t->P;
   a.qev = qev;
@@ -528,8 +527,8 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
     a.mins = t->q_sbox.as<float>();
     a.maxs = a.mins + (uint64_t)DIM * n;
     a.counts = t->q_counts.as<uint32_t>();
-    FKD_TRY(t->q_cslot.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
-    a.counts_slot = t->q_cslot.as<uint32_t>();
+    FKD_TRY(t->q_wslot.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
+    a.work_slot = t->q_wslot.as<uint32_t>();
     FKD_TRY((launch_search<DIM, fkd::kCount>(t, a, s, "search_count")));
   }
   const uint64_t st = fkd::scan_tiles(nq);
@@ -572,11 +571,28 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
   a.cap = cap;
   a.out_aligned = (reinterpret_cast<uintptr_t>(out) & 63u) == 0;
   const uint32_t nb = (uint32_t)((nq + fkd::kThreads - 1) / fkd::kThreads);
-  if (t->tune.fill_order && nb >= 64) {
-    FKD_TRY(t->q_bwork.ensure((size_t)nb * 4));
+  const uint32_t win = t->tune.fill_window;
+  const bool sorted = win != 0 && nq > win;
+  const bool ordered = t->tune.fill_order && nb >= 64;
+  if (sorted || ordered) FKD_TRY(t->q_bwork.ensure((size_t)nb * 4));
+  if (sorted) {
+    FKD_TRY(t->q_sorder.ensure((size_t)nq * 4));
+    const uint32_t nw = (uint32_t)((nq + win - 1) / win);
+    uint32_t* so = t->q_sorder.as<uint32_t>();
+    uint32_t* bw = t->q_bwork.as<uint32_t>();
+    const uint32_t* ws = t->q_wslot.as<uint32_t>();
+    if (win == 256) fkd::window_sort_kernel<1><<<nw, fkd::kThreads, 0, s>>>(ws, nq, so, bw);
+    else if (win == 1024) fkd::window_sort_kernel<4><<<nw, fkd::kThreads, 0, s>>>(ws, nq, so, bw);
+    else fkd::window_sort_kernel<16><<<nw, fkd::kThreads, 0, s>>>(ws, nq, so, bw);
+    FKD_TRY(launch_check("window_sort"));
+    a.slot_order = so;
+  }
+  if (ordered) {
     FKD_TRY(t->q_border.ensure((size_t)nb * 4));
-    fkd::block_work_kernel<<<nb, fkd::kThreads, 0, s>>>(t->q_cslot.as<uint32_t>(), nq, t->q_bwork.as<uint32_t>());
-    FKD_TRY(launch_check("block_work"));
+    if (!sorted) {
+      fkd::block_work_kernel<<<nb, fkd::kThreads, 0, s>>>(t->q_wslot.as<uint32_t>(), nq, t->q_bwork.as<uint32_t>());
+      FKD_TRY(launch_check("block_work"));
+    }
     fkd::block_order_kernel<<<1, 1024, 0, s>>>(t->q_bwork.as<uint32_t>(), nb, t->q_border.as<uint32_t>());
     FKD_TRY(launch_check("block_order"));
     a.block_order = t->q_border.as<uint32_t>();
@@ -686,8 +702,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   const int rf = dim <= 3 ? 4 : 8;
   FKD_TRY(t->Tc.ensure(S * dim * 4));
   FKD_TRY(t->Tid.ensure(S * 4));
-  FKD_TRY(t->nodes.ensure(S * rf * 4));
-  FKD_TRY(t->nodes_id.ensure(S * rf * 4));
+  FKD_TRY(t->recs.ensure(S * rf * 4));
   FKD_TRY(ensure_build_scratch(t));
   FKD_TRY(t->err.ensure(64));
   FKD_TRY(t->bbox.ensure(64));
@@ -772,8 +787,7 @@ int adopt_tree_impl(fkd_tree_s* t, hipStream_t s) {
     a.P = P;
     a.Tc = t->Tc.as<float>();
     a.Tid = t->Tid.as<uint32_t>();
-    a.nodes = t->nodes.as<float>();
-    a.nodes_id = t->nodes_id.as<float>();
+    a.recs = t->recs.as<float>();
     fkd::output_kernel<DIM><<<a.tm.T, fkd::kThreads, 0, s>>>(a, t->Xr.as<float4>());
     FKD_TRY(launch_check("output"));
   }
@@ -1131,7 +1145,7 @@ int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
   FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->Xr, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
                     &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
-                    &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_cslot, &t->q_aos, &t->zarena})
+                    &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_wslot, &t->q_sorder, &t->q_aos, &t->zarena})
     b->release();
   t->q_plan_nq = 0;
   t->last_offsets = nullptr;
@@ -1143,10 +1157,10 @@ int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
   if (!t || !bytes) return fail(FKD_ERR_INVALID, "tree / bytes is NULL");
   uint64_t b = 0;
   for (const DevBuf* d : {&t->off, &t->radix_tiles.tile_ev, &t->radix_tiles.tb, &t->level_tiles.tile_ev,
-                          &t->level_tiles.tb, &t->fin_tiles.tile_ev, &t->fin_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->nodes, &t->nodes_id, &t->A, &t->K, &t->R, &t->Xr,
+                          &t->level_tiles.tb, &t->fin_tiles.tile_ev, &t->fin_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->recs, &t->A, &t->K, &t->R, &t->Xr,
                           &t->planes, &t->F, &t->counts, &t->offsets, &t->err, &t->bbox,
                           &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
-                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_cslot,
+                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_wslot, &t->q_sorder,
                           &t->q_aos, &t->zarena})
     b += d->bytes;
   *bytes = b;
@@ -1162,6 +1176,10 @@ int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
   } else if (k == "fill_order") {
     if (value > 1) return fail(FKD_ERR_INVALID, "fill_order must be 0 or 1");
     t->tune.fill_order = value;
+  } else if (k == "fill_window") {
+    if (value != 0 && value != 256 && value != 1024 && value != 4096)
+      return fail(FKD_ERR_INVALID, "fill_window must be 0, 256, 1024 or 4096");
+    t->tune.fill_window = value;
   } else {
     return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   }
@@ -1173,6 +1191,7 @@ int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
   const std::string k(key);
   if (k == "query_key_bits") *value = t->tune.query_key_bits;
   else if (k == "fill_order") *value = t->tune.fill_order;
+  else if (k == "fill_window") *value = t->tune.fill_window;
   else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;
 }

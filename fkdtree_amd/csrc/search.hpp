@@ -21,18 +21,18 @@
 namespace fkd {
 
 struct SearchArgs {
-  const float* nodes;     // records with the split value, NodeRec<DIM>::kFloats floats each
-  const float* nodes_id;  // records with the id
+  const float* recs;      // node records, NodeRec<DIM>::kFloats floats each
   const uint32_t* off;    // event -> first node (n_events + 1), or null
   uint32_t n_single;      // node count when off == null
   const uint32_t* qev;    // query -> event, or null (event 0)
   const uint32_t* perm;   // processing slot -> query (null: identity)
   const uint32_t* block_order;  // dispatch position -> search block (null: identity)
+  const uint32_t* slot_order;   // fill pass: dispatch slot -> processing slot (null: identity)
   uint32_t nq;
   const float* mins;      // [DIM][nq], indexed by processing slot
   const float* maxs;
   uint32_t* counts;       // count pass output (query order)
-  uint32_t* counts_slot;  // count pass: the same counts in processing order, or null
+  uint32_t* work_slot;    // count pass: nodes visited per query in processing order, or null
   uint32_t* visited;      // stats pass: nodes visited per query
   const uint64_t* qoff;   // fill pass: first output slot of every query
   uint32_t* out;          // fill pass output
@@ -73,9 +73,9 @@ struct NodeSource {
   __amdgpu_buffer_rsrc_t rsrc;
 };
 
-// Coordinates and the word at index kSplit (split value or id) of record `rec`.
+// Rotated coordinates and the id bits of record `rec`.
 template <int DIM, bool BUF>
-__device__ inline void load_rec(const NodeSource& src, uint32_t rec, float (&c)[DIM], float& extra) {
+__device__ inline void load_rec(const NodeSource& src, uint32_t rec, float (&c)[DIM], uint32_t& id) {
   constexpr int RF = NodeRec<DIM>::kFloats;
   float4 q;
   if (BUF)
@@ -86,13 +86,13 @@ __device__ inline void load_rec(const NodeSource& src, uint32_t rec, float (&c)[
   if (DIM > 1) c[DIM > 1 ? 1 : 0] = q.y;
   if (DIM > 2) c[DIM > 2 ? 2 : 0] = q.z;
   if (RF == 4) {
-    extra = q.w;
+    id = __float_as_uint(q.w);
   } else {
     c[DIM > 3 ? 3 : 0] = q.w;
     if (BUF)
-      extra = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(src.rsrc, rec * (uint32_t)(RF * 4) + 16u, 0, 0));
+      id = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, rec * (uint32_t)(RF * 4) + 16u, 0, 0);
     else
-      extra = src.flat[(uint64_t)rec * RF + NodeRec<DIM>::kSplit];
+      id = __float_as_uint(src.flat[(uint64_t)rec * RF + NodeRec<DIM>::kId]);
   }
 }
 
@@ -109,18 +109,26 @@ constexpr int kRingRow = 16;  // words per lane (16-byte aligned rows)
 
 // The loop body is branch free except for writing out staged ids: the next
 // node is selected among (left son, right son, pending right son of the
-// deepest pending depth) with conditional moves.  Counting reads the split
-// value from the record and the box bounds on the split dimension from LDS
-// (DIM >= 3), so no per-lane select on the dimension is needed; the fill
-// pass reads the id from the record and selects the split value.
+// deepest pending depth) with conditional moves.  Every lane keeps its box
+// twice in LDS (the (min, max) pairs of all dimensions, then again), so the
+// box rotated like the node records of depth d (split dimension first) is
+// the DIM pairs starting at pair d % DIM, found through a 32-entry table:
+// no selection by dimension in the loop.
 template <int DIM, int MODE, bool BUF>
 __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   constexpr bool FILL = MODE == kFill;
-  constexpr bool LDS_BOX = DIM >= 3;
+  constexpr bool LDS_BOX = DIM >= 2;
+  constexpr int BS = 2 * DIM + 1;  // pairs per lane (odd: the lanes' reads avoid bank conflicts)
   __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
-  __shared__ float2 s_box[LDS_BOX ? DIM : 1][LDS_BOX ? kThreads : 1];
-  const uint32_t slot = search_block(blockIdx.x, gridDim.x, a.block_order) * kThreads + threadIdx.x;
+  __shared__ float2 s_box[LDS_BOX ? kThreads * BS : 1];
+  __shared__ uint32_t s_rot[LDS_BOX ? 32 : 1];  // depth -> byte offset of its rotation
+  if (LDS_BOX) {
+    if (threadIdx.x < 32u) s_rot[threadIdx.x] = (threadIdx.x % (uint32_t)DIM) * (uint32_t)sizeof(float2);
+    __syncthreads();
+  }
+  uint32_t slot = search_block(blockIdx.x, gridDim.x, a.block_order) * kThreads + threadIdx.x;
   if (slot >= a.nq) return;
+  if (FILL && a.slot_order) slot = a.slot_order[slot];
   const uint32_t q = a.perm ? a.perm[slot] : slot;
   uint32_t base = 0, n = a.n_single;
   if (a.off) {
@@ -129,16 +137,16 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     n = a.off[e + 1] - base;
   }
   float mn[DIM], mx[DIM];
+  float2* const my_box = &s_box[LDS_BOX ? threadIdx.x * BS : 0];
 #pragma unroll
   for (int m = 0; m < DIM; ++m) {
     mn[m] = a.mins[(uint64_t)m * a.nq + slot];
     mx[m] = a.maxs[(uint64_t)m * a.nq + slot];
-    if (LDS_BOX) s_box[LDS_BOX ? m : 0][LDS_BOX ? threadIdx.x : 0] = make_float2(mn[m], mx[m]);
+    if (LDS_BOX) my_box[m] = my_box[m + DIM] = make_float2(mn[m], mx[m]);
   }
-  const float* recs = FILL ? a.nodes_id : a.nodes;
   NodeSource src;
-  src.flat = recs;
-  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)a.node_bytes, 0x00020000);
+  src.flat = a.recs;
+  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.recs, (short)0, (int)a.node_bytes, 0x00020000);
   uint32_t c = 0;
   uint32_t* rw = &ring[FILL ? threadIdx.x * kRingRow : 0];
   const uint64_t w0 = FILL ? a.qoff[q] : 0;
@@ -166,35 +174,33 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   };
   // node has a left (right) son iff node < tl (tr), i.e. 2 node + 1 (2) < n
   const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
-  uint32_t node = 0, depth = 0, pending = 0, vis = 0;
+  // pending: bit d set = the right son at depth d is still to be visited;
+  // bit 0 is a sentinel, so the deepest pending depth is 31 - clz(pending).
+  uint32_t node = 0, depth = 0, pending = 1, vis = 0;
   bool alive = n != 0;
   while (alive) {
-    if (MODE == kStats) ++vis;
-    float x[DIM], extra;
-    load_rec<DIM, BUF>(src, base + node, x, extra);
-    const uint32_t dd = depth_dim<DIM>(depth);
-    float v = FILL ? x[0] : extra, lo = mn[0], hi = mx[0];
+    if (!FILL) ++vis;
+    float x[DIM], lo[DIM], hi[DIM];
+    uint32_t id;
+    load_rec<DIM, BUF>(src, base + node, x, id);
     if (LDS_BOX) {
-      const float2 bx = s_box[LDS_BOX ? dd : 0][LDS_BOX ? threadIdx.x : 0];
-      lo = bx.x;
-      hi = bx.y;
+      const float2* bx = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[depth]);
 #pragma unroll
-      for (int m = 1; m < DIM; ++m)
-        if (FILL) v = dd == (uint32_t)m ? x[m] : v;
-    } else {
-#pragma unroll
-      for (int m = 1; m < DIM; ++m) {
-        if (FILL) v = dd == (uint32_t)m ? x[m] : v;
-        lo = dd == (uint32_t)m ? mn[m] : lo;
-        hi = dd == (uint32_t)m ? mx[m] : hi;
+      for (int j = 0; j < DIM; ++j) {
+        const float2 t = bx[j];
+        lo[j] = t.x;
+        hi[j] = t.y;
       }
+    } else {
+      lo[0] = mn[0];
+      hi[0] = mx[0];
     }
-    const uint32_t id = __float_as_uint(extra);
-    bool in = true;
+    // x[0] is the split value and lo[0], hi[0] the bounds on the split dimension
+    const bool inter = (x[0] <= hi[0]) & (x[0] >= lo[0]);
+    bool in = inter;
 #pragma unroll
-    for (int m = 0; m < DIM; ++m) in = in & (x[m] <= mx[m]) & (x[m] >= mn[m]);
-    const bool lower = v < lo;
-    const bool inter = (v <= hi) & (v >= lo);
+    for (int j = 1; j < DIM; ++j) in = in & (x[j] <= hi[j]) & (x[j] >= lo[j]);
+    const bool lower = x[0] < lo[0];
     if (FILL) {
       rw[wr & (kRing - 1)] = id;  // the slot is free; it is kept only for a hit
       wr += in ? 1u : 0u;
@@ -208,13 +214,12 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     const bool go_l = !lower && node < tl;
     const bool go_r = (lower || inter) && node < tr;
     const bool desc = go_l || go_r;
-    pending = (go_l && go_r) ? (pending | (2u << depth)) : pending;
-    // deepest pending depth (bit b set, b >= 1); only used when pending != 0
-    const uint32_t b = 31u - (uint32_t)__builtin_clz(pending | 1u);
-    const uint32_t up = (node + 1u) >> (depth - b);
-    alive = desc || pending != 0u;
-    const uint32_t lc = 2u * node + 1u;
-    node = go_l ? lc : (go_r ? lc + 1u : up);
+    pending |= (go_l && go_r) ? (2u << depth) : 0u;
+    const uint32_t b = 31u - (uint32_t)__builtin_clz(pending);  // deepest pending depth
+    const uint32_t up = (node + 1u) >> (depth - b);              // its right son
+    const uint32_t down = 2u * node + (go_l ? 1u : 2u);
+    alive = desc || pending != 1u;
+    node = desc ? down : up;
     pending = desc ? pending : __builtin_amdgcn_ubfe(pending, 0u, b);  // clears bit b
     depth = desc ? depth + 1u : b;
   }
@@ -224,17 +229,17 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
       if (wb + j < a.cap) a.out[wb + j] = rw[j & (kRing - 1)];
   }
   if (!FILL) a.counts[q] = c;
-  if (MODE == kCount && a.counts_slot) a.counts_slot[slot] = c;
+  if (MODE == kCount && a.work_slot) a.work_slot[slot] = vis;
   if (MODE == kStats) a.visited[q] = vis;
 }
 
-// Work estimate of every search block for the fill pass: the largest hit
-// count among its queries (the count pass's counts in processing order).
-__global__ __launch_bounds__(kThreads) void block_work_kernel(const uint32_t* __restrict__ counts_slot, uint32_t nq,
+// Work estimate of every search block for the fill pass: the most nodes any
+// of its queries visited in the count pass (visits in processing order).
+__global__ __launch_bounds__(kThreads) void block_work_kernel(const uint32_t* __restrict__ work_slot, uint32_t nq,
                                                               uint32_t* __restrict__ work) {
   __shared__ uint32_t s_max[kThreads / kWave];
   const uint32_t slot = blockIdx.x * kThreads + threadIdx.x;
-  uint32_t h = slot < nq ? counts_slot[slot] : 0u;
+  uint32_t h = slot < nq ? work_slot[slot] : 0u;
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) h = max(h, (uint32_t)__shfl_xor((int)h, o));
   if (lane_id() == 0) s_max[threadIdx.x / kWave] = h;
@@ -244,6 +249,71 @@ __global__ __launch_bounds__(kThreads) void block_work_kernel(const uint32_t* __
 #pragma unroll
     for (int w = 0; w < kThreads / kWave; ++w) m = max(m, s_max[w]);
     work[blockIdx.x] = m;
+  }
+}
+
+// Fill-pass lane balance: a wave runs as long as its slowest lane, and
+// neighbouring boxes in the spatial order can differ in work by orders of
+// magnitude (box sizes vary, blobs sit next to sparse regions).  Every window
+// of 256 ITEMS consecutive processing slots is reordered by decreasing work
+// (the count pass's visits, bucketed on log2 with three fraction bits), so
+// the lanes of a wave get similar work while staying inside one spatial
+// window.  Writes slot_order (dispatch slot -> processing slot) and the work
+// of every search block (its first, heaviest, slot).  The order inside a
+// bucket follows LDS atomics and only affects time.
+constexpr uint32_t kSortBuckets = 256;
+__device__ inline uint32_t sort_bucket(uint32_t w) {
+  const uint32_t lg = floor_log2(w | 1u);
+  const uint32_t frac = lg >= 3u ? (w >> (lg - 3u)) & 7u : (w << (3u - lg)) & 7u;
+  return (31u - lg) * 8u + (7u - frac);  // 0 = heaviest
+}
+template <int ITEMS>
+__global__ __launch_bounds__(kThreads) void window_sort_kernel(const uint32_t* __restrict__ work_slot, uint32_t nq,
+                                                               uint32_t* __restrict__ slot_order,
+                                                               uint32_t* __restrict__ block_work) {
+  static_assert(kSortBuckets == kThreads, "one bucket per thread in the scan");
+  __shared__ uint32_t hist[kSortBuckets];
+  __shared__ uint32_t wsum[kThreads / kWave];
+  constexpr uint32_t W = kThreads * ITEMS;
+  const uint32_t base = blockIdx.x * W;
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t w[ITEMS], bk[ITEMS], rk[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t s = base + (uint32_t)j * kThreads + threadIdx.x;
+    w[j] = s < nq ? work_slot[s] : 0u;
+    bk[j] = sort_bucket(w[j]);
+    rk[j] = s < nq ? atomicAdd(&hist[bk[j]], 1u) : 0u;
+  }
+  __syncthreads();
+  // exclusive scan of the bucket sizes (bucket = thread)
+  const uint32_t h = hist[threadIdx.x];
+  uint32_t x = h;
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == kWave - 1) wsum[threadIdx.x / kWave] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+#pragma unroll
+  for (int v = 0; v < kThreads / kWave; ++v)
+    if ((uint32_t)v < threadIdx.x / kWave) pre += wsum[v];
+  const uint32_t start = pre + x - h;
+  __syncthreads();
+  hist[threadIdx.x] = start;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t s = base + (uint32_t)j * kThreads + threadIdx.x;
+    if (s < nq) {
+      const uint32_t d = base + hist[bk[j]] + rk[j];
+      slot_order[d] = s;
+      if ((d & (kThreads - 1u)) == 0u) block_work[d / kThreads] = w[j];
+    }
   }
 }
 

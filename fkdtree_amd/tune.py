@@ -3,8 +3,9 @@
     python -m fkdtree_amd.tune [--workload mixed_1m] [--reps 3]
 
 Sweeps `query_key_bits` (Morton bits per dimension of the box order; 0
-keeps the caller's order) and `fill_order` (heaviest search blocks first in
-the fill pass) on a workload, timing the two-pass search with HIP events,
+keeps the caller's order), `fill_order` (heaviest search blocks first in
+the fill pass) and `fill_window` (boxes reordered by work inside windows of
+that many boxes for the fill pass) on a workload, timing the two-pass search with HIP events,
 and stores the
 fastest setting in `tuning.json` next to this file, keyed by GPU
 architecture, dimension and log2 of the point count.  Trees created by this
@@ -25,7 +26,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1)}
+CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill_window": (0, 256, 1024, 4096)}
 
 
 def _arch() -> str:
