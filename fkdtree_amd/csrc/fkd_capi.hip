@@ -192,7 +192,7 @@ struct Tiling {
 struct fkd_tuning {
   uint32_t query_key_bits = 8;  // search: Morton bits per dimension of the box order
   uint32_t fill_order = 1;      // fill pass: heaviest search blocks first
-  uint32_t fill_window = 1024;  // fill pass: boxes reordered by work inside windows of this many (0: off)
+  uint32_t fill_window = 4096;  // fill pass: boxes reordered by work inside windows of this many (0: off)
 };
 
 struct fkd_tree_s {
