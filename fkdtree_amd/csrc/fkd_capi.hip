@@ -191,7 +191,7 @@ struct Tiling {
 
 struct fkd_tuning {
   uint32_t query_key_bits = 8;  // search: Morton bits per dimension of the box order
-  uint32_t fill_order = 1;      // fill pass: heaviest search blocks first
+  uint32_t fill_order = 1;      // fill pass: heaviest search blocks first (0 for handles of several events)
   uint32_t fill_window = 4096;  // fill pass: boxes reordered by work inside windows of this many (0: off)
 };
 
@@ -468,7 +468,8 @@ uint32_t* plan_perm(fkd_tree_s* t) {
 }
 
 template <int DIM>
-int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, hipStream_t s) {
+int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                 hipStream_t s) {
   const size_t n = std::max<uint32_t>(nq, 1);
   FKD_TRY(t->q_keys.ensure(n * 2 * 4));
   FKD_TRY(t->q_perm.ensure(n * 2 * 4));
@@ -480,16 +481,24 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   uint32_t* keys = t->q_keys.as<uint32_t>();
   uint32_t* perm = t->q_perm.as<uint32_t>();
   const uint32_t g = grid_for(nq);
-  const uint32_t kb = t->tune.query_key_bits;
+  // Key: the event (when queries name one of several trees) above the
+  // Morton bits; Morton bits are dropped until both fit in 32 bits.
+  uint32_t kb = t->tune.query_key_bits;
+  uint32_t ev_bits = 0;
+  if (qev && t->n_events > 1 && kb > 0) {
+    while (ev_bits < 32u && (1ull << ev_bits) < (uint64_t)t->n_events) ++ev_bits;
+    kb = std::min<uint32_t>(kb, (32u - ev_bits) / (uint32_t)DIM);
+  }
+  const uint32_t ev_shift = kb * (uint32_t)DIM;
   FKD_TRY(t->q_aos.ensure(n * 32));
-  fkd::query_keys_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), kb, keys, perm,
-                                                t->q_aos.as<float4>());
+  fkd::query_keys_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), kb, ev_bits ? qev : nullptr,
+                                                ev_shift, keys, perm, t->q_aos.as<float4>());
   FKD_TRY(launch_check("query_keys"));
   fkd::TileMap tm{};
   tm.tile_ev = nullptr;
   tm.T = T;
   tm.single_n = nq;
-  const int passes = (int)(kb * DIM + 7) / 8;
+  const int passes = (int)(ev_shift + ev_bits + 7) / 8;
   // one clear for the digit passes and the scan of the count pass
   ZeroArena za;
   FKD_TRY(zero_arena(t, passes * radix_pass_words(1, T) + fkd::scan_tiles(nq) + 1, s, za));
@@ -520,7 +529,7 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
                       uint64_t* qoff, hipStream_t s) {
   FKD_TRY(t->q_counts.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
   if (nq > 0) {
-    FKD_TRY(plan_queries<DIM>(t, nq, mins, maxs, s));
+    FKD_TRY(plan_queries<DIM>(t, nq, mins, maxs, qev, s));
     fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
     const size_t n = std::max<uint32_t>(nq, 1);
     a.perm = plan_perm(t);
@@ -755,6 +764,9 @@ int create_common(int dim, uint32_t n_events, const uint64_t* event_offsets, fkd
   t->dim = dim;
   t->n_events = n_events;
   t->P = (uint32_t)P;
+  // Several trees: starting the heaviest blocks first scatters the writes of
+  // an event's boxes over the whole output (measured slower for batches).
+  if (n_events > 1) t->tune.fill_order = 0;
   t->h_off.assign(event_offsets, event_offsets + n_events + 1);
   const size_t S = std::max<uint64_t>(P, 1);
   int r = t->X.ensure(S * dim * 4);

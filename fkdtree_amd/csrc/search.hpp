@@ -353,13 +353,16 @@ __global__ __launch_bounds__(1024) void block_order_kernel(const uint32_t* __res
 }
 
 // Spatial sort key of every query: Morton interleave of its box centre,
-// quantised to kb bits per dimension inside the tree's bounding box.  Also
+// quantised to kb bits per dimension inside the tree's bounding box, below
+// the query's event (qev, at bit ev_shift) when the handle holds several
+// trees, so that a wave walks one event's tree.  Also
 // writes the boxes as 32-byte records (mins, then maxs, each padded to 4), so
 // that the permutation gathers one record per box instead of 2 DIM floats.
 // Any key order gives the same results; this one only groups similar work.
 template <int DIM>
 __global__ void query_keys_kernel(const float* __restrict__ mins, const float* __restrict__ maxs,
                                   uint32_t nq, const uint32_t* __restrict__ bbox, uint32_t kb,
+                                  const uint32_t* __restrict__ qev, uint32_t ev_shift,
                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                   float4* __restrict__ aos) {
   static_assert(DIM <= 4, "box records hold 4 floats per bound");
@@ -390,6 +393,7 @@ __global__ void query_keys_kernel(const float* __restrict__ mins, const float* _
     for (uint32_t b = 0; b < kb; ++b)
 #pragma unroll
       for (int m = 0; m < DIM; ++m) key |= ((u[m] >> b) & 1u) << (b * DIM + m);
+    if (qev && ev_shift < 32u) key |= qev[i] << ev_shift;
     keys[i] = key;
     vals[i] = i;
     aos[2 * (uint64_t)i] = make_float4(b[0], b[1], b[2], b[3]);

@@ -36,8 +36,10 @@ def _arch() -> str:
     return name.split("(")[-1].split(":")[0].rstrip(")") or "unknown"
 
 
-def _key(dim: int, n: int) -> str:
-    return f"dim{dim}_log2n{int(round(math.log2(max(n, 2))))}"
+def _key(dim: int, n: int, events: bool = False) -> str:
+    """Table key: dimension, log2 of the (mean per-event) tree size, and
+    whether the handle holds several events (their searches tune apart)."""
+    return f"dim{dim}_{'ev_' if events else ''}log2n{int(round(math.log2(max(n, 2))))}"
 
 
 def load_table(path: str = TABLE) -> Dict:
@@ -47,14 +49,17 @@ def load_table(path: str = TABLE) -> Dict:
         return json.load(f)
 
 
-def lookup(arch: str, dim: int, n: int, table: Optional[Dict] = None) -> Optional[Dict[str, int]]:
-    """Stored parameters of the nearest size bucket for (arch, dim), if any."""
+def lookup(arch: str, dim: int, n: int, table: Optional[Dict] = None,
+           events: bool = False) -> Optional[Dict[str, int]]:
+    """Stored parameters of the nearest size bucket for (arch, dim, one tree
+    or several events), if any."""
     table = load_table() if table is None else table
     entries = table.get(arch, {})
     best, dist = None, None
     want = math.log2(max(n, 2))
+    prefix = f"dim{dim}_ev_" if events else f"dim{dim}_"
     for k, v in entries.items():
-        if not k.startswith(f"dim{dim}_"):
+        if not k.startswith(prefix) or (not events and k.startswith(f"dim{dim}_ev_")):
             continue
         d = abs(int(k.split("log2n")[1]) - want)
         if dist is None or d < dist:
@@ -65,7 +70,8 @@ def lookup(arch: str, dim: int, n: int, table: Optional[Dict] = None) -> Optiona
 def apply(tree, arch: Optional[str] = None) -> Optional[Dict[str, int]]:
     """Set the stored parameters on a tree handle; returns them (or None)."""
     try:
-        params = lookup(arch or _arch(), tree.dim, max(tree.n // max(tree.n_events, 1), 1))
+        params = lookup(arch or _arch(), tree.dim, max(tree.n // max(tree.n_events, 1), 1),
+                        events=tree.n_events > 1)
     except Exception:
         return None
     applied = {}
@@ -110,28 +116,45 @@ def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=Non
 
 
 def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path: str = TABLE) -> Dict:
-    from . import DeviceArray, FKDTree, Stream, gen
+    """Sweep on a workload: one tree (mixed_1m, uniform_100k, uniform_4m) or
+    events_256 (the first 256 events of the 2,048-event set, every point's
+    box of half-width 0.01 searched in its own event's tree)."""
+    from . import DeviceArray, FKDEvents, FKDTree, Stream, gen
 
-    pts, boxes = {"mixed_1m": gen.workload_mixed_1m, "uniform_100k": gen.workload_uniform_100k,
-                  "uniform_4m": gen.workload_uniform_4m}[workload]()
     s = Stream()
+    d_q = None
+    if workload == "events_256":
+        seed = gen.workload_seed("events_2048")
+        sizes = gen.event_sizes(2048, seed, 1000, gen.LOG2_200)[:256]
+        batch = gen.concat_events([gen.mixed_event(int(sz), seed, e) for e, sz in enumerate(sizes)])
+        pts, boxes = batch.points, gen.event_self_boxes(batch, 0.01)
+        qev = np.repeat(np.arange(256, dtype=np.uint32), np.diff(batch.offsets))
+        d_q = DeviceArray.from_host(qev, s)
+    else:
+        pts, boxes = {"mixed_1m": gen.workload_mixed_1m, "uniform_100k": gen.workload_uniform_100k,
+                      "uniform_4m": gen.workload_uniform_4m}[workload]()
     d_c = DeviceArray.from_host(pts.coords, s)
     d_i = DeviceArray.from_host(pts.ids, s)
     d_mi = DeviceArray.from_host(boxes.mins, s)
     d_ma = DeviceArray.from_host(boxes.maxs, s)
-    tree = FKDTree.from_device(pts.dim, pts.n, d_c.ptr, d_i.ptr, s)
+    if d_q is not None:
+        tree = FKDEvents.from_device(pts.dim, batch.offsets, d_c.ptr, d_i.ptr, s)
+        key = _key(pts.dim, pts.n // 256, events=True)
+    else:
+        tree = FKDTree.from_device(pts.dim, pts.n, d_c.ptr, d_i.ptr, s)
+        key = _key(pts.dim, pts.n)
     tree.build()
-    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps)
+    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=d_q.ptr if d_q is not None else None)
     tree.close()
     arch = _arch()
     entry = {"params": res[0][0], "ms": res[0][1], "workload": workload,
              "sweep": [{"params": p, "ms": ms} for p, ms in res]}
     if write:
         table = load_table(path)
-        table.setdefault(arch, {})[_key(pts.dim, pts.n)] = entry
+        table.setdefault(arch, {})[key] = entry
         with open(path, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
-    return {"arch": arch, "key": _key(pts.dim, pts.n), **entry}
+    return {"arch": arch, "key": key, **entry}
 
 
 def main(argv=None):

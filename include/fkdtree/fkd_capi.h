@@ -119,11 +119,12 @@ int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t str
  *   "query_key_bits"  Morton bits per dimension of the box order (0 keeps
  *                     the caller's order), key bits * dim <= 32;
  *   "fill_order"      1: the fill pass starts the search blocks with the
- *                     most work (nodes visited) first, 0: in box order;
+ *                     most work (nodes visited) first, 0: in box order
+ *                     (default 1 for one tree, 0 for several events);
  *   "fill_window"     the fill pass reorders the boxes by work inside
  *                     windows of this many consecutive boxes of the box
  *                     order, so the lanes of a wave get similar work
- *                     (0, 256, 1024 or 4096; 0 = off). */
+ *                     (0, 256, 1024 or 4096; 0 = off; default 4096). */
 int fkd_set_tuning(fkd_tree_t tree, const char* key, uint32_t value);
 int fkd_get_tuning(fkd_tree_t tree, const char* key, uint32_t* value);
 

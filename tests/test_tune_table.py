@@ -23,12 +23,25 @@ def test_lookup_nearest_bucket(tmp_path):
 def test_keys():
     assert tune._key(3, 1_000_000) == "dim3_log2n20"
     assert tune._key(3, 1) == "dim3_log2n1"
+    assert tune._key(3, 41_000, events=True) == "dim3_ev_log2n15"
+
+
+def test_event_entries_are_kept_apart():
+    """Handles of several events read only the event entries, single trees
+    only the others (their best launch parameters differ)."""
+    table = {"gfx950": {"dim3_log2n20": {"params": {"fill_window": 4096}},
+                        "dim3_ev_log2n15": {"params": {"fill_window": 0}}}}
+    assert tune.lookup("gfx950", 3, 1 << 15, table) == {"fill_window": 4096}
+    assert tune.lookup("gfx950", 3, 1 << 15, table, events=True) == {"fill_window": 0}
+    assert tune.lookup("gfx950", 3, 1 << 22, table, events=True) == {"fill_window": 0}
+    assert tune.lookup("gfx950", 2, 1 << 15, table, events=True) is None
 
 
 def test_candidates_are_valid():
     # query_key_bits * dim must fit a 32-bit key at dim 3; fill_order is 0/1
     assert all(0 <= kb <= 10 for kb in tune.CANDIDATES["query_key_bits"])
     assert set(tune.CANDIDATES["fill_order"]) <= {0, 1}
+    assert set(tune.CANDIDATES["fill_window"]) <= {0, 256, 1024, 4096}
 
 
 class _FakeTree:
