@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--serve-events", type=int, default=128,
                     help="events of the serving stream (0 skips)")
     ap.add_argument("--serve-rates", default="1000,5000,20000", help="Poisson arrival rates (events/s)")
+    ap.add_argument("--events-only", action="store_true",
+                    help="time only the batched events (build + search per batch) and exit; for profiling")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 256 MiB copy kernel (counter calibration) and exit")
     return ap.parse_args()
@@ -78,6 +80,9 @@ def main():
             check(lib().fkd_copy_u32(b.ptr, a.ptr, n, s.handle))
         s.sync()
         print(json.dumps({"calibration": "copy_u32_kernel", "bytes_read": 4 * n, "bytes_written": 4 * n}))
+        return 0
+    if args.events_only:
+        print(json.dumps({"events": bench_events(args.events or 256, args.event_steps, batched_only=True)}))
         return 0
     pts, boxes = workload(args.workload)
     n, q, dim = pts.n, boxes.q, pts.dim
@@ -246,7 +251,7 @@ def events_batch(n_events: int):
     return batch, gen.event_self_boxes(batch, 0.01)
 
 
-def bench_events(n_events: int, steps: int) -> dict:
+def bench_events(n_events: int, steps: int, batched_only: bool = False) -> dict:
     """Events per second of build + self-search (every point of an event is
     the centre of one box of half-width 0.01 searched in its own tree), for
     the first n_events events of the 2,048-event workload (sizes log-uniform
@@ -294,6 +299,10 @@ def bench_events(n_events: int, steps: int) -> dict:
         batched()
     tm.stop(s)
     batched_ms = tm.ms() / steps
+    if batched_only:
+        ev.close()
+        return {"n_events": n_events, "points": P, "queries": P, "hits": int(total), "batched_ms": batched_ms,
+                "batched_events_per_s": n_events / (batched_ms / 1e3)}
     g = F.Graph(ev, P, d_mi.ptr, d_ma.ptr, d_off.ptr, d_out.ptr, cap, s, d_qev=d_qev.ptr)
     g.launch()
     s.sync()
