@@ -53,17 +53,6 @@ __device__ inline uint32_t search_block(uint32_t b, uint32_t nb, const uint32_t*
   return order ? order[j] : j;
 }
 
-// depth % DIM for depth < 32 without a 32-bit multiply: for DIM 3,
-// floor(depth / 3) == (depth * 11) >> 5 on 0..31.
-template <int DIM>
-__device__ inline uint32_t depth_dim(uint32_t depth) {
-  if (DIM == 1) return 0u;
-  if (DIM == 2) return depth & 1u;
-  if (DIM == 4) return depth & 3u;
-  if (DIM == 3) return depth - 3u * (__umul24(depth, 11u) >> 5);
-  return depth % (uint32_t)DIM;
-}
-
 // Node records are read through a buffer resource (32-bit byte offsets,
 // reads past num_records return zero) when the records fit in 4 GiB, else
 // through a flat pointer.  The resource spans the whole batch (uniform across
