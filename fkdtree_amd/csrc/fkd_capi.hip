@@ -454,6 +454,7 @@ fkd::SearchArgs search_args(fkd_tree_s* t, uint32_t nq, const uint32_t* qev) {
   fkd::SearchArgs a{};
   a.recs = t->recs.as<float>();
   a.off = t->n_events > 1 || qev ? t->off.as<uint32_t>() : nullptr;
+  a.n_events = t->n_events;
   a.n_single = t->P;
   a.qev = qev;
   a.nq = nq;

@@ -23,6 +23,7 @@ namespace fkd {
 struct SearchArgs {
   const float* recs;      // node records, NodeRec<DIM>::kFloats floats each
   const uint32_t* off;    // event -> first node (n_events + 1), or null
+  uint32_t n_events;      // events of off (a query naming another event matches nothing)
   uint32_t n_single;      // node count when off == null
   const uint32_t* qev;    // query -> event, or null (event 0)
   const uint32_t* perm;   // processing slot -> query (null: identity)
@@ -122,8 +123,12 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   uint32_t base = 0, n = a.n_single;
   if (a.off) {
     const uint32_t e = a.qev ? a.qev[q] : 0u;
-    base = a.off[e];
-    n = a.off[e + 1] - base;
+    if (e < a.n_events) {
+      base = a.off[e];
+      n = a.off[e + 1] - base;
+    } else {
+      n = 0;
+    }
   }
   float mn[DIM], mx[DIM];
   float2* const my_box = &s_box[LDS_BOX ? threadIdx.x * BS : 0];

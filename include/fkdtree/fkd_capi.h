@@ -135,7 +135,8 @@ int fkd_device_bytes(fkd_tree_t tree, uint64_t* bytes);
 
 /* Count pass + scan: dev_offsets[q] = first output slot of query q,
  * dev_offsets[nq] = total hits (uint64).  dev_query_events (may be NULL:
- * event 0) names the event whose tree query q searches.  Enqueues only.
+ * event 0) names the event whose tree query q searches; a query naming no
+ * event of the handle (>= n_events) matches nothing.  Enqueues only.
  * Replaces, with search_fill: search_in_the_box (FKDTree_cpu.h:123-169) for
  * nq boxes at once; the counts alone are the result of
  * FKDTree_OpenCL::search_in_the_box_multiple (FKDTree_opencl.h:123-164). */

@@ -82,6 +82,29 @@ def test_events_search(require_gpu):
     assert np.array_equal(sorted_sets(goff, gids), sorted_sets(ooff, oids))
 
 
+def test_events_query_naming_no_event(require_gpu):
+    """A box whose event id is not an event of the handle matches nothing;
+    the other boxes keep their results."""
+    b = gen.workload_events_2048(n_events=8, scale=0.02)
+    boxes = gen.event_self_boxes(b, 0.02)
+    qev = np.repeat(np.arange(b.n_events, dtype=np.uint32), np.diff(b.offsets))
+    bad = qev.copy()
+    bad[::7] = np.uint32(b.n_events)
+    bad[3::11] = np.uint32(0xFFFFFFFF)
+    print(f"[search] events with bad ids n={b.points.n}", flush=True)
+    t = FKDEvents(b.points.coords, b.points.ids, b.offsets)
+    t.build()
+    goff, gids = t.search(boxes.mins, boxes.maxs, bad)
+    ooff, oids = t.search(boxes.mins, boxes.maxs, qev)
+    t.close()
+    cnt, ocnt = np.diff(goff), np.diff(ooff)
+    named = bad < b.n_events
+    assert np.all(cnt[~named] == 0)
+    assert np.array_equal(cnt[named], ocnt[named])
+    keep = np.repeat(named, ocnt.astype(np.int64))
+    assert np.array_equal(sorted_sets(goff, gids), sorted_sets(np.concatenate([[0], np.cumsum(ocnt * named)]).astype(np.uint64), oids[keep]))
+
+
 def _device_search(t, boxes, single: bool, capacity=None):
     from fkdtree_amd import DeviceArray
     q = boxes.q
