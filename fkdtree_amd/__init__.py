@@ -352,7 +352,7 @@ class _TreeBase:
         return tot.value, bool(ok.value)
 
     def set_tuning(self, **params) -> None:
-        """Launch parameters (fkd_set_tuning): query_key_bits, fill_order, fill_window."""
+        """Launch parameters (fkd_set_tuning): query_key_bits, fill_order, fill_window, count_window."""
         for k, v in params.items():
             check(lib().fkd_set_tuning(self._h, k.encode(), int(v)))
 

@@ -193,6 +193,7 @@ struct fkd_tuning {
   uint32_t query_key_bits = 8;  // search: Morton bits per dimension of the box order
   uint32_t fill_order = 1;      // fill pass: heaviest search blocks first (0 for handles of several events)
   uint32_t fill_window = 4096;  // fill pass: boxes reordered by work inside windows of this many (0: off)
+  uint32_t count_window = 0;    // count pass: boxes reordered by volume inside windows of this many (0: off)
 };
 
 struct fkd_tree_s {
@@ -525,6 +526,22 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   return FKD_OK;
 }
 
+// Dispatch order (t->q_sorder) with every window of `win` processing slots
+// ordered by decreasing work[slot], and the work of every search block
+// (t->q_bwork).
+int window_sort(fkd_tree_s* t, uint32_t win, const uint32_t* work, uint32_t nq, hipStream_t s) {
+  const uint32_t nb = (uint32_t)((nq + fkd::kThreads - 1) / fkd::kThreads);
+  FKD_TRY(t->q_sorder.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
+  FKD_TRY(t->q_bwork.ensure((size_t)std::max<uint32_t>(nb, 1) * 4));
+  const uint32_t nw = (uint32_t)((nq + win - 1) / win);
+  uint32_t* so = t->q_sorder.as<uint32_t>();
+  uint32_t* bw = t->q_bwork.as<uint32_t>();
+  if (win == 256) fkd::window_sort_kernel<1><<<nw, fkd::kThreads, 0, s>>>(work, nq, so, bw);
+  else if (win == 1024) fkd::window_sort_kernel<4><<<nw, fkd::kThreads, 0, s>>>(work, nq, so, bw);
+  else fkd::window_sort_kernel<16><<<nw, fkd::kThreads, 0, s>>>(work, nq, so, bw);
+  return launch_check("window_sort");
+}
+
 template <int DIM>
 int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                       uint64_t* qoff, hipStream_t s) {
@@ -539,6 +556,17 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
     a.counts = t->q_counts.as<uint32_t>();
     FKD_TRY(t->q_wslot.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
     a.work_slot = t->q_wslot.as<uint32_t>();
+    const uint32_t cw = t->tune.count_window;
+    if (cw != 0 && nq > cw) {
+      // Work is unknown before counting; the box volume (expected hits for
+      // the handle's mean point density) stands in for it.  The count pass
+      // overwrites the estimate with the visited counts.
+      const float density = (float)t->P / (float)std::max<uint32_t>(t->n_events, 1u);
+      fkd::volume_work_kernel<DIM><<<grid_for(nq), 256, 0, s>>>(a.mins, a.maxs, nq, density, a.work_slot);
+      FKD_TRY(launch_check("volume_work"));
+      FKD_TRY(window_sort(t, cw, a.work_slot, nq, s));
+      a.slot_order = t->q_sorder.as<uint32_t>();
+    }
     FKD_TRY((launch_search<DIM, fkd::kCount>(t, a, s, "search_count")));
   }
   const uint64_t st = fkd::scan_tiles(nq);
@@ -586,16 +614,8 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
   const bool ordered = t->tune.fill_order && nb >= 64;
   if (sorted || ordered) FKD_TRY(t->q_bwork.ensure((size_t)nb * 4));
   if (sorted) {
-    FKD_TRY(t->q_sorder.ensure((size_t)nq * 4));
-    const uint32_t nw = (uint32_t)((nq + win - 1) / win);
-    uint32_t* so = t->q_sorder.as<uint32_t>();
-    uint32_t* bw = t->q_bwork.as<uint32_t>();
-    const uint32_t* ws = t->q_wslot.as<uint32_t>();
-    if (win == 256) fkd::window_sort_kernel<1><<<nw, fkd::kThreads, 0, s>>>(ws, nq, so, bw);
-    else if (win == 1024) fkd::window_sort_kernel<4><<<nw, fkd::kThreads, 0, s>>>(ws, nq, so, bw);
-    else fkd::window_sort_kernel<16><<<nw, fkd::kThreads, 0, s>>>(ws, nq, so, bw);
-    FKD_TRY(launch_check("window_sort"));
-    a.slot_order = so;
+    FKD_TRY(window_sort(t, win, t->q_wslot.as<uint32_t>(), nq, s));
+    a.slot_order = t->q_sorder.as<uint32_t>();
   }
   if (ordered) {
     FKD_TRY(t->q_border.ensure((size_t)nb * 4));
@@ -1189,10 +1209,10 @@ int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
   } else if (k == "fill_order") {
     if (value > 1) return fail(FKD_ERR_INVALID, "fill_order must be 0 or 1");
     t->tune.fill_order = value;
-  } else if (k == "fill_window") {
+  } else if (k == "fill_window" || k == "count_window") {
     if (value != 0 && value != 256 && value != 1024 && value != 4096)
-      return fail(FKD_ERR_INVALID, "fill_window must be 0, 256, 1024 or 4096");
-    t->tune.fill_window = value;
+      return fail(FKD_ERR_INVALID, k + " must be 0, 256, 1024 or 4096");
+    (k == "fill_window" ? t->tune.fill_window : t->tune.count_window) = value;
   } else {
     return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   }
@@ -1205,6 +1225,7 @@ int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
   if (k == "query_key_bits") *value = t->tune.query_key_bits;
   else if (k == "fill_order") *value = t->tune.fill_order;
   else if (k == "fill_window") *value = t->tune.fill_window;
+  else if (k == "count_window") *value = t->tune.count_window;
   else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;
 }

@@ -28,7 +28,7 @@ struct SearchArgs {
   const uint32_t* qev;    // query -> event, or null (event 0)
   const uint32_t* perm;   // processing slot -> query (null: identity)
   const uint32_t* block_order;  // dispatch position -> search block (null: identity)
-  const uint32_t* slot_order;   // fill pass: dispatch slot -> processing slot (null: identity)
+  const uint32_t* slot_order;   // dispatch slot -> processing slot (null: identity)
   uint32_t nq;
   const float* mins;      // [DIM][nq], indexed by processing slot
   const float* maxs;
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   }
   uint32_t slot = search_block(blockIdx.x, gridDim.x, a.block_order) * kThreads + threadIdx.x;
   if (slot >= a.nq) return;
-  if (FILL && a.slot_order) slot = a.slot_order[slot];
+  if (a.slot_order) slot = a.slot_order[slot];
   const uint32_t q = a.perm ? a.perm[slot] : slot;
   uint32_t base = 0, n = a.n_single;
   if (a.off) {
@@ -308,6 +308,24 @@ __global__ __launch_bounds__(kThreads) void window_sort_kernel(const uint32_t* _
       slot_order[d] = s;
       if ((d & (kThreads - 1u)) == 0u) block_work[d / kThreads] = w[j];
     }
+  }
+}
+
+// Work estimate of every box before counting (processing order): the
+// expected number of hits, box volume times the mean point density of the
+// trees (at least 1, saturating).  Only its order matters.
+template <int DIM>
+__global__ void volume_work_kernel(const float* __restrict__ mins, const float* __restrict__ maxs, uint32_t nq,
+                                   float density, uint32_t* __restrict__ work) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+    float v = density;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+      const float e = maxs[(uint64_t)m * nq + i] - mins[(uint64_t)m * nq + i];
+      v *= e > 0.f ? e : 0.f;  // also maps NaN to 0
+    }
+    v = v * 16.f + 1.f;
+    work[i] = v < 2.0e9f ? (uint32_t)v : 2000000000u;
   }
 }
 

@@ -26,7 +26,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill_window": (0, 256, 1024, 4096)}
+CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill_window": (0, 256, 1024, 4096),
+              "count_window": (0, 1024, 4096)}
 
 
 def _arch() -> str:

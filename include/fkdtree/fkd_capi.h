@@ -124,7 +124,11 @@ int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t str
  *   "fill_window"     the fill pass reorders the boxes by work inside
  *                     windows of this many consecutive boxes of the box
  *                     order, so the lanes of a wave get similar work
- *                     (0, 256, 1024 or 4096; 0 = off; default 4096). */
+ *                     (0, 256, 1024 or 4096; 0 = off; default 4096);
+ *   "count_window"    the count pass reorders the boxes by volume (their
+ *                     expected work) inside windows of this many boxes
+ *                     (0, 256, 1024 or 4096; default 0 = off: it pays when
+ *                     box sizes vary widely). */
 int fkd_set_tuning(fkd_tree_t tree, const char* key, uint32_t value);
 int fkd_get_tuning(fkd_tree_t tree, const char* key, uint32_t* value);
 

@@ -1,8 +1,9 @@
 """Launch parameters change time only: the two-pass search and the one-call
 search return the oracle's id sets (FKDTree_cpu.h:123-169) for every query
-order (`query_key_bits`, 0 = the caller's order), fill dispatch order and
-fill window (boxes reordered by work; 30,000 boxes, so windows of 256, 1024
-and 4096 with a partial last window)."""
+order (`query_key_bits`, 0 = the caller's order), fill dispatch order, fill
+window (boxes reordered by work) and count window (boxes reordered by
+volume before counting); 30,000 boxes, so windows of 256, 1024 and 4096 with
+a partial last window."""
 import numpy as np
 import pytest
 
@@ -46,16 +47,17 @@ def test_tuning_keeps_results(require_gpu):
     t.build()
     dmi = DeviceArray.from_host(boxes.mins)
     dma = DeviceArray.from_host(boxes.maxs)
-    for kb, fo, fw in ((0, 1, 0), (1, 0, 256), (5, 1, 4096), (10, 0, 1024), (8, 1, 1024), (8, 0, 0)):
-        t.set_tuning(query_key_bits=kb, fill_order=fo, fill_window=fw)
+    for kb, fo, fw, cw in ((0, 1, 0, 0), (1, 0, 256, 4096), (5, 1, 4096, 1024), (10, 0, 1024, 0), (8, 1, 1024, 256),
+                           (8, 0, 0, 4096)):
+        t.set_tuning(query_key_bits=kb, fill_order=fo, fill_window=fw, count_window=cw)
         assert t.get_tuning("query_key_bits") == kb and t.get_tuning("fill_order") == fo
-        assert t.get_tuning("fill_window") == fw
-        tag = f"key bits {kb}, fill order {fo}, window {fw}"
+        assert t.get_tuning("fill_window") == fw and t.get_tuning("count_window") == cw
+        tag = f"key bits {kb}, fill order {fo}, window {fw}, count window {cw}"
         o, ids = _two_pass(t, dmi, dma, boxes.q)
         assert np.array_equal(o, ooff), f"{tag}: counts differ"
         assert np.array_equal(sorted_sets(o, ids), want), f"{tag}: ids differ"
-    for fo, fw in ((0, 0), (1, 1024), (0, 4096)):
-        t.set_tuning(fill_order=fo, fill_window=fw)
+    for fo, fw, cw in ((0, 0, 0), (1, 1024, 4096), (0, 4096, 1024)):
+        t.set_tuning(fill_order=fo, fill_window=fw, count_window=cw)
         o, ids = _one_call(t, dmi, dma, boxes.q, int(ooff[-1]))
         assert np.array_equal(o, ooff), f"single call, fill order {fo}, window {fw}: counts differ"
         assert np.array_equal(sorted_sets(o, ids), want), f"single call, fill order {fo}, window {fw}: ids differ"
@@ -66,7 +68,7 @@ def test_tuning_rejects_bad_values(require_gpu):
     p = adversarial_points("mixed", 100)
     t = FKDTree(coords=p.coords, ids=p.ids)
     for k, v in (("pool_batch", 32), ("query_key_bits", 11), ("fill_order", 2), ("fill_window", 512),
-                 ("no_such_key", 1)):
+                 ("count_window", 100), ("no_such_key", 1)):
         with pytest.raises(RuntimeError):
             t.set_tuning(**{k: v})
     t.close()

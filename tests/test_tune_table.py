@@ -42,6 +42,7 @@ def test_candidates_are_valid():
     assert all(0 <= kb <= 10 for kb in tune.CANDIDATES["query_key_bits"])
     assert set(tune.CANDIDATES["fill_order"]) <= {0, 1}
     assert set(tune.CANDIDATES["fill_window"]) <= {0, 256, 1024, 4096}
+    assert set(tune.CANDIDATES["count_window"]) <= {0, 256, 1024, 4096}
 
 
 class _FakeTree:
