@@ -72,3 +72,26 @@ def test_tuning_rejects_bad_values(require_gpu):
         with pytest.raises(RuntimeError):
             t.set_tuning(**{k: v})
     t.close()
+
+
+def test_event_batch_tuning_keeps_results(require_gpu):
+    """A handle of several events: every launch setting returns the oracle's
+    per-event id sets (FKDTree_cpu.h:123-169 on each event's own tree)."""
+    from fkdtree_amd import FKDEvents, gen
+    b = gen.workload_events_2048(n_events=24, scale=0.04)
+    boxes = gen.event_self_boxes(b, 0.015)
+    qev = np.repeat(np.arange(b.n_events, dtype=np.uint32), np.diff(b.offsets))
+    print(f"[tuning] events n={b.points.n} q={boxes.q}", flush=True)
+    tc, ti = oracle.build_events(b.points.coords, b.points.ids, b.offsets)
+    ooff, oids = oracle.search_events(tc, ti, b.offsets, qev, boxes.mins, boxes.maxs)
+    want = sorted_sets(ooff, oids)
+    t = FKDEvents(b.points.coords, b.points.ids, b.offsets)
+    t.build()
+    assert t.get_tuning("fill_order") == 0   # the default for several events
+    for kb, fo, fw, cw in ((8, 0, 4096, 0), (8, 1, 256, 1024), (3, 0, 0, 4096), (0, 1, 1024, 256)):
+        t.set_tuning(query_key_bits=kb, fill_order=fo, fill_window=fw, count_window=cw)
+        off, ids = t.search(boxes.mins, boxes.maxs, qev)
+        tag = f"key bits {kb}, fill order {fo}, window {fw}, count window {cw}"
+        assert np.array_equal(off, ooff), f"{tag}: counts differ"
+        assert np.array_equal(sorted_sets(off, ids), want), f"{tag}: ids differ"
+    t.close()
