@@ -184,6 +184,44 @@ def test_oracle_events_fixture():
     assert gen.digest(c, i) == w["tree_sha"]
 
 
+def test_oracle_event_search_vs_single_trees_and_brute_force():
+    """The per-event search (query q walks the tree of event qev[q]) equals
+    the single-tree search on each event's own tree and a brute-force scan of
+    that event's points; queries come in mixed event order, and empty events
+    are included."""
+    sizes = [300, 0, 1, 2000, 17, 0, 900]
+    evs = [adversarial_points("mixed" if k % 2 else "shuffled_ids", s, seed=50 + k) if s else
+           gen.PointSet(np.zeros((3, 0), np.float32), np.zeros(0, np.uint32)) for k, s in enumerate(sizes)]
+    b = gen.concat_events(evs)
+    c, i = oracle.build_events(b.points.coords, b.points.ids, b.offsets)
+    rng = np.random.default_rng(3)
+    nq = 600
+    qev = rng.integers(0, len(sizes), nq).astype(np.uint32)
+    lo = rng.random((3, nq), dtype=np.float32) * np.float32(0.8)
+    hi = (lo + rng.random((3, nq), dtype=np.float32) * np.float32(0.3)).astype(np.float32)
+    off, ids = oracle.search_events(c, i, b.offsets, qev, lo, hi)
+    counts = oracle.search_events_counts(c, i, b.offsets, qev, lo, hi)
+    assert np.array_equal(np.diff(off), counts.astype(np.uint64))
+    srt = oracle.sort_segments(off, ids)
+    for e in range(len(sizes)):
+        a0, a1 = int(b.offsets[e]), int(b.offsets[e + 1])
+        sel = np.flatnonzero(qev == e)
+        if sel.size == 0:
+            continue
+        emin, emax = np.ascontiguousarray(lo[:, sel]), np.ascontiguousarray(hi[:, sel])
+        if a1 == a0:
+            assert np.all(counts[sel] == 0)
+            continue
+        soff, sids = oracle.search(np.ascontiguousarray(c[:, a0:a1]), np.ascontiguousarray(i[a0:a1]), emin, emax)
+        boff, bids = oracle.brute(np.ascontiguousarray(b.points.coords[:, a0:a1]),
+                                  np.ascontiguousarray(b.points.ids[a0:a1]), emin, emax)
+        assert np.array_equal(soff, boff)
+        want = oracle.sort_segments(soff, sids)
+        assert np.array_equal(want, oracle.sort_segments(boff, bids))
+        got = np.concatenate([srt[int(off[q]):int(off[q + 1])] for q in sel])
+        assert np.array_equal(got, want), e
+
+
 # ------------------------------------------------- live reference (here)
 
 @pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
