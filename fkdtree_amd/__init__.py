@@ -352,7 +352,8 @@ class _TreeBase:
         return tot.value, bool(ok.value)
 
     def set_tuning(self, **params) -> None:
-        """Launch parameters (fkd_set_tuning): query_key_bits, fill_order, fill_window, count_window."""
+        """Launch parameters (fkd_set_tuning): query_key_bits, fill_order, fill_window, count_window,
+        key_volume_bits."""
         for k, v in params.items():
             check(lib().fkd_set_tuning(self._h, k.encode(), int(v)))
 

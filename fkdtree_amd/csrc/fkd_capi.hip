@@ -194,6 +194,7 @@ struct fkd_tuning {
   uint32_t fill_order = 1;      // fill pass: heaviest search blocks first (0 for handles of several events)
   uint32_t fill_window = 4096;  // fill pass: boxes reordered by work inside windows of this many (0: off)
   uint32_t count_window = 0;    // count pass: boxes reordered by volume inside windows of this many (0: off)
+  uint32_t key_volume_bits = 0; // box order: bits of a box-volume class below the top Morton regions
 };
 
 struct fkd_tree_s {
@@ -486,15 +487,17 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   // Key: the event (when queries name one of several trees) above the
   // Morton bits; Morton bits are dropped until both fit in 32 bits.
   uint32_t kb = t->tune.query_key_bits;
+  uint32_t vb = kb > 0 ? t->tune.key_volume_bits : 0u;
   uint32_t ev_bits = 0;
-  if (qev && t->n_events > 1 && kb > 0) {
+  if (qev && t->n_events > 1 && kb > 0)
     while (ev_bits < 32u && (1ull << ev_bits) < (uint64_t)t->n_events) ++ev_bits;
-    kb = std::min<uint32_t>(kb, (32u - ev_bits) / (uint32_t)DIM);
-  }
-  const uint32_t ev_shift = kb * (uint32_t)DIM;
+  vb = std::min<uint32_t>(vb, 32u - std::min<uint32_t>(ev_bits, 32u));
+  if (kb > 0) kb = std::min<uint32_t>(kb, (32u - ev_bits - vb) / (uint32_t)DIM);
+  const uint32_t ev_shift = kb * (uint32_t)DIM + vb;
   FKD_TRY(t->q_aos.ensure(n * 32));
-  fkd::query_keys_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), kb, ev_bits ? qev : nullptr,
-                                                ev_shift, keys, perm, t->q_aos.as<float4>());
+  fkd::query_keys_kernel<DIM><<<g, 256, 0, s>>>(mins, maxs, nq, t->bbox.as<uint32_t>(), kb, vb,
+                                                ev_bits ? qev : nullptr, ev_shift, keys, perm,
+                                                t->q_aos.as<float4>());
   FKD_TRY(launch_check("query_keys"));
   fkd::TileMap tm{};
   tm.tile_ev = nullptr;
@@ -1209,6 +1212,9 @@ int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
   } else if (k == "fill_order") {
     if (value > 1) return fail(FKD_ERR_INVALID, "fill_order must be 0 or 1");
     t->tune.fill_order = value;
+  } else if (k == "key_volume_bits") {
+    if (value > 6) return fail(FKD_ERR_INVALID, "key_volume_bits must be <= 6");
+    t->tune.key_volume_bits = value;
   } else if (k == "fill_window" || k == "count_window") {
     if (value != 0 && value != 256 && value != 1024 && value != 4096)
       return fail(FKD_ERR_INVALID, k + " must be 0, 256, 1024 or 4096");
@@ -1226,6 +1232,7 @@ int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
   else if (k == "fill_order") *value = t->tune.fill_order;
   else if (k == "fill_window") *value = t->tune.fill_window;
   else if (k == "count_window") *value = t->tune.count_window;
+  else if (k == "key_volume_bits") *value = t->tune.key_volume_bits;
   else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;
 }

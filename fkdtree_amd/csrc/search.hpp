@@ -367,13 +367,16 @@ __global__ __launch_bounds__(1024) void block_order_kernel(const uint32_t* __res
 // Spatial sort key of every query: Morton interleave of its box centre,
 // quantised to kb bits per dimension inside the tree's bounding box, below
 // the query's event (qev, at bit ev_shift) when the handle holds several
-// trees, so that a wave walks one event's tree.  Also
+// trees, so that a wave walks one event's tree.  With vb > 0 a class of the
+// box volume (relative to the bounding box, 2.5 octaves per class) is
+// inserted below the top two Morton bits per dimension: inside each of those
+// regions boxes of similar size, and so similar work, are neighbours.  Also
 // writes the boxes as 32-byte records (mins, then maxs, each padded to 4), so
 // that the permutation gathers one record per box instead of 2 DIM floats.
 // Any key order gives the same results; this one only groups similar work.
 template <int DIM>
 __global__ void query_keys_kernel(const float* __restrict__ mins, const float* __restrict__ maxs,
-                                  uint32_t nq, const uint32_t* __restrict__ bbox, uint32_t kb,
+                                  uint32_t nq, const uint32_t* __restrict__ bbox, uint32_t kb, uint32_t vb,
                                   const uint32_t* __restrict__ qev, uint32_t ev_shift,
                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                   float4* __restrict__ aos) {
@@ -405,6 +408,22 @@ __global__ void query_keys_kernel(const float* __restrict__ mins, const float* _
     for (uint32_t b = 0; b < kb; ++b)
 #pragma unroll
       for (int m = 0; m < DIM; ++m) key |= ((u[m] >> b) & 1u) << (b * DIM + m);
+    if (vb) {
+      // -log2 of the box volume relative to the bounding box
+      float nl = 0.f;
+#pragma unroll
+      for (int m = 0; m < DIM; ++m) {
+        const float r = (b[4 + m] - b[m]) * sc[m] / cells;
+        nl -= r > 0.f ? __log2f(r) : -1000.f;  // empty, flat or NaN boxes: the last class
+      }
+      const float cf = nl * 0.4f;
+      const uint32_t cmax = (1u << vb) - 1u;
+      const uint32_t cls = cf > 0.f ? (cf < (float)cmax ? (uint32_t)cf : cmax) : 0u;
+      const uint32_t mb = kb * (uint32_t)DIM;
+      const uint32_t lowb = mb > 2u * DIM ? mb - 2u * DIM : 0u;  // bits below the top regions
+      const uint32_t low = key & ((1u << lowb) - 1u);
+      key = ((key >> lowb) << (lowb + vb)) | (cls << lowb) | low;
+    }
     if (qev && ev_shift < 32u) key |= qev[i] << ev_shift;
     keys[i] = key;
     vals[i] = i;

@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
 CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill_window": (0, 256, 1024, 4096),
-              "count_window": (0, 1024, 4096)}
+              "count_window": (0, 1024, 4096), "key_volume_bits": (0, 3, 4)}
 
 
 def _arch() -> str:

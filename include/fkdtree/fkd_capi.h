@@ -128,7 +128,11 @@ int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t str
  *   "count_window"    the count pass reorders the boxes by volume (their
  *                     expected work) inside windows of this many boxes
  *                     (0, 256, 1024 or 4096; default 0 = off: it pays when
- *                     box sizes vary widely). */
+ *                     box sizes vary widely);
+ *   "key_volume_bits" bits of a box-volume class inserted in the box order
+ *                     below the top two Morton bits per dimension (0..6,
+ *                     default 0), so boxes of similar size are neighbours;
+ *                     it pays when box sizes vary widely. */
 int fkd_set_tuning(fkd_tree_t tree, const char* key, uint32_t value);
 int fkd_get_tuning(fkd_tree_t tree, const char* key, uint32_t* value);
 

@@ -47,12 +47,13 @@ def test_tuning_keeps_results(require_gpu):
     t.build()
     dmi = DeviceArray.from_host(boxes.mins)
     dma = DeviceArray.from_host(boxes.maxs)
-    for kb, fo, fw, cw in ((0, 1, 0, 0), (1, 0, 256, 4096), (5, 1, 4096, 1024), (10, 0, 1024, 0), (8, 1, 1024, 256),
-                           (8, 0, 0, 4096)):
-        t.set_tuning(query_key_bits=kb, fill_order=fo, fill_window=fw, count_window=cw)
+    for kb, fo, fw, cw, vb in ((0, 1, 0, 0, 4), (1, 0, 256, 4096, 3), (5, 1, 4096, 1024, 4), (10, 0, 1024, 0, 6),
+                               (8, 1, 1024, 256, 0), (8, 0, 0, 4096, 2), (2, 1, 4096, 0, 4)):
+        t.set_tuning(query_key_bits=kb, fill_order=fo, fill_window=fw, count_window=cw, key_volume_bits=vb)
         assert t.get_tuning("query_key_bits") == kb and t.get_tuning("fill_order") == fo
         assert t.get_tuning("fill_window") == fw and t.get_tuning("count_window") == cw
-        tag = f"key bits {kb}, fill order {fo}, window {fw}, count window {cw}"
+        assert t.get_tuning("key_volume_bits") == vb
+        tag = f"key bits {kb}, fill order {fo}, window {fw}, count window {cw}, volume bits {vb}"
         o, ids = _two_pass(t, dmi, dma, boxes.q)
         assert np.array_equal(o, ooff), f"{tag}: counts differ"
         assert np.array_equal(sorted_sets(o, ids), want), f"{tag}: ids differ"
@@ -68,7 +69,7 @@ def test_tuning_rejects_bad_values(require_gpu):
     p = adversarial_points("mixed", 100)
     t = FKDTree(coords=p.coords, ids=p.ids)
     for k, v in (("pool_batch", 32), ("query_key_bits", 11), ("fill_order", 2), ("fill_window", 512),
-                 ("count_window", 100), ("no_such_key", 1)):
+                 ("count_window", 100), ("key_volume_bits", 7), ("no_such_key", 1)):
         with pytest.raises(RuntimeError):
             t.set_tuning(**{k: v})
     t.close()
@@ -88,10 +89,11 @@ def test_event_batch_tuning_keeps_results(require_gpu):
     t = FKDEvents(b.points.coords, b.points.ids, b.offsets)
     t.build()
     assert t.get_tuning("fill_order") == 0   # the default for several events
-    for kb, fo, fw, cw in ((8, 0, 4096, 0), (8, 1, 256, 1024), (3, 0, 0, 4096), (0, 1, 1024, 256)):
-        t.set_tuning(query_key_bits=kb, fill_order=fo, fill_window=fw, count_window=cw)
+    for kb, fo, fw, cw, vb in ((8, 0, 4096, 0, 4), (8, 1, 256, 1024, 0), (3, 0, 0, 4096, 3), (0, 1, 1024, 256, 4),
+                               (10, 0, 4096, 0, 6)):
+        t.set_tuning(query_key_bits=kb, fill_order=fo, fill_window=fw, count_window=cw, key_volume_bits=vb)
         off, ids = t.search(boxes.mins, boxes.maxs, qev)
-        tag = f"key bits {kb}, fill order {fo}, window {fw}, count window {cw}"
+        tag = f"key bits {kb}, fill order {fo}, window {fw}, count window {cw}, volume bits {vb}"
         assert np.array_equal(off, ooff), f"{tag}: counts differ"
         assert np.array_equal(sorted_sets(off, ids), want), f"{tag}: ids differ"
     t.close()

@@ -43,6 +43,7 @@ def test_candidates_are_valid():
     assert set(tune.CANDIDATES["fill_order"]) <= {0, 1}
     assert set(tune.CANDIDATES["fill_window"]) <= {0, 256, 1024, 4096}
     assert set(tune.CANDIDATES["count_window"]) <= {0, 256, 1024, 4096}
+    assert all(0 <= v <= 6 for v in tune.CANDIDATES["key_volume_bits"])
 
 
 class _FakeTree:
