@@ -95,3 +95,46 @@ def test_events_2048_full(require_gpu):
     assert np.array_equal(soff, ooff), "sampled events: counts differ"
     assert np.array_equal(sorted_sets(soff, sids), sorted_sets(ooff, oids)), "sampled events: ids differ"
     t.close()
+
+
+@pytest.mark.timeout(900)
+def test_output_beyond_2_32_ids(require_gpu):
+    """More than 2^32 ids in one result (52,000 boxes of half-width 0.25 in a
+    1,048,576-point uniform tree, ≈ 4.6e9 hits, 18 GB of output): 64-bit
+    offsets, every box's count against the oracle, and the exact id sets of
+    the boxes around id index 2^32, past it and at both ends."""
+    from fkdtree_amd._lib import check, lib
+    from fkdtree_amd import _sh
+    p = gen.uniform_points(1 << 20, 77)
+    nq = 52_000
+    rng = np.random.default_rng(5)
+    b = gen.boxes_around(rng.random((3, nq), dtype=np.float32), 0.25)
+    print(f"[fullsize] beyond 2^32 n={p.n} q={nq}", flush=True)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    t.build()
+    dmi, dma = DeviceArray.from_host(b.mins), DeviceArray.from_host(b.maxs)
+    doff = DeviceArray(nq + 1, np.uint64)
+    t.search_device(nq, dmi.ptr, dma.ptr, doff.ptr, None, 0, fill=False)
+    o = doff.download()
+    total = int(o[-1])
+    assert total > 1 << 32
+    tc, ti = t.tree_arrays()
+    cnt, _ = oracle.search_counts(tc, ti, b.mins, b.maxs)
+    assert np.array_equal(np.diff(o), cnt.astype(np.uint64)), "counts differ"
+    out = DeviceArray(total, np.uint32)
+    t.search_device(nq, dmi.ptr, dma.ptr, doff.ptr, out.ptr, total, count=False)
+    at = int(np.searchsorted(o, 1 << 32, side="right")) - 1   # the box holding id index 2^32
+    assert o[at] <= 1 << 32 < o[at + 1]
+    sel = sorted({0, 1, at - 1, at, at + 1, nq - 2, nq - 1, *range(at + 500, at + 510)})
+    for q in sel:
+        n = int(o[q + 1] - o[q])
+        got = np.empty(n, np.uint32)
+        if n:
+            check(lib().fkd_memcpy_dtoh(got.ctypes.data, out.ptr + 4 * int(o[q]), 4 * n, _sh(None)))
+        bmin = np.ascontiguousarray(b.mins[:, q:q + 1])
+        bmax = np.ascontiguousarray(b.maxs[:, q:q + 1])
+        boff, bids = oracle.brute(p.coords, p.ids, bmin, bmax)
+        assert np.array_equal(np.sort(got), np.sort(bids)), f"box {q} (offset {int(o[q])})"
+    out.free()
+    t.close()
+    print(f"[fullsize] beyond 2^32 hits={total}", flush=True)
