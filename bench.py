@@ -65,7 +65,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the sharded layout is described in DESIGN.md", file=sys.stderr)
+              "it measures one GPU's share; trees shard by event across GPUs", file=sys.stderr)
         return 2
     import fkdtree_amd as F
     from fkdtree_amd import DeviceArray, Stream, Timer

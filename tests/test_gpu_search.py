@@ -1,4 +1,4 @@
-"""GPU box search parity: every query's sorted id set bit-exact against the
+"""Box search parity on the GPU: every query's sorted id set bit-exact against the
 oracle's level-BFS search (FKDTree_cpu.h:123-169) and a brute-force scan
 (FKDTree.h:54-86), through the C ABI."""
 import numpy as np
