@@ -12,7 +12,7 @@ every hit id (CSR output), all enqueued without a host round trip.
 in their own loop (``builds_per_s``).  Inputs and outputs stay in HBM; timing
 uses HIP events on the stream the kernels run on.
 
-Prints one JSON line.  ``--gpus N`` with N > 1 is refused: this build runs on
+Prints one JSON line.  ``--gpus N`` with N > 1 is refused: the benchmark runs on
 one GPU.
 """
 from __future__ import annotations

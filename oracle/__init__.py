@@ -1,8 +1,8 @@
 """CPU oracle for the FKDTree hot paths -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only as the checker or the timed CPU
-baseline.  The product (``fkdtree_amd``) never calls it.
+``bench.py`` may import this package, and only to check results or as the
+timed CPU baseline.  The product (``fkdtree_amd``) never calls it.
 
 Two native libraries are wrapped:
 

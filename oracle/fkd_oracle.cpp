@@ -1,6 +1,6 @@
 // CPU oracle for the FKDTree hot paths.  TEST INFRASTRUCTURE ONLY: tests,
-// the smoke check and the benchmark's CPU-baseline leg load this library as
-// the checker; the product path never calls it.
+// the smoke check and the benchmark's CPU-baseline leg load this library to
+// check results; the product path never calls it.
 //
 // It restates the reference's CPU tree (FKDTree_cpu.h) for float points of
 // dimension 1..4:
