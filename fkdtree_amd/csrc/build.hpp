@@ -138,6 +138,27 @@ struct LevelArgs {
   uint32_t* ctr;              // dynamic block counter, zero at launch
 };
 
+// Event, event-local tile, first point and size of level tile t; a map
+// without a tile table is one tree of single_n points.
+struct LevelTile {
+  uint32_t e, lt, o, n;
+};
+__device__ inline LevelTile level_tile(const TileMap& tm, uint32_t t) {
+  LevelTile r;
+  if (!tm.tile_ev) {
+    r.e = 0;
+    r.lt = t;
+    r.o = 0;
+    r.n = tm.single_n;
+    return r;
+  }
+  r.e = tm.tile_ev[t];
+  r.lt = t - tm.tb[r.e];
+  r.o = tm.off[r.e];
+  r.n = tm.off[r.e + 1] - r.o;
+  return r;
+}
+
 template <int DIM>
 __device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t node, uint32_t depth, uint32_t p,
                                   const float4* prec) {
@@ -220,10 +241,8 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   const uint32_t bid = dynamic_block(a.ctr, &s_blk);
   const uint32_t arr = M ? bid % M : 0u;
   const uint32_t t = M ? bid / M : bid;
-  const uint32_t e = a.tm.tile_ev[t];
-  const uint32_t lt = t - a.tm.tb[e];
-  const uint32_t o = a.tm.off[e];
-  const uint32_t n = a.tm.off[e + 1] - o;
+  const LevelTile lti = level_tile(a.tm, t);
+  const uint32_t lt = lti.lt, o = lti.o, n = lti.n;
   const Shape sh = shape_of(n);
   const bool split = a.L < fin_depth(sh.D);  // deeper levels: finish_kernel
   const bool root_only = (n == 1 && a.L == 0);
@@ -611,10 +630,8 @@ __global__ __launch_bounds__(kThreads) void boxes_around_kernel(const float* __r
                                                                 TileMap tm, float* __restrict__ mins,
                                                                 float* __restrict__ maxs, uint32_t* __restrict__ qev) {
   const uint32_t t = blockIdx.x;
-  const uint32_t e = tm.tile_ev[t];
-  const uint32_t lt = t - tm.tb[e];
-  const uint32_t o = tm.off[e];
-  const uint32_t n = tm.off[e + 1] - o;
+  const LevelTile lti = level_tile(tm, t);
+  const uint32_t e = lti.e, lt = lti.lt, o = lti.o, n = lti.n;
   for (uint32_t j = lt * (uint32_t)kLevelTile + threadIdx.x; j < min(n, (lt + 1) * (uint32_t)kLevelTile);
        j += kThreads) {
     const uint64_t i = (uint64_t)o + j;
@@ -650,10 +667,8 @@ __global__ void records_kernel(const float* __restrict__ X, const uint32_t* __re
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a, const float4* __restrict__ rec) {
   const uint32_t t = blockIdx.x;
-  const uint32_t e = a.tm.tile_ev[t];
-  const uint32_t lt = t - a.tm.tb[e];
-  const uint32_t o = a.tm.off[e];
-  const uint32_t n = a.tm.off[e + 1] - o;
+  const LevelTile lti = level_tile(a.tm, t);
+  const uint32_t lt = lti.lt, o = lti.o, n = lti.n;
   for (uint32_t v = lt * (uint32_t)kLevelTile + threadIdx.x; v < min(n, (lt + 1) * (uint32_t)kLevelTile);
        v += kThreads) {
     uint32_t p = o + v;  // S0 == null: the points already are the heap-ordered tree

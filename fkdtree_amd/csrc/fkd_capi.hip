@@ -297,6 +297,17 @@ int run_radix_pass(fkd_tree_s* t, ZeroArena& za, uint32_t M, const uint32_t* kin
                     vout, stride, ranks, shift, write_keys, write_ranks, s);
 }
 
+// Level tiles of the handle; one tree needs no tile table (its kernels take
+// event 0, offset 0 and the point count from the map itself).
+fkd::TileMap level_map(fkd_tree_s* t) {
+  fkd::TileMap m = t->level_tiles.map(t->off.as<uint32_t>());
+  if (t->n_events == 1) {
+    m.tile_ev = nullptr;
+    m.single_n = t->P;
+  }
+  return m;
+}
+
 template <int DIM>
 int build_impl(fkd_tree_s* t, hipStream_t s) {
   const uint32_t P = t->P;
@@ -306,7 +317,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   }
   FKD_TRY(ensure_build_scratch(t));
   const uint32_t levels = t->global_levels;
-  const fkd::TileMap tm = t->level_tiles.map(t->off.as<uint32_t>());
+  const fkd::TileMap tm = level_map(t);
   constexpr uint32_t M = DIM - 1;
   const uint32_t RT = t->radix_tiles.T;
   ZeroArena za;
@@ -815,7 +826,7 @@ int adopt_tree_impl(fkd_tree_s* t, hipStream_t s) {
       FKD_TRY(launch_check("records"));
     }
     fkd::LevelArgs<DIM> a{};
-    a.tm = t->level_tiles.map(t->off.as<uint32_t>());
+    a.tm = level_map(t);
     a.S0 = nullptr;
     a.F = nullptr;
     a.X = t->X.as<float>();
@@ -1095,7 +1106,7 @@ int fkd_boxes_around(fkd_tree_t t, float half_width, float* dev_mins, float* dev
   if (!(half_width >= 0.f)) return fail(FKD_ERR_INVALID, "half_width must be >= 0");
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (!t->P) return FKD_OK;
-  const fkd::TileMap tm = t->level_tiles.map(t->off.as<uint32_t>());
+  const fkd::TileMap tm = level_map(t);
   return dispatch_dim(t->dim, [&](auto D) {
     fkd::boxes_around_kernel<decltype(D)::value><<<tm.T, fkd::kThreads, 0, s>>>(
         t->X.as<float>(), t->P, half_width, tm, dev_mins, dev_maxs, dev_query_events);
