@@ -138,3 +138,26 @@ def test_output_beyond_2_32_ids(require_gpu):
     out.free()
     t.close()
     print(f"[fullsize] beyond 2^32 hits={total}", flush=True)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("dim", [1, 2, 4])
+def test_other_dims_1m(require_gpu, dim):
+    """1,048,576 points of dimension 1, 2 or 4 (mixed distribution): the tree
+    arrays bit for bit and the id sets of 200,000 self-centred boxes exactly
+    against the oracle (FKDTree_cpu.h:123-169, 221-296)."""
+    p = gen.mixed_points(1 << 20, 900 + dim, n_blobs=1000, dim=dim)
+    hw = {1: 2e-6, 2: 2e-3, 4: 0.03}[dim]
+    b = gen.boxes_around(np.ascontiguousarray(p.coords[:, ::5][:, :200_000]), hw)
+    print(f"[fullsize] dim={dim} n={p.n} q={b.q}", flush=True)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    t.build()
+    gc, gi = t.tree_arrays()
+    oc, oi = oracle.build(p.coords, p.ids)
+    assert np.array_equal(gi, oi) and np.array_equal(bits(gc), bits(oc)), f"dim {dim}: tree differs"
+    goff, gids = t.search_in_the_box_multiple_csr(b.mins, b.maxs)
+    t.close()
+    ooff, oids = oracle.search(oc, oi, b.mins, b.maxs)
+    assert np.array_equal(goff, ooff), f"dim {dim}: counts differ"
+    assert np.array_equal(sorted_sets(goff, gids), sorted_sets(ooff, oids)), f"dim {dim}: ids differ"
+    print(f"[fullsize] dim={dim} hits={int(goff[-1])}", flush=True)
