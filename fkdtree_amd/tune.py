@@ -30,11 +30,19 @@ CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill
               "count_window": (0, 1024, 4096), "key_volume_bits": (0, 3, 4)}
 
 
-def _arch() -> str:
-    from . import device_info
+_ARCH: Optional[str] = None
+_TABLES: Dict[str, tuple] = {}  # path -> (modification time, table)
 
-    name = device_info()["name"]
-    return name.split("(")[-1].split(":")[0].rstrip(")") or "unknown"
+
+def _arch() -> str:
+    """GPU architecture name (e.g. gfx950), asked once per process."""
+    global _ARCH
+    if _ARCH is None:
+        from . import device_info
+
+        name = device_info()["name"]
+        _ARCH = name.split("(")[-1].split(":")[0].rstrip(")") or "unknown"
+    return _ARCH
 
 
 def _key(dim: int, n: int, events: bool = False) -> str:
@@ -44,10 +52,18 @@ def _key(dim: int, n: int, events: bool = False) -> str:
 
 
 def load_table(path: str = TABLE) -> Dict:
+    """The tuning table at `path` ({} if absent), parsed once per process and
+    again only when the file changes (every new handle looks it up)."""
     if not os.path.exists(path):
         return {}
+    mtime = os.path.getmtime(path)
+    hit = _TABLES.get(path)
+    if hit is not None and hit[0] == mtime:
+        return hit[1]
     with open(path) as f:
-        return json.load(f)
+        table = json.load(f)
+    _TABLES[path] = (mtime, table)
+    return table
 
 
 def lookup(arch: str, dim: int, n: int, table: Optional[Dict] = None,

@@ -65,3 +65,17 @@ def test_apply_skips_unknown_keys(tmp_path, monkeypatch):
     t = _FakeTree()
     assert tune.apply(t, arch="gfx950") == {"query_key_bits": 7}
     assert t.set == {"query_key_bits": 7}
+
+
+def test_table_is_reread_when_the_file_changes(tmp_path):
+    """The table is parsed once per process but a rewritten file (a new
+    autotuning run) is picked up."""
+    import os
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps({"gfx950": {"dim3_log2n20": {"params": {"query_key_bits": 5}}}}))
+    assert tune.lookup("gfx950", 3, 1 << 20, tune.load_table(str(p))) == {"query_key_bits": 5}
+    assert tune.load_table(str(p)) is tune.load_table(str(p))
+    p.write_text(json.dumps({"gfx950": {"dim3_log2n20": {"params": {"query_key_bits": 7}}}}))
+    st = os.stat(p)
+    os.utime(p, ns=(st.st_atime_ns, st.st_mtime_ns + 1_000_000))
+    assert tune.lookup("gfx950", 3, 1 << 20, tune.load_table(str(p))) == {"query_key_bits": 7}
