@@ -102,9 +102,24 @@ __device__ inline uint64_t poll(uint64_t* p) {
   return v;
 }
 
-// Dynamic block index (in launch order of acquisition), broadcast to the block.
+// Dynamic block index, broadcast to the block.  A block of dispatch class
+// c = blockIdx.x % 8 (the XCD it runs on) takes the next rank r of its class
+// and becomes block 8 r + c, so the indices are exactly 0 .. gridDim.x - 1.
+// One counter per class, kCtrStride words apart: a single counter for the
+// whole grid serialises every workgroup of a launch on one address (about
+// 17 ns each, 10 us of a 1,024-block launch's start).  Look-back waits only
+// on lower indices and stays deadlock-free under in-order dispatch, per
+// queue or per XCD: every block dispatched before a class's next block holds
+// an index below the one that block will take.  ctr needs kCtrWords zero
+// 64-bit words.
+constexpr uint32_t kCtrClasses = 8;
+constexpr uint32_t kCtrStride = 512;  // uint32 words between two class counters
+constexpr uint32_t kCtrWords = kCtrClasses * kCtrStride / 2;
 __device__ inline uint32_t dynamic_block(uint32_t* ctr, uint32_t* lds) {
-  if (threadIdx.x == 0) *lds = atomicAdd(ctr, 1u);
+  if (threadIdx.x == 0) {
+    const uint32_t c = blockIdx.x % kCtrClasses;
+    *lds = atomicAdd(ctr + c * kCtrStride, 1u) * kCtrClasses + c;
+  }
   __syncthreads();
   const uint32_t b = *lds;
   __syncthreads();

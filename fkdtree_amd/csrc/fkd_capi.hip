@@ -256,7 +256,7 @@ struct ZeroArena {
 int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za);
 
 // Words one radix pass takes from the arena: scan descriptors + counter.
-uint64_t radix_pass_words(uint32_t M, uint32_t T) { return fkd::scan_tiles((uint64_t)M * 256u * T) + 1; }
+uint64_t radix_pass_words(uint32_t M, uint32_t T) { return fkd::scan_tiles((uint64_t)M * 256u * T) + fkd::kCtrWords; }
 
 // One stable LSD digit pass over M key arrays (count, scan, scatter); desc
 // (scan_tiles of the digit counts) and ctr must be zero.
@@ -323,7 +323,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   ZeroArena za;
   FKD_TRY(zero_arena(t,
                      (t->ids_in_order ? 0 : 4 * radix_pass_words(1, RT)) + 4 * radix_pass_words(DIM, RT) +
-                         (uint64_t)levels * ((uint64_t)M * tm.T + 1),
+                         (uint64_t)levels * ((uint64_t)M * tm.T + fkd::kCtrWords),
                      s, za));
   const uint64_t S = P;  // stride of one array
   uint32_t* A = t->A.as<uint32_t>();  // A[m][2][P]
@@ -404,7 +404,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     a.Tc = t->Tc.as<float>();
     a.Tid = t->Tid.as<uint32_t>();
     a.recs = t->recs.as<float>();
-    a.desc = za.take((uint64_t)M * tm.T + 1);
+    a.desc = za.take((uint64_t)M * tm.T + fkd::kCtrWords);
     a.ctr = reinterpret_cast<uint32_t*>(a.desc + (uint64_t)M * tm.T);
     const uint32_t blocks = (M ? M : 1u) * tm.T;
     fkd::level_kernel<DIM><<<blocks, fkd::kThreads, 0, s>>>(a);
@@ -517,7 +517,7 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   const int passes = (int)(ev_shift + ev_bits + 7) / 8;
   // one clear for the digit passes and the scan of the count pass
   ZeroArena za;
-  FKD_TRY(zero_arena(t, passes * radix_pass_words(1, T) + fkd::scan_tiles(nq) + 1, s, za));
+  FKD_TRY(zero_arena(t, passes * radix_pass_words(1, T) + fkd::scan_tiles(nq) + fkd::kCtrWords, s, za));
   int c = 0;
   for (int pass = 0; pass < passes; ++pass) {
     const bool lastp = pass == passes - 1;
@@ -530,7 +530,7 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
     c = 1 - c;
   }
   t->q_perm_half = (uint32_t)c;
-  t->q_scan_desc = za.take(fkd::scan_tiles(nq) + 1);
+  t->q_scan_desc = za.take(fkd::scan_tiles(nq) + fkd::kCtrWords);
   float* smins = t->q_sbox.as<float>();
   float* smaxs = smins + (uint64_t)DIM * n;
   fkd::permute_boxes_kernel<DIM><<<g, 256, 0, s>>>(t->q_aos.as<float4>(), perm + (uint64_t)c * n, nq, smins,
@@ -587,8 +587,8 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
   uint64_t* desc = t->q_scan_desc;
   if (nq == 0) {
     ZeroArena za;
-    FKD_TRY(zero_arena(t, st + 1, s, za));
-    desc = za.take(st + 1);
+    FKD_TRY(zero_arena(t, st + fkd::kCtrWords, s, za));
+    desc = za.take(st + fkd::kCtrWords);
   }
   fkd::scan_exclusive_kernel<uint64_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
       t->q_counts.as<uint32_t>(), qoff, nq, desc, reinterpret_cast<uint32_t*>(desc + st));
