@@ -260,6 +260,27 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   // the loads of all items can be in flight together).
   uint32_t cls[kLevelItems], dest0[kLevelItems], rk[kLevelItems][DIM], rm[kLevelItems], fr[kLevelItems];
   uint32_t st0[kLevelItems], mp[kLevelItems], hn[kLevelItems];
+  // The thread's 8 ranks of a plane as two 16-byte loads where the planes
+  // are 16-byte aligned (plane stride and event offset multiples of 4) and
+  // the tile holds all 8 items; one 4-byte load per item otherwise.  The
+  // vector loads also read separator positions, whose ranks go unused.
+  static_assert(kLevelItems == 8, "two 16-byte loads per plane");
+  const bool vec = M != 0 && first + (uint32_t)kLevelItems <= n && ((P | (uint64_t)o) & 3u) == 0u;
+  if (vec) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const uint4* q = reinterpret_cast<const uint4*>(S + (uint64_t)k * P + o + first);
+      const uint4 x0 = q[0], x1 = q[1];
+      rk[0][k] = x0.x;
+      rk[1][k] = x0.y;
+      rk[2][k] = x0.z;
+      rk[3][k] = x0.w;
+      rk[4][k] = x1.x;
+      rk[5][k] = x1.y;
+      rk[6][k] = x1.z;
+      rk[7][k] = x1.w;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < kLevelItems; ++i) {
     const uint32_t jl = first + i;
@@ -269,8 +290,10 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
     hn[i] = 0;
     rm[i] = 0;
     fr[i] = 0;
+    if (!vec) {
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) rk[i][k] = 0;
+      for (int k = 0; k < DIM; ++k) rk[i][k] = 0;
+    }
     if (jl < n) {
       const uint32_t ts = subtree_at(sh, a.L, jl);
       uint32_t start, size;
@@ -283,8 +306,10 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
         cls[i] = 8u;  // in an interval
         if (M) {
           rm[i] = Ad_d[o + mpos];
+          if (!vec) {
 #pragma unroll
-          for (int k = 0; k < DIM; ++k) rk[i][k] = S[(uint64_t)k * P + o + jl];
+            for (int k = 0; k < DIM; ++k) rk[i][k] = S[(uint64_t)k * P + o + jl];
+          }
         }
         if (arr == 0 && jl == mpos) fr[i] = a.Ad[o + jl];  // plane 0 of array d
       }
@@ -348,9 +373,10 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   excl = seg_op(wcarry, excl);
 
   if (wave == 0) {
-    // Windowed look-back: lane l polls the descriptor of tile lt-1-base-l of
+    // Windowed look-back: lane l reads the descriptor of tile lt-1-base-l of
     // the same array; the nearest tile that has its prefix or an interval
-    // start ends the walk.  Values combine far-to-near with seg_op.
+    // start ends the walk, and only it and the tiles nearer than it are
+    // waited for.  Values combine far-to-near with seg_op.
     SegCount carry{0u, 0u, 0u};
     if (lt == 0) {
       if (lane == 0) publish(&a.desc[bid], kStPrefix | seg_pack(tile_agg));
@@ -358,11 +384,11 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
       if (lane == 0) publish(&a.desc[bid], kStAggregate | seg_pack(tile_agg));
       for (uint32_t base = 0;; base += kWave) {
         const int64_t j = (int64_t)lt - 1 - (int64_t)base - (int64_t)lane;
-        const uint64_t w = j >= 0 ? poll(&a.desc[bid - (base + lane + 1u) * M]) : kStPrefix;
+        uint32_t first;
+        const uint64_t w = look_window(
+            j >= 0 ? &a.desc[bid - (base + lane + 1u) * M] : nullptr,
+            [](uint64_t d) { return (d & kStMask) == kStPrefix || seg_unpack(d).f; }, &first);
         SegCount v = seg_unpack(w);
-        const bool stop = (w & kStMask) == kStPrefix || v.f;
-        const uint64_t sm = __ballot(stop);
-        const uint32_t first = sm ? (uint32_t)__builtin_ctzll(sm) : (uint32_t)kWave;
         if (lane > first) v = SegCount{0u, 0u, 0u};
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
@@ -373,7 +399,7 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
           if (lane + o < (uint32_t)kWave) v = seg_op(u, v);
         }
         carry = seg_op(v, carry);
-        if (sm) break;
+        if (first < (uint32_t)kWave) break;
       }
       if (lane == 0) publish(&a.desc[bid], kStPrefix | seg_pack(seg_op(carry, tile_agg)));
     }

@@ -94,25 +94,44 @@ constexpr uint64_t kStMask = 3ull << 62;
 __device__ inline void publish(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline uint64_t poll(uint64_t* p) {
-  uint64_t v;
-  while (((v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & kStMask) ==
-         kStInvalid)
+// One window of a look-back (wave-wide): lane l reads descriptor p[l] (null:
+// a virtual prefix 0 before the first tile) until the nearest stopper (a
+// descriptor for which is_stop holds) and every tile nearer than it have
+// published; without a stopper, until all 64 have.  Waiting for all 64
+// would hold every tile until the slowest of its 64 predecessors publishes.
+// Returns the lane's descriptor and in *first the lane of the nearest
+// stopper (kWave if none); lanes beyond it may be unpublished.  Only lanes
+// still unpublished read again (the descriptors live past the caches, and
+// re-reading all 64 per spin slows every waiting tile).
+template <class Stop>
+__device__ inline uint64_t look_window(uint64_t* p, Stop is_stop, uint32_t* first) {
+  uint64_t w = p ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kStPrefix;
+  while (true) {
+    const bool valid = (w & kStMask) != kStInvalid;
+    const uint64_t sm = __ballot(valid && is_stop(w));
+    const uint32_t f = sm ? (uint32_t)__builtin_ctzll(sm) : (uint32_t)kWave;
+    const uint64_t need = f >= (uint32_t)kWave - 1u ? ~0ull : (2ull << f) - 1ull;
+    if ((__ballot(!valid) & need) == 0ull) {
+      *first = f;
+      return w;
+    }
     __builtin_amdgcn_s_sleep(1);
-  return v;
+    if (!valid) w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Dynamic block index, broadcast to the block.  A block of dispatch class
-// c = blockIdx.x % 8 (the XCD it runs on) takes the next rank r of its class
-// and becomes block 8 r + c, so the indices are exactly 0 .. gridDim.x - 1.
-// One counter per class, kCtrStride words apart: a single counter for the
-// whole grid serialises every workgroup of a launch on one address (about
-// 17 ns each, 10 us of a 1,024-block launch's start).  Look-back waits only
+// c = blockIdx.x % 32 (four classes per XCD) takes the next rank r of its
+// class and becomes block 32 r + c, so the indices are exactly
+// 0 .. gridDim.x - 1.  One counter per class, kCtrStride words apart: a
+// single counter for the whole grid serialises every workgroup of a launch
+// on one address (about 17 ns each, 10 us of a 1,024-block launch's start;
+// 8 classes still leave a 9 us tail, 32 none).  Look-back waits only
 // on lower indices and stays deadlock-free under in-order dispatch, per
 // queue or per XCD: every block dispatched before a class's next block holds
 // an index below the one that block will take.  ctr needs kCtrWords zero
 // 64-bit words.
-constexpr uint32_t kCtrClasses = 8;
+constexpr uint32_t kCtrClasses = 32;
 constexpr uint32_t kCtrStride = 512;  // uint32 words between two class counters
 constexpr uint32_t kCtrWords = kCtrClasses * kCtrStride / 2;
 __device__ inline uint32_t dynamic_block(uint32_t* ctr, uint32_t* lds) {

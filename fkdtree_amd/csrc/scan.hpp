@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kThreads) void scan_exclusive_kernel(const uint32_t
     total += s_wave[w];
   }
   if (wave == 0) {
-    // Look back over a window of 64 predecessors per step: lane l polls tile
+    // Look back over a window of 64 predecessors per step: lane l reads tile
     // tile-1-base-l; the sum stops at the nearest tile that has its prefix.
     uint64_t carry = 0;
     if (tile == 0) {
@@ -53,15 +53,14 @@ __global__ __launch_bounds__(kThreads) void scan_exclusive_kernel(const uint32_t
       if (lane_id() == 0) publish(&desc[tile], kStAggregate | total);
       for (int64_t base = 0;; base += kWave) {
         const int64_t j = (int64_t)tile - 1 - base - (int64_t)lane_id();
-        const uint64_t d = j >= 0 ? poll(&desc[j]) : kStPrefix;
-        const bool stop = (d & kStMask) == kStPrefix;
-        const uint64_t sm = __ballot(stop);
-        const uint32_t first = sm ? (uint32_t)__builtin_ctzll(sm) : (uint32_t)kWave;
+        uint32_t first;
+        const uint64_t d = look_window(j >= 0 ? &desc[j] : nullptr,
+                                       [](uint64_t x) { return (x & kStMask) == kStPrefix; }, &first);
         uint64_t v = lane_id() <= first ? (d & ~kStMask) : 0ull;
 #pragma unroll
         for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
         carry += v;
-        if (sm) break;
+        if (first < (uint32_t)kWave) break;
       }
       if (lane_id() == 0) publish(&desc[tile], kStPrefix | (carry + total));
     }
