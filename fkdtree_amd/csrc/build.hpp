@@ -41,11 +41,14 @@ __global__ void aos_to_soa_kernel(const uint32_t* __restrict__ aos, float* __res
 }
 
 // Keys and values for the coordinate sorts.  order == nullptr means point
-// order already is (event, id) order.  Sets bit 0 of *err on a NaN.
+// order already is (event, id) order.  Sets bit 0 of *err on a NaN.  Also
+// sets the bounding box that bbox_kernel narrows to its starting values
+// (min keys all ones, max keys zero).
 template <int DIM>
 __global__ void keys_kernel(const float* __restrict__ X, const uint32_t* __restrict__ order,
                             uint32_t P, uint32_t* __restrict__ K, uint32_t* __restrict__ V,
-                            uint64_t stride, uint32_t* err) {
+                            uint64_t stride, uint32_t* err, uint32_t* bbox) {
+  if (blockIdx.x == 0 && threadIdx.x < 2u * DIM) bbox[threadIdx.x] = threadIdx.x < (uint32_t)DIM ? 0xffffffffu : 0u;
   bool nan = false;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
     const uint32_t p = order ? order[i] : i;
@@ -429,29 +432,61 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   }
 }
 
+// Segmented counts of N arrays at once: one interval-start flag, and the
+// left / right counts of every array.
+template <int N>
+struct SegCountN {
+  uint32_t f, l[N], r[N];
+};
+template <int N>
+__device__ inline SegCountN<N> seg_zero() {
+  SegCountN<N> z;
+  z.f = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) z.l[k] = z.r[k] = 0;
+  return z;
+}
+template <int N>
+__device__ inline SegCountN<N> seg_op(const SegCountN<N>& a, const SegCountN<N>& b) {
+  SegCountN<N> c;
+  c.f = a.f | b.f;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    c.l[k] = b.f ? b.l[k] : a.l[k] + b.l[k];
+    c.r[k] = b.f ? b.r[k] : a.r[k] + b.r[k];
+  }
+  return c;
+}
+template <int N>
+__device__ inline SegCountN<N> seg_shfl_up(const SegCountN<N>& v, int ofs) {
+  SegCountN<N> u;
+  u.f = __shfl_up(v.f, ofs);
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    u.l[k] = __shfl_up(v.l[k], ofs);
+    u.r[k] = __shfl_up(v.r[k], ofs);
+  }
+  return u;
+}
+
 // Block-wide exclusive segmented scan of the threads' aggregates (every
 // thread calls it); returns the thread's exclusive prefix.  s_wave holds
 // kThreads / kWave entries; the caller syncs before reusing it.
-__device__ inline SegCount block_seg_excl(SegCount agg, SegCount* s_wave) {
-  SegCount incl = agg;
+template <int N>
+__device__ inline SegCountN<N> block_seg_excl(SegCountN<N> agg, SegCountN<N>* s_wave) {
+  SegCountN<N> incl = agg;
   const uint32_t lane = lane_id();
 #pragma unroll
   for (int ofs = 1; ofs < kWave; ofs <<= 1) {
-    SegCount u;
-    u.f = __shfl_up(incl.f, ofs);
-    u.l = __shfl_up(incl.l, ofs);
-    u.r = __shfl_up(incl.r, ofs);
+    const SegCountN<N> u = seg_shfl_up(incl, ofs);
     if ((int)lane >= ofs) incl = seg_op(u, incl);
   }
-  SegCount excl;
-  excl.f = __shfl_up(incl.f, 1);
-  excl.l = __shfl_up(incl.l, 1);
-  excl.r = __shfl_up(incl.r, 1);
-  if (lane == 0) excl = SegCount{0u, 0u, 0u};
+  SegCountN<N> excl = seg_shfl_up(incl, 1);
+  if (lane == 0) excl = seg_zero<N>();
   const int wave = threadIdx.x / kWave;
   if (lane == kWave - 1) s_wave[wave] = incl;
   __syncthreads();
-  SegCount wcarry{0u, 0u, 0u};
+  SegCountN<N> wcarry = seg_zero<N>();
 #pragma unroll
   for (int w = 0; w < kThreads / kWave; ++w)
     if (w < wave) wcarry = seg_op(wcarry, s_wave[w]);
@@ -483,8 +518,9 @@ struct FinArgs {
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
   __shared__ uint32_t s_rank[DIM][kFinCap];
+  constexpr int MA = DIM > 1 ? DIM - 1 : 1;  // arrays moved per depth
   __shared__ uint16_t s_arr[DIM][2][kFinCap + 1];
-  __shared__ SegCount s_wave[kThreads / kWave];
+  __shared__ SegCountN<MA> s_wave[kThreads / kWave];
   const uint32_t e = a.fm.tile_ev[blockIdx.x];
   const uint32_t t0 = blockIdx.x - a.fm.tb[e];
   const uint32_t o = a.fm.off[e];
@@ -520,13 +556,12 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
       s_arr[m][0][j] = (uint16_t)(lo < size0 ? lo : size0 - 1u);  // always found; kept in range
     }
   __syncthreads();
-  uint32_t cur[DIM];
-#pragma unroll
-  for (int m = 0; m < DIM; ++m) cur[m] = 0;
+  uint32_t curbits = 0;  // bit m: the buffer of s_arr[m] that holds array m
   const uint32_t first = threadIdx.x * (uint32_t)kFinItems;
   for (uint32_t L = L0; L < sh.D; ++L) {
     const uint32_t d = L % (uint32_t)DIM;
     const bool last = L + 1u == sh.D;
+    const uint16_t* med = s_arr[d][(curbits >> d) & 1u];
     uint32_t st[kFinItems], mp[kFinItems];
     bool inside[kFinItems];
 #pragma unroll
@@ -544,65 +579,82 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
         // node of depth L, or (last split) a leaf of depth D
         if (inside[i] && (jl == mp[i] || last)) {
           const uint32_t h = jl == mp[i] ? (1u << L) - 1u + ts : (1u << sh.D) - 1u + subtree_at(sh, sh.D, jl);
-          a.F[o + h] = s_rank[0][s_arr[d][cur[d]][j]];
+          a.F[o + h] = s_rank[0][med[j]];
         }
       }
     }
-    if (last) break;
+    if (last || DIM == 1) continue;
+    // The DIM-1 arrays other than d (array (d + 1 + k) % DIM for slot k) are
+    // partitioned together: one segmented scan of all their counts.
+    uint32_t rmed[kFinItems];  // rank in dimension d of the item's interval median
 #pragma unroll
-    for (int m = 0; m < DIM; ++m) {
-      if ((uint32_t)m == d) continue;
-      const uint16_t* src = s_arr[m][cur[m]];
-      uint16_t* dst = s_arr[m][cur[m] ^ 1u];
-      uint32_t cls[kFinItems], dest0[kFinItems], who[kFinItems];
-      SegCount agg{0u, 0u, 0u};
+    for (int i = 0; i < kFinItems; ++i) rmed[i] = inside[i] ? s_rank[d][med[mp[i] - start0]] : 0u;
+    uint32_t cls[MA][kFinItems], dest0[MA][kFinItems], who[MA][kFinItems];
+    SegCountN<MA> agg = seg_zero<MA>();
 #pragma unroll
-      for (int i = 0; i < kFinItems; ++i) {
-        const uint32_t j = first + i;
-        cls[i] = kNone;
-        dest0[i] = 0;
-        who[i] = 0;
+    for (int i = 0; i < kFinItems; ++i) {
+      const uint32_t j = first + i;
+      const uint32_t jl = start0 + j;
+      if (inside[i] && jl == st[i]) {
+        agg = seg_zero<MA>();
+        agg.f = 1u;
+      }
+#pragma unroll
+      for (int k = 0; k < MA; ++k) {
+        cls[k][i] = kNone;
+        dest0[k][i] = 0;
+        who[k][i] = 0;
         if (!inside[i]) continue;
-        const uint32_t jl = start0 + j;
-        who[i] = src[j];
-        const uint32_t r = s_rank[d][who[i]];
-        const uint32_t rm = s_rank[d][s_arr[d][cur[d]][mp[i] - start0]];
-        if (jl == st[i]) agg = SegCount{1u, 0u, 0u};
+        uint32_t m = d + 1u + (uint32_t)k;
+        m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
+        who[k][i] = s_arr[m][(curbits >> m) & 1u][j];
+        const uint32_t r = s_rank[d][who[k][i]];
         uint32_t c;
-        if (r < rm) {
+        if (r < rmed[i]) {
           c = kLeft;
-          dest0[i] = st[i];
-          agg.l++;
-        } else if (r > rm) {
+          dest0[k][i] = st[i];
+          agg.l[k]++;
+        } else if (r > rmed[i]) {
           c = kRight;
-          dest0[i] = mp[i] + 1u;
-          agg.r++;
+          dest0[k][i] = mp[i] + 1u;
+          agg.r[k]++;
         } else {
           c = kMed;
-          dest0[i] = mp[i];
+          dest0[k][i] = mp[i];
         }
         if (jl == st[i]) c |= 4u;
-        cls[i] = c;
+        cls[k][i] = c;
       }
-      const SegCount run0 = block_seg_excl(agg, s_wave);
-      uint32_t nl = run0.l, nr = run0.r;
+    }
+    const SegCountN<MA> run0 = block_seg_excl<MA>(agg, s_wave);
+#pragma unroll
+    for (int k = 0; k < MA; ++k) {
+      uint32_t m = d + 1u + (uint32_t)k;
+      m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
+      uint16_t* dst = s_arr[m][((curbits >> m) & 1u) ^ 1u];
+      uint32_t nl = run0.l[k], nr = run0.r[k];
 #pragma unroll
       for (int i = 0; i < kFinItems; ++i) {
-        const uint32_t c = cls[i];
+        const uint32_t c = cls[k][i];
         if (c & 4u) {
           nl = 0;
           nr = 0;
         }
-        uint32_t dest = dest0[i];
+        uint32_t dest = dest0[k][i];
         switch (c & 3u) {
           case kLeft: dest += nl; ++nl; break;
           case kRight: dest += nr; ++nr; break;
           default: break;
         }
-        if ((c & 3u) != kNone) dst[dest - start0] = (uint16_t)who[i];
+        if ((c & 3u) != kNone) dst[dest - start0] = (uint16_t)who[k][i];
       }
-      __syncthreads();
-      cur[m] ^= 1u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MA; ++k) {
+      uint32_t m = d + 1u + (uint32_t)k;
+      m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
+      curbits ^= 1u << m;
     }
   }
 }

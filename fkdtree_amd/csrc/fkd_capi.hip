@@ -350,7 +350,8 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     order = R;  // R holds the (event, id) order until the last key pass writes the ranks
   }
   // 2. Keys of every coordinate, in tie order.
-  fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>());
+  fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>(),
+                                             t->bbox.as<uint32_t>());
   FKD_TRY(launch_check("keys"));
   // 3. Stable LSD sort of every coordinate inside every event; the last pass
   //    stores each point's rank R[p][m] (the values end in buffer 0).
@@ -363,9 +364,8 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
       c = 1 - c;
     }
   }
-  // Bounding box (used to order search queries) from the sorted arrays.
-  FKD_TRY(fill_u32(t->bbox.as<uint32_t>(), 0xffffffffu, DIM, s));
-  FKD_TRY(fill_u32(t->bbox.as<uint32_t>() + DIM, 0u, DIM, s));
+  // Bounding box (used to order search queries) from the sorted arrays;
+  // keys_kernel set its starting values.
   fkd::bbox_kernel<DIM><<<grid_for(t->n_events), 256, 0, s>>>(X, Abuf(0, 0), 2 * S, t->off.as<uint32_t>(),
                                                                t->n_events, P, t->bbox.as<uint32_t>());
   FKD_TRY(launch_check("bbox"));

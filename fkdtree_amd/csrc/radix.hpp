@@ -72,6 +72,10 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
 // all arrays (array m's entries start at m * P).  Keys are written only when
 // WRITE_KEYS; when WRITE_RANKS, ranks[4 val + m] = final slot of val (the
 // ranks of a point in all dimensions form one 16-byte record).
+// The tile is first ordered by digit in LDS (stable: wave by wave, item by
+// item, lane by lane), then written out in that order, so consecutive threads
+// write consecutive slots of one digit's run instead of up to 64 buckets per
+// store.
 template <bool WRITE_KEYS, bool WRITE_RANKS>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, uint32_t* __restrict__ keys_out,
@@ -80,6 +84,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ offsets) {
   __shared__ uint32_t cnt[4][256];
   __shared__ uint32_t goff[256];
+  __shared__ uint32_t tstart[256];
+  __shared__ uint32_t s_wsum[kThreads / kWave];
+  __shared__ uint32_t s_key[kRadixTile];
+  __shared__ uint32_t s_val[kRadixTile];
   const uint32_t t = blockIdx.x, m = blockIdx.y;
   const int wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
@@ -122,25 +130,47 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   }
   __syncthreads();
   {
+    // tile-local start of every (wave, digit): digits in order, waves in order
     const uint32_t d = threadIdx.x;
-    const uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d];
-    cnt[0][d] = goff[d];
-    cnt[1][d] = goff[d] + c0;
-    cnt[2][d] = goff[d] + c0 + c1;
-    cnt[3][d] = goff[d] + c0 + c1 + c2;
+    const uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d], c3 = cnt[3][d];
+    const uint32_t tot = c0 + c1 + c2 + c3;
+    const uint32_t incl = wave_incl_sum(tot);
+    if (lane == kWave - 1) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t ts = incl - tot;
+#pragma unroll
+    for (int w = 0; w < kThreads / kWave; ++w) ts += (w < wave) ? s_wsum[w] : 0u;
+    tstart[d] = ts;
+    cnt[0][d] = ts;
+    cnt[1][d] = ts + c0;
+    cnt[2][d] = ts + c0 + c1;
+    cnt[3][d] = ts + c0 + c1 + c2;
   }
   __syncthreads();
-  uint32_t* Ko = keys_out + m * stride;
-  uint32_t* Vo = vals_out + m * stride;
 #pragma unroll
   for (int r = 0; r < kRadixItems; ++r) {
     const uint32_t pos = wbase + (uint32_t)r * kWave;
     if (pos < ti.end) {
       const uint32_t dig = (key[r] >> shift) & 255u;
-      const uint32_t dest = cnt[wave][dig] + rank[r];
-      Vo[dest] = val[r];
-      if (WRITE_KEYS) Ko[dest] = key[r];
-      if (WRITE_RANKS) ranks[4 * (uint64_t)val[r] + m] = dest;
+      const uint32_t local = cnt[wave][dig] + rank[r];
+      s_key[local] = key[r];
+      s_val[local] = val[r];
+    }
+  }
+  __syncthreads();
+  uint32_t* Ko = keys_out + m * stride;
+  uint32_t* Vo = vals_out + m * stride;
+  const uint32_t tn = ti.end - ti.base;
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const uint32_t i = threadIdx.x + (uint32_t)r * kThreads;
+    if (i < tn) {
+      const uint32_t k = s_key[i], v = s_val[i];
+      const uint32_t dig = (k >> shift) & 255u;
+      const uint32_t dest = goff[dig] + (i - tstart[dig]);
+      Vo[dest] = v;
+      if (WRITE_KEYS) Ko[dest] = k;
+      if (WRITE_RANKS) ranks[4 * (uint64_t)v + m] = dest;
     }
   }
 }
