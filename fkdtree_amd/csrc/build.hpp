@@ -241,6 +241,9 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   __shared__ uint32_t s_blk;
   __shared__ SegCount s_wave[kThreads / kWave];
   __shared__ SegCount s_carry;
+  __shared__ uint32_t s_cls[kThreads / kWave];
+  // moved elements of the tile in write-out order, with their destinations
+  __shared__ uint32_t s_out[M ? DIM + 1 : 1][M ? kLevelTile : 1];
   const uint32_t bid = dynamic_block(a.ctr, &s_blk);
   const uint32_t arr = M ? bid % M : 0u;
   const uint32_t t = M ? bid / M : bid;
@@ -318,8 +321,12 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
       }
     }
   }
-  // Phase 2: classification and the thread's segmented counts.
+  // Phase 2: classification and the thread's segmented counts.  cnt packs
+  // the thread's unsegmented counts of left (bits 0-11), right (12-23) and
+  // median (24-31) elements: a tile of 2,048 holds at most three intervals
+  // (every interval here has at least 1,023 points), so no field overflows.
   SegCount agg{0u, 0u, 0u};
+  uint32_t cnt = 0;
 #pragma unroll
   for (int i = 0; i < kLevelItems; ++i) {
     const uint32_t jl = first + i;
@@ -335,13 +342,16 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
       c = kLeft;
       dest0[i] = o + st0[i];
       agg.l++;
+      cnt += 1u;
     } else if (r > rm[i]) {
       c = kRight;
       dest0[i] = o + mp[i] + 1u;
       agg.r++;
+      cnt += 1u << 12;
     } else {
       c = kMed;
       dest0[i] = o + mp[i];
+      cnt += 1u << 24;
     }
     if (jl == st0[i]) c |= 4u;  // interval starts here
     cls[i] = c;
@@ -365,13 +375,22 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   excl.r = __shfl_up(incl.r, 1);
   if (lane == 0) excl = SegCount{0u, 0u, 0u};
   const int wave = threadIdx.x / kWave;
-  if (lane == kWave - 1) s_wave[wave] = incl;
+  const uint32_t cnt_incl = wave_incl_sum(cnt);
+  if (lane == kWave - 1) {
+    s_wave[wave] = incl;
+    s_cls[wave] = cnt_incl;
+  }
   __syncthreads();
   SegCount wcarry{0u, 0u, 0u}, tile_agg{0u, 0u, 0u};
+  uint32_t cnt_excl = cnt_incl - cnt, cnt_tot = 0;
 #pragma unroll
   for (int w = 0; w < kThreads / kWave; ++w) {
-    if (w < wave) wcarry = seg_op(wcarry, s_wave[w]);
+    if (w < wave) {
+      wcarry = seg_op(wcarry, s_wave[w]);
+      cnt_excl += s_cls[w];
+    }
     tile_agg = seg_op(tile_agg, s_wave[w]);
+    cnt_tot += s_cls[w];
   }
   excl = seg_op(wcarry, excl);
 
@@ -411,7 +430,14 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   __syncthreads();
   const SegCount run0 = seg_op(s_carry, excl);
   uint32_t nl = run0.l, nr = run0.r;
-  uint32_t* D = a.dst[arr];
+  // Staging: the tile's left elements in tile order, then its right ones,
+  // then its medians.  In tile order the destinations of each class rise,
+  // jumping only between intervals, so the write-out below stores
+  // consecutive slots of one run with consecutive threads.
+  const uint32_t tot_l = cnt_tot & 0xfffu, tot_r = (cnt_tot >> 12) & 0xfffu;
+  uint32_t sl = cnt_excl & 0xfffu;
+  uint32_t sr = tot_l + ((cnt_excl >> 12) & 0xfffu);
+  uint32_t sm = tot_l + tot_r + (cnt_excl >> 24);
 #pragma unroll
   for (int i = 0; i < kLevelItems; ++i) {
     const uint32_t c = (cls[i] & 8u) ? 0u : cls[i];
@@ -419,16 +445,26 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
       nl = 0;
       nr = 0;
     }
-    uint32_t dest = dest0[i];
+    uint32_t dest = dest0[i], slot = sm;
     switch (c & 3u) {
-      case kLeft: dest += nl; ++nl; break;
-      case kRight: dest += nr; ++nr; break;
+      case kLeft: dest += nl; ++nl; slot = sl++; break;
+      case kRight: dest += nr; ++nr; slot = sr++; break;
+      case kMed: ++sm; break;
       default: break;
     }
     if ((c & 3u) != kNone) {
+      s_out[DIM][slot] = dest;
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) D[(uint64_t)k * P + dest] = rk[i][k];
+      for (int k = 0; k < DIM; ++k) s_out[k][slot] = rk[i][k];
     }
+  }
+  __syncthreads();
+  uint32_t* D = a.dst[arr];
+  const uint32_t tn = tot_l + tot_r + (cnt_tot >> 24);
+  for (uint32_t q = threadIdx.x; q < tn; q += kThreads) {
+    const uint32_t dest = s_out[DIM][q];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) D[(uint64_t)k * P + dest] = s_out[k][q];
   }
 }
 
