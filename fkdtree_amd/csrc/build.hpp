@@ -82,9 +82,10 @@ __global__ void bbox_kernel(const float* __restrict__ X, const uint32_t* __restr
 // Rank planes of the level arrays: array m at position j holds the ranks of
 // its point in every dimension k: (k == m) ? j : R[A_m[j]][k], where the
 // ranks of a point form one 16-byte record (one gather per array element).
+// Plane k of an array starts at k * ps; ps = plane_stride(P).
 template <int DIM>
 __global__ void planes_kernel(const uint32_t* __restrict__ A, uint64_t astride, const uint4* __restrict__ R,
-                              uint32_t P, uint32_t* __restrict__ planes, uint64_t pstride) {
+                              uint32_t P, uint32_t* __restrict__ planes, uint64_t ps, uint64_t pstride) {
   static_assert(DIM <= 4, "rank records hold 4 dimensions");
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
 #pragma unroll
@@ -94,7 +95,7 @@ __global__ void planes_kernel(const uint32_t* __restrict__ A, uint64_t astride, 
       const uint32_t rk[4] = {r.x, r.y, r.z, r.w};
       uint32_t* dst = planes + m * pstride;
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * P + j] = (k == m) ? j : rk[k];
+      for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * ps + j] = (k == m) ? j : rk[k];
     }
   }
 }
@@ -107,6 +108,11 @@ __global__ void id_keys_kernel(const uint32_t* __restrict__ ids, uint32_t P, uin
     V[i] = i;
   }
 }
+
+// Stride of the rank planes: a multiple of 4 (16-byte aligned planes) with
+// at least 4 slack elements, so the level kernel's aligned 16-byte windows
+// around an event's positions stay inside their plane.
+__host__ __device__ inline uint64_t plane_stride(uint64_t P) { return ((P + 3u) & ~(uint64_t)3u) + 4u; }
 
 constexpr int kLevelItems = 8;
 constexpr int kLevelTile = kThreads * kLevelItems;
@@ -133,7 +139,8 @@ struct LevelArgs {
   TileMap tm;                 // tiles of kLevelTile points, event aligned
   const float* X;             // input points
   const uint32_t* ids;
-  uint32_t P;                 // plane stride
+  uint32_t P;                 // points of the batch (stride of X and the tree arrays)
+  uint64_t PS;                // plane stride, plane_stride(P)
   float* Tc;                  // tree outputs
   uint32_t* Tid;
   float* recs;                // search records (rotated coordinates, id)
@@ -252,7 +259,7 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   const Shape sh = shape_of(n);
   const bool split = a.L < fin_depth(sh.D);  // deeper levels: finish_kernel
   const bool root_only = (n == 1 && a.L == 0);
-  const uint64_t P = a.P;
+  const uint64_t P = a.PS;  // plane stride
   if (!split) {
     if (M && threadIdx.x == 0) publish(&a.desc[bid], kStPrefix | seg_pack(SegCount{1u, 0u, 0u}));
     if (root_only && arr == 0 && threadIdx.x == 0) a.F[o] = a.Ad[o];
@@ -266,25 +273,25 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   // the loads of all items can be in flight together).
   uint32_t cls[kLevelItems], dest0[kLevelItems], rk[kLevelItems][DIM], rm[kLevelItems], fr[kLevelItems];
   uint32_t st0[kLevelItems], mp[kLevelItems], hn[kLevelItems];
-  // The thread's 8 ranks of a plane as two 16-byte loads where the planes
-  // are 16-byte aligned (plane stride and event offset multiples of 4) and
-  // the tile holds all 8 items; one 4-byte load per item otherwise.  The
-  // vector loads also read separator positions, whose ranks go unused.
-  static_assert(kLevelItems == 8, "two 16-byte loads per plane");
-  const bool vec = M != 0 && first + (uint32_t)kLevelItems <= n && ((P | (uint64_t)o) & 3u) == 0u;
+  // The thread's 8 ranks of a plane from aligned 16-byte loads where the
+  // tile holds all 8 items: two when the event starts on a multiple of 4,
+  // else three around them, shifted by sh4 = o % 4 (the same for the whole
+  // block; planes are 16-byte aligned and padded, see plane_stride).  One
+  // 4-byte load per item otherwise.  The windows also read separator
+  // positions and up to 3 positions past the items, whose ranks go unused.
+  static_assert(kLevelItems == 8, "two or three 16-byte loads per plane");
+  const bool vec = M != 0 && first + (uint32_t)kLevelItems <= n;
+  const uint32_t sh4 = o & 3u;
   if (vec) {
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
-      const uint4* q = reinterpret_cast<const uint4*>(S + (uint64_t)k * P + o + first);
+      const uint4* q = reinterpret_cast<const uint4*>(S + (uint64_t)k * P + o + first - sh4);
       const uint4 x0 = q[0], x1 = q[1];
-      rk[0][k] = x0.x;
-      rk[1][k] = x0.y;
-      rk[2][k] = x0.z;
-      rk[3][k] = x0.w;
-      rk[4][k] = x1.x;
-      rk[5][k] = x1.y;
-      rk[6][k] = x1.z;
-      rk[7][k] = x1.w;
+      const uint4 x2 = sh4 ? q[2] : make_uint4(0u, 0u, 0u, 0u);
+      const uint32_t e[12] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w, x2.x, x2.y, x2.z, x2.w};
+#pragma unroll
+      for (int i = 0; i < kLevelItems; ++i)
+        rk[i][k] = sh4 == 0u ? e[i] : sh4 == 1u ? e[i + 1] : sh4 == 2u ? e[i + 2] : e[i + 3];
     }
   }
 #pragma unroll
@@ -543,7 +550,7 @@ struct FinArgs {
   const uint32_t* planes[4][2];  // array m, buffer c: plane k (rank in dimension k) at planes[m][c] + k * P
   uint32_t* F;                // heap index -> rank in dimension 0 of the node's point
   TileMap fm;                 // workgroup -> event; local index = subtree of depth fin_depth(D)
-  uint32_t P;
+  uint64_t P;                 // plane stride, plane_stride(points of the batch)
 };
 
 // The levels fin_depth(D) .. D-1 of one subtree in LDS: the same medians and

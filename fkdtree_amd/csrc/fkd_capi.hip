@@ -370,10 +370,11 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
                                                                t->n_events, P, t->bbox.as<uint32_t>());
   FKD_TRY(launch_check("bbox"));
   // 4. Rank planes: every array element carries its ranks in all dimensions.
-  uint32_t* PL = t->planes.as<uint32_t>();  // PL[m][c][k][P]
-  auto Pbuf = [&](int m, int c) { return PL + (((uint64_t)m * 2 + c) * DIM) * S; };
+  uint32_t* PL = t->planes.as<uint32_t>();  // PL[m][c][k][PS]
+  const uint64_t PS = fkd::plane_stride(P);
+  auto Pbuf = [&](int m, int c) { return PL + (((uint64_t)m * 2 + c) * DIM) * PS; };
   fkd::planes_kernel<DIM><<<grid, 256, 0, s>>>(Abuf(0, 0), 2 * S, reinterpret_cast<const uint4*>(R), P, Pbuf(0, 0),
-                                               2 * (uint64_t)DIM * S);
+                                               PS, 2 * (uint64_t)DIM * PS);
   FKD_TRY(launch_check("planes"));
   // 5. One partition launch per depth.
   int cur[4] = {0, 0, 0, 0};
@@ -401,6 +402,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     a.X = X;
     a.ids = t->ids.as<uint32_t>();
     a.P = P;
+    a.PS = PS;
     a.Tc = t->Tc.as<float>();
     a.Tid = t->Tid.as<uint32_t>();
     a.recs = t->recs.as<float>();
@@ -420,7 +422,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     }
     f.F = t->F.as<uint32_t>();
     f.fm = t->fin_tiles.map(t->off.as<uint32_t>());
-    f.P = P;
+    f.P = PS;
     fkd::finish_kernel<DIM><<<t->fin_tiles.T, fkd::kThreads, 0, s>>>(f);
     FKD_TRY(launch_check("finish"));
   }
@@ -712,7 +714,7 @@ int ensure_build_scratch(fkd_tree_s* t) {
   FKD_TRY(t->A.ensure(S * dim * 2 * 4));
   FKD_TRY(t->K.ensure(S * dim * 2 * 4));
   FKD_TRY(t->R.ensure(S * 4 * 4));
-  FKD_TRY(t->planes.ensure(S * dim * dim * 2 * 4));
+  FKD_TRY(t->planes.ensure(fkd::plane_stride(S) * dim * dim * 2 * 4));
   FKD_TRY(t->F.ensure(S * 4));
   FKD_TRY(t->Xr.ensure(S * 16));
   const uint64_t nc = (uint64_t)dim * 256u * std::max<uint32_t>(t->radix_tiles.T, 1);
