@@ -109,10 +109,12 @@ __global__ void id_keys_kernel(const uint32_t* __restrict__ ids, uint32_t P, uin
   }
 }
 
-// Stride of the rank planes: a multiple of 4 (16-byte aligned planes) with
-// at least 4 slack elements, so the level kernel's aligned 16-byte windows
-// around an event's positions stay inside their plane.
-__host__ __device__ inline uint64_t plane_stride(uint64_t P) { return ((P + 3u) & ~(uint64_t)3u) + 4u; }
+// Stride of the rank planes: a multiple of 64 (256-byte aligned planes, so a
+// thread's 8 ranks never straddle a 32-byte sector when the event starts on
+// a multiple of 8) with at least 4 slack elements, so the level kernel's
+// aligned 16-byte windows around an event's positions stay inside their
+// plane.
+__host__ __device__ inline uint64_t plane_stride(uint64_t P) { return (P + 4u + 63u) & ~(uint64_t)63u; }
 
 constexpr int kLevelItems = 8;
 constexpr int kLevelTile = kThreads * kLevelItems;
@@ -282,16 +284,23 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   static_assert(kLevelItems == 8, "two or three 16-byte loads per plane");
   const bool vec = M != 0 && first + (uint32_t)kLevelItems <= n;
   const uint32_t sh4 = o & 3u;
-  if (vec) {
+  if (vec && sh4 == 0u) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const uint4* q = reinterpret_cast<const uint4*>(S + (uint64_t)k * P + o + first);
+      const uint4 x0 = q[0], x1 = q[1];
+      const uint32_t e[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+      for (int i = 0; i < kLevelItems; ++i) rk[i][k] = e[i];
+    }
+  } else if (vec) {
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
       const uint4* q = reinterpret_cast<const uint4*>(S + (uint64_t)k * P + o + first - sh4);
-      const uint4 x0 = q[0], x1 = q[1];
-      const uint4 x2 = sh4 ? q[2] : make_uint4(0u, 0u, 0u, 0u);
+      const uint4 x0 = q[0], x1 = q[1], x2 = q[2];
       const uint32_t e[12] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w, x2.x, x2.y, x2.z, x2.w};
 #pragma unroll
-      for (int i = 0; i < kLevelItems; ++i)
-        rk[i][k] = sh4 == 0u ? e[i] : sh4 == 1u ? e[i + 1] : sh4 == 2u ? e[i + 2] : e[i + 3];
+      for (int i = 0; i < kLevelItems; ++i) rk[i][k] = sh4 == 1u ? e[i + 1] : sh4 == 2u ? e[i + 2] : e[i + 3];
     }
   }
 #pragma unroll
