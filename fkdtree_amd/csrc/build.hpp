@@ -128,6 +128,21 @@ constexpr uint32_t kFinCap = (1u << kFinLog) - 1u;
 constexpr int kFinItems = 4;  // consecutive positions per thread in the finisher
 static_assert(kFinCap <= (uint32_t)(kThreads * kFinItems), "a finisher subtree fits one pass of the workgroup");
 __host__ __device__ inline uint32_t fin_depth(uint32_t D) { return D + 1u > kFinLog ? D + 1u - kFinLog : 0u; }
+// Largest finisher subtree of an event of n >= 1 points: H - 1 + min(R, H)
+// with H = 2^(D - fin_depth(D)) and R the nodes on the last depth D.
+__host__ __device__ inline uint32_t fin_subtree_max(uint32_t n) {
+  const uint32_t D = floor_log2(n);
+  const uint32_t H = 1u << (D - fin_depth(D));
+  const uint32_t R = n - ((1u << D) - 1u);
+  return H - 1u + (R < H ? R : H);
+}
+// Dynamic LDS of finish_kernel for subtrees of at most cap points: the rank
+// table (dim x cap uint32) and the names (dim x 2 x (cap + 1) uint16).  Sized
+// per launch so that trees whose subtrees are half full (2^k points) keep
+// twice as many workgroups resident.
+__host__ __device__ inline uint32_t fin_lds_bytes(uint32_t dim, uint32_t cap) {
+  return (dim * cap * 4u + dim * 2u * (cap + 1u) * 2u + 15u) & ~15u;
+}
 
 template <int DIM>
 struct LevelArgs {
@@ -560,6 +575,7 @@ struct FinArgs {
   uint32_t* F;                // heap index -> rank in dimension 0 of the node's point
   TileMap fm;                 // workgroup -> event; local index = subtree of depth fin_depth(D)
   uint64_t P;                 // plane stride, plane_stride(points of the batch)
+  uint32_t cap;               // most points of one subtree (fin_subtree_max over the events)
 };
 
 // The levels fin_depth(D) .. D-1 of one subtree in LDS: the same medians and
@@ -568,11 +584,19 @@ struct FinArgs {
 // and their ranks kept in a table, so only 16-bit names move.  Writes F
 // (heap index -> rank) for the subtree's nodes, leaves included.
 template <int DIM>
-__global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
-  __shared__ uint32_t s_rank[DIM][kFinCap];
+// Eight waves per SIMD (at most 64 registers) keep a whole launch of
+// half-full subtrees resident (2,048 workgroups on 1M points); dim 4 would
+// spill at that bound and keeps the default.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DIM <= 3 ? 8 : 1))) void finish_kernel(FinArgs<DIM> a) {
   constexpr int MA = DIM > 1 ? DIM - 1 : 1;  // arrays moved per depth
-  __shared__ uint16_t s_arr[DIM][2][kFinCap + 1];
   __shared__ SegCountN<MA> s_wave[kThreads / kWave];
+  extern __shared__ __align__(16) uint32_t fin_lds[];
+  const uint32_t cap = a.cap;
+  // s_rank(k, j): rank in dimension k of name j; s_arr(m, c, j): buffer c of array m
+  uint32_t* const s_rank_base = fin_lds;
+  uint16_t* const s_arr_base = reinterpret_cast<uint16_t*>(fin_lds + (uint64_t)DIM * cap);
+  auto s_rank = [&](uint32_t k) { return s_rank_base + (uint64_t)k * cap; };
+  auto s_arr = [&](uint32_t m, uint32_t c) { return s_arr_base + ((uint64_t)m * 2u + c) * (cap + 1u); };
   const uint32_t e = a.fm.tile_ev[blockIdx.x];
   const uint32_t t0 = blockIdx.x - a.fm.tb[e];
   const uint32_t o = a.fm.off[e];
@@ -587,8 +611,8 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
   for (int m = 0; m < DIM; ++m) pl[m] = a.planes[m][moves_before(L0, (uint32_t)m, (uint32_t)DIM) & 1u];
   for (uint32_t j = threadIdx.x; j < size0; j += kThreads) {
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) s_rank[k][j] = pl[0][(uint64_t)k * P + o + start0 + j];
-    s_arr[0][0][j] = (uint16_t)j;
+    for (int k = 0; k < DIM; ++k) s_rank(k)[j] = pl[0][(uint64_t)k * P + o + start0 + j];
+    s_arr(0, 0)[j] = (uint16_t)j;
   }
   if (sh.D == 0) {  // a single point
     if (threadIdx.x == 0 && n == 1) a.F[o] = pl[0][o];
@@ -600,20 +624,20 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
   for (int m = 1; m < DIM; ++m)
     for (uint32_t j = threadIdx.x; j < size0; j += kThreads) {
       const uint32_t r0 = pl[m][o + start0 + j];
-      uint32_t lo = 0, hi = size0;  // first index with s_rank[0][idx] >= r0
+      uint32_t lo = 0, hi = size0;  // first index with s_rank(0)[idx] >= r0
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (s_rank[0][mid] < r0) lo = mid + 1u; else hi = mid;
+        if (s_rank(0)[mid] < r0) lo = mid + 1u; else hi = mid;
       }
-      s_arr[m][0][j] = (uint16_t)(lo < size0 ? lo : size0 - 1u);  // always found; kept in range
+      s_arr(m, 0)[j] = (uint16_t)(lo < size0 ? lo : size0 - 1u);  // always found; kept in range
     }
   __syncthreads();
-  uint32_t curbits = 0;  // bit m: the buffer of s_arr[m] that holds array m
+  uint32_t curbits = 0;  // bit m: the buffer of s_arr(m, .) that holds array m
   const uint32_t first = threadIdx.x * (uint32_t)kFinItems;
   for (uint32_t L = L0; L < sh.D; ++L) {
     const uint32_t d = L % (uint32_t)DIM;
     const bool last = L + 1u == sh.D;
-    const uint16_t* med = s_arr[d][(curbits >> d) & 1u];
+    const uint16_t* med = s_arr(d, (curbits >> d) & 1u);
     uint32_t st[kFinItems], mp[kFinItems];
     bool inside[kFinItems];
 #pragma unroll
@@ -631,7 +655,7 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
         // node of depth L, or (last split) a leaf of depth D
         if (inside[i] && (jl == mp[i] || last)) {
           const uint32_t h = jl == mp[i] ? (1u << L) - 1u + ts : (1u << sh.D) - 1u + subtree_at(sh, sh.D, jl);
-          a.F[o + h] = s_rank[0][med[j]];
+          a.F[o + h] = s_rank(0)[med[j]];
         }
       }
     }
@@ -640,7 +664,7 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
     // partitioned together: one segmented scan of all their counts.
     uint32_t rmed[kFinItems];  // rank in dimension d of the item's interval median
 #pragma unroll
-    for (int i = 0; i < kFinItems; ++i) rmed[i] = inside[i] ? s_rank[d][med[mp[i] - start0]] : 0u;
+    for (int i = 0; i < kFinItems; ++i) rmed[i] = inside[i] ? s_rank(d)[med[mp[i] - start0]] : 0u;
     uint32_t cls[MA][kFinItems], dest0[MA][kFinItems], who[MA][kFinItems];
     SegCountN<MA> agg = seg_zero<MA>();
 #pragma unroll
@@ -659,8 +683,8 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
         if (!inside[i]) continue;
         uint32_t m = d + 1u + (uint32_t)k;
         m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
-        who[k][i] = s_arr[m][(curbits >> m) & 1u][j];
-        const uint32_t r = s_rank[d][who[k][i]];
+        who[k][i] = s_arr(m, (curbits >> m) & 1u)[j];
+        const uint32_t r = s_rank(d)[who[k][i]];
         uint32_t c;
         if (r < rmed[i]) {
           c = kLeft;
@@ -683,7 +707,7 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinArgs<DIM> a) {
     for (int k = 0; k < MA; ++k) {
       uint32_t m = d + 1u + (uint32_t)k;
       m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
-      uint16_t* dst = s_arr[m][((curbits >> m) & 1u) ^ 1u];
+      uint16_t* dst = s_arr(m, ((curbits >> m) & 1u) ^ 1u);
       uint32_t nl = run0.l[k], nr = run0.r[k];
 #pragma unroll
       for (int i = 0; i < kFinItems; ++i) {

@@ -204,6 +204,7 @@ struct fkd_tree_s {
   uint32_t P = 0;
   uint32_t Dmax = 0;
   uint32_t global_levels = 0;  // depths built by level_kernel; the rest by finish_kernel
+  uint32_t fin_cap = 1;        // most points of one finisher subtree over all events
   bool ids_in_order = true;
   bool built = false;
   std::vector<uint64_t> h_off;
@@ -423,7 +424,8 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     f.F = t->F.as<uint32_t>();
     f.fm = t->fin_tiles.map(t->off.as<uint32_t>());
     f.P = PS;
-    fkd::finish_kernel<DIM><<<t->fin_tiles.T, fkd::kThreads, 0, s>>>(f);
+    f.cap = t->fin_cap;
+    fkd::finish_kernel<DIM><<<t->fin_tiles.T, fkd::kThreads, fkd::fin_lds_bytes(DIM, f.cap), s>>>(f);
     FKD_TRY(launch_check("finish"));
   }
   // 7. Tree arrays and search records from F.
@@ -730,6 +732,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   for (uint32_t e = 0; e <= E; ++e) off32[e] = (uint32_t)t->h_off[e];
   t->Dmax = 0;
   t->global_levels = 0;
+  t->fin_cap = 1;
   std::vector<uint32_t> fin_count(E, 0);
   for (uint32_t e = 0; e < E; ++e) {
     const uint64_t n = t->h_off[e + 1] - t->h_off[e];
@@ -738,6 +741,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
     t->Dmax = std::max(t->Dmax, D);
     t->global_levels = std::max(t->global_levels, fkd::fin_depth(D));
     fin_count[e] = 1u << fkd::fin_depth(D);  // subtrees the finisher builds
+    t->fin_cap = std::max(t->fin_cap, fkd::fin_subtree_max((uint32_t)n));
   }
   FKD_TRY(t->off.ensure((E + 1) * 4));
   FKD_HIP(hipMemcpyAsync(t->off.p, off32.data(), (E + 1) * 4, hipMemcpyHostToDevice, s));
