@@ -87,15 +87,30 @@ template <int DIM>
 __global__ void planes_kernel(const uint32_t* __restrict__ A, uint64_t astride, const uint4* __restrict__ R,
                               uint32_t P, uint32_t* __restrict__ planes, uint64_t ps, uint64_t pstride) {
   static_assert(DIM <= 4, "rank records hold 4 dimensions");
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
+  constexpr int U = 4;  // elements per thread and round, their gathers in flight together
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j0 < P; j0 += U * stride) {
+    uint32_t pt[U][DIM];
+    uint4 rr[U][DIM];
 #pragma unroll
-    for (int m = 0; m < DIM; ++m) {
-      const uint32_t p = A[m * astride + j];
-      const uint4 r = R[p];
-      const uint32_t rk[4] = {r.x, r.y, r.z, r.w};
-      uint32_t* dst = planes + m * pstride;
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * ps + j] = (k == m) ? j : rk[k];
+      for (int m = 0; m < DIM; ++m) pt[u][m] = j0 + u * stride < P ? A[m * astride + j0 + u * stride] : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int m = 0; m < DIM; ++m) rr[u][m] = j0 + u * stride < P ? R[pt[u][m]] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t j = j0 + u * stride;
+      if (j >= P) continue;
+#pragma unroll
+      for (int m = 0; m < DIM; ++m) {
+        const uint32_t rk[4] = {rr[u][m].x, rr[u][m].y, rr[u][m].z, rr[u][m].w};
+        uint32_t* dst = planes + m * pstride;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * ps + j] = (k == m) ? (uint32_t)j : rk[k];
+      }
     }
   }
 }
@@ -186,13 +201,11 @@ __device__ inline LevelTile level_tile(const TileMap& tm, uint32_t t) {
   return r;
 }
 
+// The node record of point p: its coordinates and id (one 16-byte gather of
+// the point record for dim <= 3).
 template <int DIM>
-__device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t node, uint32_t depth, uint32_t p,
-                                  const float4* prec) {
-  constexpr int RF = NodeRec<DIM>::kFloats;
-  float c[DIM];
-  uint32_t id;
-  const uint32_t dd = depth % (uint32_t)DIM;
+__device__ inline void node_point(const LevelArgs<DIM>& a, uint32_t p, const float4* prec, float (&c)[DIM],
+                                  uint32_t& id) {
   if (DIM <= 3) {
     const float4 r = prec[p];
     const float rc[3] = {r.x, r.y, r.z};
@@ -204,6 +217,14 @@ __device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t 
     for (int m = 0; m < DIM; ++m) c[m] = a.X[(uint64_t)m * a.P + p];
     id = a.ids[p];
   }
+}
+
+// Heap-ordered tree arrays and the rotated search record of one node.
+template <int DIM>
+__device__ inline void write_node(const LevelArgs<DIM>& a, uint32_t o, uint32_t node, uint32_t depth,
+                                  const float (&c)[DIM], uint32_t id) {
+  constexpr int RF = NodeRec<DIM>::kFloats;
+  const uint32_t dd = depth % (uint32_t)DIM;
   float r[4] = {0.f, 0.f, 0.f, 0.f};  // rotated coordinates
 #pragma unroll
   for (int m = 0; m < DIM; ++m) {
@@ -820,18 +841,45 @@ __global__ void records_kernel(const float* __restrict__ X, const uint32_t* __re
 // coalesced.
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a, const float4* __restrict__ rec) {
+  constexpr int IT = kLevelTile / kThreads;
   const uint32_t t = blockIdx.x;
   const LevelTile lti = level_tile(a.tm, t);
   const uint32_t lt = lti.lt, o = lti.o, n = lti.n;
-  for (uint32_t v = lt * (uint32_t)kLevelTile + threadIdx.x; v < min(n, (lt + 1) * (uint32_t)kLevelTile);
-       v += kThreads) {
-    uint32_t p = o + v;  // S0 == null: the points already are the heap-ordered tree
-    if (a.S0) {
-      const uint32_t f = a.F[o + v];
-      if (f >= a.P) continue;  // F is complete; the test keeps reads in range
-      p = a.S0[f];
+  const uint32_t v0 = lt * (uint32_t)kLevelTile + threadIdx.x;
+  const uint32_t vend = min(n, (lt + 1u) * (uint32_t)kLevelTile);
+  // each round of loads for all the thread's nodes before the next (F, then
+  // the point index, then the point record), so they are in flight together
+  uint32_t p[IT];
+  bool ok[IT];
+#pragma unroll
+  for (int r = 0; r < IT; ++r) {
+    const uint32_t v = v0 + (uint32_t)r * kThreads;
+    ok[r] = v < vend;
+    p[r] = o + v;  // S0 == null: the points already are the heap-ordered tree
+  }
+  if (a.S0) {
+    uint32_t f[IT];
+#pragma unroll
+    for (int r = 0; r < IT; ++r) f[r] = ok[r] ? a.F[o + v0 + (uint32_t)r * kThreads] : a.P;
+#pragma unroll
+    for (int r = 0; r < IT; ++r) {
+      ok[r] = ok[r] && f[r] < a.P;  // F is complete; the test keeps reads in range
+      p[r] = ok[r] ? a.S0[f[r]] : 0u;
     }
-    write_node(a, o, v, floor_log2(v + 1u), p, rec);
+  }
+  float c[IT][DIM];
+  uint32_t id[IT];
+#pragma unroll
+  for (int r = 0; r < IT; ++r) {
+    id[r] = 0;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) c[r][m] = 0.f;
+    if (ok[r]) node_point<DIM>(a, p[r], rec, c[r], id[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < IT; ++r) {
+    const uint32_t v = v0 + (uint32_t)r * kThreads;
+    if (ok[r]) write_node<DIM>(a, o, v, floor_log2(v + 1u), c[r], id[r]);
   }
 }
 

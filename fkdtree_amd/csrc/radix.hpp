@@ -60,8 +60,16 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
   __syncthreads();
   const TileInfo ti = tile_info<kRadixTile>(tm, t);
   const uint32_t* K = keys + m * kstride;
-  for (uint32_t i = ti.base + threadIdx.x; i < ti.end; i += kThreads)
-    atomicAdd(&hist[wave][(K[i] >> shift) & 255u], 1u);
+  // all loads of the thread first, so they are in flight together
+  uint32_t key[kRadixItems];
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const uint32_t i = ti.base + threadIdx.x + (uint32_t)r * kThreads;
+    key[r] = i < ti.end ? K[i] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r)
+    if (ti.base + threadIdx.x + (uint32_t)r * kThreads < ti.end) atomicAdd(&hist[wave][(key[r] >> shift) & 255u], 1u);
   __syncthreads();
   const uint32_t c = hist[0][threadIdx.x] + hist[1][threadIdx.x] + hist[2][threadIdx.x] +
                      hist[3][threadIdx.x];
