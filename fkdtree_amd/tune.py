@@ -132,6 +132,9 @@ def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=Non
     return out
 
 
+KEEP_SWEEP = 16  # sweep rows kept per table entry
+
+
 def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path: str = TABLE) -> Dict:
     """Sweep on a workload: one tree (mixed_1m, uniform_100k, uniform_4m) or
     events_256 (the first 256 events of the 2,048-event set, every point's
@@ -164,14 +167,15 @@ def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path
     res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=d_q.ptr if d_q is not None else None)
     tree.close()
     arch = _arch()
+    # the table keeps the fastest settings of the sweep; the whole sweep is returned
     entry = {"params": res[0][0], "ms": res[0][1], "workload": workload,
-             "sweep": [{"params": p, "ms": ms} for p, ms in res]}
+             "sweep": [{"params": p, "ms": ms} for p, ms in res[:KEEP_SWEEP]]}
     if write:
         table = load_table(path)
         table.setdefault(arch, {})[key] = entry
         with open(path, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
-    return {"arch": arch, "key": key, **entry}
+    return {"arch": arch, "key": key, **entry, "sweep": [{"params": p, "ms": ms} for p, ms in res]}
 
 
 def main(argv=None):
