@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--events", type=int, default=256,
                     help="events of the batched workload (first N of the 2,048-event set; 0 skips)")
     ap.add_argument("--event-steps", type=int, default=3)
-    ap.add_argument("--serve-events", type=int, default=128,
+    ap.add_argument("--serve-events", type=int, default=512,
                     help="events of the serving stream (0 skips)")
     ap.add_argument("--serve-rates", default="1000,5000,20000", help="Poisson arrival rates (events/s)")
     ap.add_argument("--events-only", action="store_true",
@@ -208,13 +208,16 @@ def bench_serving(n_events: int, rates) -> list:
     Poisson streams of n_events events (sizes log-uniform in [1,000, 500,000]
     points, mixed distribution; every point searched with half-width 0.01),
     received as FKDPoint records into a page-locked ingest ring."""
-    from fkdtree_amd.serving import EventServer, poisson_requests
+    from fkdtree_amd.serving import EventServer, Request, poisson_requests
 
     out = []
+    # one stream of events, generated once: the sizes and points do not depend
+    # on the rate, only the arrival times do (Poisson gaps scale as 1 / rate)
+    base = poisson_requests(n_events, 1.0, records=True)
+    ring = max(16_000_000, 2 * max(r.n for r in base))
     for rate in rates:
         # events arrive as FKDPoint records into the page-locked ingest ring
-        reqs = poisson_requests(n_events, rate, records=True)
-        ring = max(16_000_000, 2 * max(r.n for r in reqs))
+        reqs = [Request(r.event, None, None, r.arrival / rate, records=r.records) for r in base]
         srv = EventServer(n_streams=3, max_batch_points=4_000_000, max_batch_events=256, ring_points=ring)
         srv.run(reqs[: min(4, n_events)])          # warm-up (first allocations)
         srv.close()
