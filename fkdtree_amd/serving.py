@@ -4,7 +4,8 @@ Events (independent point clouds) arrive over time.  Each must be built into
 its own tree and have every one of its points searched with a box of
 half-width `half_width`.
 
-The server drives the GPU from one host thread:
+The server drives the GPU from one serving thread, with one worker thread
+per stream that creates batches (creation synchronises the batch's stream):
 
 * Requests that have arrived queue up in arrival order.  Whenever a stream
   slot is free, the longest prefix of the queue that fits the limits (points,
@@ -13,7 +14,8 @@ The server drives the GPU from one host thread:
   not a single queue drained batch by batch.
 * A batch is one FKDEvents handle: one build launch per depth for all its
   events, then one search of all its boxes (count, scan and fill enqueued
-  together), on its own stream.
+  together), on its own stream.  A worker thread creates it and enqueues its
+  work; the serving thread meanwhile completes and admits other batches.
   Several batches overlap on the device.
 * When a batch completes, its results are checked for completeness (the
   output is regrown and the search re-run if needed) and its latency is
