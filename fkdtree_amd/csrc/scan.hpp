@@ -68,10 +68,24 @@ __global__ __launch_bounds__(kThreads) void scan_exclusive_kernel(const uint32_t
   }
   __syncthreads();
   uint64_t run = s_carry + wave_off + incl - sum;
+  OutT o[kScanItems];
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    if (base + i <= n) out[base + i] = (OutT)run;
+    o[i] = (OutT)run;
     run += v[i];
+  }
+  // the thread's outputs as 16-byte stores where all are in range and aligned
+  constexpr int kVec = (int)(kScanItems * sizeof(OutT) / 16u);
+  static_assert(kScanItems * sizeof(OutT) % 16u == 0, "whole 16-byte stores");
+  if (base + kScanItems <= n + 1 && (reinterpret_cast<uintptr_t>(out + base) & 15u) == 0) {
+    uint4 w[kVec];
+    __builtin_memcpy(w, o, sizeof(o));
+#pragma unroll
+    for (int j = 0; j < kVec; ++j) reinterpret_cast<uint4*>(out + base)[j] = w[j];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+      if (base + i <= n) out[base + i] = o[i];
   }
 }
 
