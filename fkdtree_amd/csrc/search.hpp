@@ -136,6 +136,12 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   for (int m = 0; m < DIM; ++m) {
     mn[m] = a.mins[(uint64_t)m * a.nq + slot];
     mx[m] = a.maxs[(uint64_t)m * a.nq + slot];
+    // A NaN lower bound matches nothing and sends the walk left only; with
+    // (-inf, NaN) the three compares below give exactly that.
+    if (mn[m] != mn[m]) {
+      mn[m] = -__builtin_huge_valf();
+      mx[m] = __builtin_nanf("");
+    }
     if (LDS_BOX) my_box[m] = my_box[m + DIM] = make_float2(mn[m], mx[m]);
   }
   NodeSource src;
@@ -189,12 +195,15 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
       lo[0] = mn[0];
       hi[0] = mx[0];
     }
-    // x[0] is the split value and lo[0], hi[0] the bounds on the split dimension
-    const bool inter = (x[0] <= hi[0]) & (x[0] >= lo[0]);
-    bool in = inter;
+    // x[0] is the split value and lo[0], hi[0] the bounds on the split
+    // dimension.  The reference's rule (left son unless the split value is
+    // below the box, right son if it is below or inside) from three compares:
+    // with lo[0] not NaN, !(x < lo) is x >= lo, and (x < lo) || inside is
+    // (x < lo) || (x <= hi).
+    const bool ge = x[0] >= lo[0], le = x[0] <= hi[0], lower = x[0] < lo[0];
+    bool in = ge & le;
 #pragma unroll
     for (int j = 1; j < DIM; ++j) in = in & (x[j] <= hi[j]) & (x[j] >= lo[j]);
-    const bool lower = x[0] < lo[0];
     if (FILL) {
       rw[wr & (kRing - 1)] = id;  // the slot is free; it is kept only for a hit
       wr += in ? 1u : 0u;
@@ -205,8 +214,8 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     } else {
       c += in ? 1u : 0u;
     }
-    const bool go_l = !lower && node < tl;
-    const bool go_r = (lower || inter) && node < tr;
+    const bool go_l = ge && node < tl;
+    const bool go_r = (lower || le) && node < tr;
     const bool desc = go_l || go_r;
     pending |= (go_l && go_r) ? (2u << depth) : 0u;
     const uint32_t b = 31u - (uint32_t)__builtin_clz(pending);  // deepest pending depth

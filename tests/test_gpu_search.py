@@ -151,3 +151,27 @@ def test_one_call_search_capacity_and_parity(require_gpu):
     ooff, oids = oracle.search(tc, ti, boxes.mins, boxes.maxs)
     assert np.array_equal(o1, ooff)
     assert np.array_equal(sorted_sets(o1, i1), sorted_sets(ooff, oids))
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_visited_counts_match_oracle(require_gpu, kind):
+    """fkd_search_stats' per-box hit and visited-node counts equal the
+    oracle's level-BFS walk (FKDTree_cpu.h:150-157): the push rule visits the
+    same nodes, including for inverted boxes and NaN bounds."""
+    from fkdtree_amd import DeviceArray
+    from fkdtree_amd._lib import check, lib
+    for n in (1, 2, 7, 100, 4097):
+        p = adversarial_points(kind, n)
+        for label, boxes in (("hw0.05", self_boxes(p, 0.05)), ("special", special_boxes(p))):
+            print(f"[visited] {kind} n={n} {label} q={boxes.q}", flush=True)
+            t = FKDTree(coords=p.coords, ids=p.ids)
+            t.build()
+            dmi = DeviceArray.from_host(boxes.mins)
+            dma = DeviceArray.from_host(boxes.maxs)
+            cnt = DeviceArray(boxes.q, np.uint32)
+            vis = DeviceArray(boxes.q, np.uint32)
+            check(lib().fkd_search_stats(t._h, boxes.q, dmi.ptr, dma.ptr, None, cnt.ptr, vis.ptr, None))
+            tc, ti = oracle.build(p.coords, p.ids)
+            ocnt, ovis = oracle.search_counts(tc, ti, boxes.mins, boxes.maxs)
+            assert np.array_equal(cnt.download(), ocnt), f"{kind} n={n} {label}: counts differ"
+            assert np.array_equal(vis.download(), ovis), f"{kind} n={n} {label}: visited differ"
+            t.close()
