@@ -38,6 +38,8 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr uint32_t kMaxPoints = 1u << 30;
+// Events per handle: n_events + 1 offsets stay far from any 32-bit wrap.
+constexpr uint32_t kMaxEvents = 1u << 30;
 
 // Stream on which the current call allocates and frees its buffers
 // (stream-ordered allocator), set by every entry point that enqueues work;
@@ -83,11 +85,18 @@ hipError_t dev_free(void* p) { return tl_ordered ? hipFreeAsync(p, tl_stream) : 
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  // Live HIP graphs of the owning handle (null: not a handle buffer).  A
+  // captured graph holds this buffer's address, so while one exists the
+  // buffer neither moves nor goes away.
+  const uint32_t* graphs = nullptr;
   ~DevBuf() {
     if (p) (void)dev_free(p);
   }
   int ensure(size_t want) {
     if (want <= bytes) return FKD_OK;
+    if (graphs && *graphs)
+      return fail(FKD_ERR_STATE, "a HIP graph of this handle holds its buffers: this call needs larger ones "
+                                 "(destroy the graph first)");
     if (p) {
       FKD_HIP(dev_free(p));
       p = nullptr;
@@ -198,6 +207,7 @@ struct fkd_tuning {
 };
 
 struct fkd_tree_s {
+  fkd_tree_s();
   fkd_tuning tune;
   int dim = 3;
   uint32_t n_events = 0;
@@ -227,16 +237,62 @@ struct fkd_tree_s {
   uint32_t q_perm_half = 0;  // which half of q_perm holds it
   uint64_t* q_scan_desc = nullptr;  // zeroed descriptors for the scan of the count pass
   uint64_t last_nq = 0, last_total = 0;
+  // boxes of the plan in q_perm / q_sbox / q_wslot (fkd_search_fill must name the same)
+  const float* plan_mins = nullptr;
+  const float* plan_maxs = nullptr;
+  const uint32_t* plan_qev = nullptr;
+  uint32_t graph_count = 0;              // live graphs captured on this handle
+  std::vector<fkd_graph_s*> graph_list;  // ... detached when the handle goes first
+  ~fkd_tree_s();
+
+  // Every device buffer of the handle.
+  std::vector<DevBuf*> buffers() {
+    return {&off, &radix_tiles.tile_ev, &radix_tiles.tb, &level_tiles.tile_ev, &level_tiles.tb,
+            &fin_tiles.tile_ev, &fin_tiles.tb, &X, &ids, &Tc, &Tid, &recs, &A, &K, &R, &Xr, &planes, &F,
+            &counts, &offsets, &err, &bbox, &q_counts, &q_off, &q_hits, &q_boxes, &q_ev, &q_keys, &q_perm,
+            &q_sbox, &q_rcounts, &q_roffsets, &q_bwork, &q_border, &q_wslot, &q_sorder, &q_aos, &zarena};
+  }
+  // The buffers a captured build + fkd_search can reference: all but the
+  // host-search staging (q_boxes, q_ev, q_off, q_hits of fkd_search_host).
+  std::vector<DevBuf*> graph_held() {
+    std::vector<DevBuf*> v;
+    for (DevBuf* b : buffers())
+      if (b != &q_boxes && b != &q_ev && b != &q_off && b != &q_hits) v.push_back(b);
+    return v;
+  }
+  // The build and search scratch fkd_trim releases.
+  std::vector<DevBuf*> scratch() {
+    return {&A, &K, &R, &planes, &F, &Xr, &counts, &offsets, &q_counts, &q_off, &q_hits, &q_boxes, &q_ev,
+            &q_keys, &q_perm, &q_sbox, &q_rcounts, &q_roffsets, &q_bwork, &q_border, &q_wslot, &q_sorder,
+            &q_aos, &zarena};
+  }
 };
 
 struct fkd_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  fkd_tree_s* tree = nullptr;  // null once the handle is destroyed
+  uint32_t nq = 0;
+  uint64_t* offsets = nullptr;
+  uint64_t capacity = 0;
   ~fkd_graph_s() {
+    if (tree) {
+      --tree->graph_count;
+      auto& l = tree->graph_list;
+      l.erase(std::remove(l.begin(), l.end(), this), l.end());
+    }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
   }
 };
+
+fkd_tree_s::fkd_tree_s() {
+  for (DevBuf* b : graph_held()) b->graphs = &graph_count;
+}
+
+fkd_tree_s::~fkd_tree_s() {
+  for (fkd_graph_s* g : graph_list) g->tree = nullptr;
+}
 
 namespace {
 
@@ -566,6 +622,9 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
   FKD_TRY(t->q_counts.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
   if (nq > 0) {
     FKD_TRY(plan_queries<DIM>(t, nq, mins, maxs, qev, s));
+    t->plan_mins = mins;
+    t->plan_maxs = maxs;
+    t->plan_qev = qev;
     fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
     const size_t n = std::max<uint32_t>(nq, 1);
     a.perm = plan_perm(t);
@@ -615,7 +674,7 @@ template <int DIM>
 int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
                      const uint64_t* qoff, uint32_t* out, uint64_t cap, hipStream_t s) {
   if (nq == 0) return FKD_OK;
-  if (t->q_plan_nq != nq)
+  if (t->q_plan_nq != nq || t->plan_mins != mins || t->plan_maxs != maxs || t->plan_qev != qev)
     return fail(FKD_ERR_STATE, "fkd_search_fill must follow fkd_search_count with the same queries");
   fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
   const size_t n = std::max<uint32_t>(nq, 1);
@@ -687,8 +746,26 @@ __global__ void check_input_kernel(const float* X, const uint32_t* ids, const ui
   if (unordered) atomicOr(err, 2u);
 }
 
+// Sets *err when two neighbours of the (event, id)-sorted ids are equal
+// inside one event.
+__global__ void duplicate_ids_kernel(const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ off,
+                                     uint32_t n_events, uint32_t P, uint32_t* err) {
+  bool dup = false;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < P; i += gridDim.x * blockDim.x) {
+    if (sorted[i] != sorted[i + 1]) continue;
+    uint32_t lo = 0, hi = n_events;  // event e with off[e] <= i < off[e+1]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (off[mid] <= i) lo = mid; else hi = mid;
+    }
+    dup |= i + 1 < off[lo + 1];
+  }
+  if (dup) atomicOr(err, 1u);
+}
+
 int validate_offsets(uint32_t n_events, const uint64_t* event_offsets, uint64_t* P) {
   if (n_events == 0) return fail(FKD_ERR_INVALID, "n_events must be >= 1");
+  if (n_events > kMaxEvents) return fail(FKD_ERR_INVALID, "more than 2^30 events in one handle");
   if (!event_offsets) return fail(FKD_ERR_INVALID, "event_offsets is NULL");
   if (event_offsets[0] != 0) return fail(FKD_ERR_INVALID, "event_offsets[0] must be 0");
   for (uint32_t e = 0; e < n_events; ++e) {
@@ -725,11 +802,46 @@ int ensure_build_scratch(fkd_tree_s* t) {
   return FKD_OK;
 }
 
+// Ids must be unique within an event: the reference comparator
+// (FKDTree_cpu.h:246-252) is a strict order only then, and the tree is a
+// function of the points only then.  Increasing ids are unique; otherwise the
+// ids of every event are sorted (the build's own id-sort passes) and
+// neighbours compared.  Synchronous.
+int check_unique_ids(fkd_tree_s* t, hipStream_t s) {
+  const uint32_t P = t->P;
+  if (P < 2) return FKD_OK;
+  const uint64_t S = P;
+  uint32_t* K = t->K.as<uint32_t>();
+  uint32_t* A = t->A.as<uint32_t>();
+  const fkd::TileMap tm = t->radix_tiles.map(t->off.as<uint32_t>());
+  ZeroArena za;
+  FKD_TRY(zero_arena(t, 4 * radix_pass_words(1, tm.T), s, za));
+  fkd::id_keys_kernel<<<grid_for(P), 256, 0, s>>>(t->ids.as<uint32_t>(), P, K, A);
+  FKD_TRY(launch_check("id_keys"));
+  int c = 0;
+  for (int pass = 0; pass < 4; ++pass) {
+    uint64_t* desc = za.take(radix_pass_words(1, tm.T));
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(desc + fkd::scan_tiles(256ull * tm.T));
+    FKD_TRY(radix_pass(tm, P, 1, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, K + c * S,
+                       K + (1 - c) * S, A + c * S, A + (1 - c) * S, S, nullptr, 8u * pass, true, false, s));
+    c = 1 - c;
+  }
+  uint32_t* flag = t->err.as<uint32_t>() + 1;
+  FKD_TRY(fill_u32(flag, 0, 1, s));
+  duplicate_ids_kernel<<<grid_for(P), 256, 0, s>>>(K + c * S, t->off.as<uint32_t>(), t->n_events, P, flag);
+  FKD_TRY(launch_check("duplicate_ids"));
+  uint32_t dup = 0;
+  FKD_HIP(hipMemcpyAsync(&dup, flag, 4, hipMemcpyDeviceToHost, s));
+  FKD_HIP(hipStreamSynchronize(s));
+  if (dup) return fail(FKD_ERR_INVALID, "duplicate point id within an event (ids must be unique)");
+  return FKD_OK;
+}
+
 int finish_create(fkd_tree_s* t, hipStream_t s) {
   const uint32_t P = t->P, E = t->n_events;
   const int dim = t->dim;
-  std::vector<uint32_t> off32(E + 1);
-  for (uint32_t e = 0; e <= E; ++e) off32[e] = (uint32_t)t->h_off[e];
+  std::vector<uint32_t> off32((size_t)E + 1);
+  for (size_t e = 0; e <= E; ++e) off32[e] = (uint32_t)t->h_off[e];
   t->Dmax = 0;
   t->global_levels = 0;
   t->fin_cap = 1;
@@ -743,8 +855,8 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
     fin_count[e] = 1u << fkd::fin_depth(D);  // subtrees the finisher builds
     t->fin_cap = std::max(t->fin_cap, fkd::fin_subtree_max((uint32_t)n));
   }
-  FKD_TRY(t->off.ensure((E + 1) * 4));
-  FKD_HIP(hipMemcpyAsync(t->off.p, off32.data(), (E + 1) * 4, hipMemcpyHostToDevice, s));
+  FKD_TRY(t->off.ensure(((size_t)E + 1) * 4));
+  FKD_HIP(hipMemcpyAsync(t->off.p, off32.data(), ((size_t)E + 1) * 4, hipMemcpyHostToDevice, s));
   FKD_TRY(t->radix_tiles.init(t->h_off, fkd::kRadixTile, s));
   FKD_TRY(t->level_tiles.init(t->h_off, fkd::kLevelTile, s));
   FKD_TRY(t->fin_tiles.init_counts(fin_count, s));
@@ -767,6 +879,7 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
   FKD_HIP(hipStreamSynchronize(s));
   if (err & 1u) return fail(FKD_ERR_NAN, "NaN coordinate in the input points");
   t->ids_in_order = (err & 2u) == 0;
+  if (!t->ids_in_order) FKD_TRY(check_unique_ids(t, s));
   return FKD_OK;
 }
 
@@ -808,7 +921,7 @@ int create_common(int dim, uint32_t n_events, const uint64_t* event_offsets, fkd
   // Several trees: starting the heaviest blocks first scatters the writes of
   // an event's boxes over the whole output (measured slower for batches).
   if (n_events > 1) t->tune.fill_order = 0;
-  t->h_off.assign(event_offsets, event_offsets + n_events + 1);
+  t->h_off.assign(event_offsets, event_offsets + (size_t)n_events + 1);
   const size_t S = std::max<uint64_t>(P, 1);
   int r = t->X.ensure(S * dim * 4);
   if (r == FKD_OK) r = t->ids.ensure(S * 4);
@@ -859,7 +972,7 @@ constexpr char kWireMagic[8] = {'F', 'K', 'D', 'T', 'R', 'E', 'E', '1'};
 constexpr uint64_t kWireHeader = 24;  // magic, uint32 dim, uint32 n_events, uint64 n_points
 
 uint64_t wire_bytes(int dim, uint32_t n_events, uint64_t P) {
-  return kWireHeader + 8ull * (n_events + 1) + 4ull * P * (uint64_t)(dim + 1);
+  return kWireHeader + 8ull * ((uint64_t)n_events + 1) + 4ull * P * (uint64_t)(dim + 1);
 }
 
 }  // namespace
@@ -881,8 +994,8 @@ int fkd_serialize(fkd_tree_t t, void* buf, uint64_t capacity, uint64_t* size, fk
   std::memcpy(b + 8, &dim, 4);
   std::memcpy(b + 12, &E, 4);
   std::memcpy(b + 16, &P, 8);
-  std::memcpy(b + kWireHeader, t->h_off.data(), 8ull * (E + 1));
-  char* data = b + kWireHeader + 8ull * (E + 1);
+  std::memcpy(b + kWireHeader, t->h_off.data(), 8ull * ((uint64_t)E + 1));
+  char* data = b + kWireHeader + 8ull * ((uint64_t)E + 1);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (P) {
     FKD_HIP(hipMemcpyAsync(data, t->Tc.p, P * dim * 4, hipMemcpyDeviceToHost, s));
@@ -904,17 +1017,17 @@ int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t str
   std::memcpy(&dim, b + 8, 4);
   std::memcpy(&E, b + 12, 4);
   std::memcpy(&P, b + 16, 8);
-  if (dim < 1 || dim > 4 || E < 1 || P > (1ull << 30)) return fail(FKD_ERR_INVALID, "bad FKDTREE1 header");
+  if (dim < 1 || dim > 4 || E < 1 || E > kMaxEvents || P > kMaxPoints) return fail(FKD_ERR_INVALID, "bad FKDTREE1 header");
   if (size != wire_bytes((int)dim, E, P))
     return fail(FKD_ERR_INVALID, "FKDTREE1 size " + std::to_string(size) + " != " +
                                      std::to_string(wire_bytes((int)dim, E, P)));
-  std::vector<uint64_t> off(E + 1);
-  std::memcpy(off.data(), b + kWireHeader, 8ull * (E + 1));
+  std::vector<uint64_t> off((size_t)E + 1);
+  std::memcpy(off.data(), b + kWireHeader, 8ull * ((uint64_t)E + 1));
   if (off[E] != P) return fail(FKD_ERR_INVALID, "FKDTREE1 offsets do not end at n_points");
   hipStream_t s = static_cast<hipStream_t>(stream);
   fkd_tree_s* t = nullptr;
   FKD_TRY(create_common((int)dim, E, off.data(), &t));
-  const char* data = b + kWireHeader + 8ull * (E + 1);
+  const char* data = b + kWireHeader + 8ull * ((uint64_t)E + 1);
   int r = FKD_OK;
   if (P) {
     hipError_t e = hipMemcpyAsync(t->X.p, data, P * dim * 4, hipMemcpyHostToDevice, s);
@@ -1180,13 +1293,24 @@ int fkd_graph_create(fkd_tree_t t, uint32_t nq, const float* mins, const float* 
     delete g;
     return rc != FKD_OK ? rc : fail(FKD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
   }
+  g->tree = t;
+  g->nq = nq;
+  g->offsets = dev_offsets;
+  g->capacity = capacity;
+  ++t->graph_count;
+  t->graph_list.push_back(g);
   *out = g;
   return FKD_OK;
 }
 
 int fkd_graph_launch(fkd_graph_t g, fkd_stream_t stream) {
   if (!g) return fail(FKD_ERR_INVALID, "graph is NULL");
+  if (!g->tree) return fail(FKD_ERR_STATE, "the graph's tree was destroyed");
   FKD_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  // fkd_search_status reports on the replayed search
+  g->tree->last_nq = g->nq;
+  g->tree->last_offsets = g->offsets;
+  g->tree->last_capacity = g->capacity;
   return FKD_OK;
 }
 
@@ -1195,11 +1319,10 @@ void fkd_graph_destroy(fkd_graph_t g) { delete g; }
 int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
   StreamScope scope(static_cast<hipStream_t>(stream));
   if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (t->graph_count)
+    return fail(FKD_ERR_STATE, "a HIP graph of this handle holds its scratch (destroy the graph first)");
   FKD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
-  for (DevBuf* b : {&t->A, &t->K, &t->R, &t->planes, &t->F, &t->Xr, &t->counts, &t->offsets, &t->q_counts, &t->q_off,
-                    &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm, &t->q_sbox, &t->q_rcounts,
-                    &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_wslot, &t->q_sorder, &t->q_aos, &t->zarena})
-    b->release();
+  for (DevBuf* b : t->scratch()) b->release();
   t->q_plan_nq = 0;
   t->last_offsets = nullptr;
   t->last_total = 0;
@@ -1209,13 +1332,7 @@ int fkd_trim(fkd_tree_t t, fkd_stream_t stream) {
 int fkd_device_bytes(fkd_tree_t t, uint64_t* bytes) {
   if (!t || !bytes) return fail(FKD_ERR_INVALID, "tree / bytes is NULL");
   uint64_t b = 0;
-  for (const DevBuf* d : {&t->off, &t->radix_tiles.tile_ev, &t->radix_tiles.tb, &t->level_tiles.tile_ev,
-                          &t->level_tiles.tb, &t->fin_tiles.tile_ev, &t->fin_tiles.tb, &t->X, &t->ids, &t->Tc, &t->Tid, &t->recs, &t->A, &t->K, &t->R, &t->Xr,
-                          &t->planes, &t->F, &t->counts, &t->offsets, &t->err, &t->bbox,
-                          &t->q_counts, &t->q_off, &t->q_hits, &t->q_boxes, &t->q_ev, &t->q_keys, &t->q_perm,
-                          &t->q_sbox, &t->q_rcounts, &t->q_roffsets, &t->q_bwork, &t->q_border, &t->q_wslot, &t->q_sorder,
-                          &t->q_aos, &t->zarena})
-    b += d->bytes;
+  for (const DevBuf* d : t->buffers()) b += d->bytes;
   *bytes = b;
   return FKD_OK;
 }

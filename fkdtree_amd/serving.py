@@ -119,6 +119,19 @@ class IngestRing:
         return self.buf[start:start + n]
 
 
+def same_kind_prefix(queue: Sequence[Request], k: int) -> int:
+    """How many of the first k queued requests arrive the way the head does
+    (FKDPoint records through the ring, or SoA arrays): a batch is built from
+    one kind only."""
+    if k <= 1:
+        return k
+    head = queue[0].records is not None
+    for i in range(1, k):
+        if (queue[i].records is not None) != head:
+            return i
+    return k
+
+
 def contiguous_prefix(queue: Sequence[Request], k: int) -> int:
     """How many of the first k queued events lie end to end in the ring."""
     if k <= 1:
@@ -425,6 +438,7 @@ class EventServer:
                     if inflight or creating:
                         break           # wait for memory held by batches in flight
                     k = 1               # nothing else holds memory: try the head alone
+                k = same_kind_prefix(queue, k)
                 if queue[0].ring_start >= 0:
                     k = contiguous_prefix(queue, k)
                 members = [queue.popleft() for _ in range(k)]
@@ -477,9 +491,23 @@ class EventServer:
         return b.handle.search(mins, maxs, qev)
 
     def close(self):
+        """Evict every resident tree, stop the workers, and release the
+        streams and the page-locked ingest ring (a server holds about
+        256 MB of page-locked memory and three streams until then)."""
         while self._evict_one():
             pass
         self.pool.shutdown(wait=True)
+        for s in self.streams:
+            s.sync()
+            s.close()
+        self.streams = []
+        self.ring = None
+
+    def __enter__(self) -> "EventServer":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
 
 
 def poisson_requests(n_events: int, rate_hz: float, size_scale: float = 1.0, seed_offset: int = 0,

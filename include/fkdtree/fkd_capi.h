@@ -18,7 +18,9 @@
  *    (P = number of points).  Query boxes are SoA too: mins[d * nq + q].
  *  - Limits: at most 2^30 points per tree / batch and per event.
  *  - Preconditions (as for the reference comparator, FKDTree_cpu.h:246-252):
- *    ids unique within a tree; coordinates not NaN (rejected at creation).
+ *    ids unique within a tree (an event of a batch); coordinates not NaN.
+ *    Creation checks both and fails with FKD_ERR_NAN / FKD_ERR_INVALID.
+ *  - At most 2^30 events per handle.
  */
 #ifndef FKDTREE_FKD_CAPI_H_
 #define FKDTREE_FKD_CAPI_H_
@@ -111,7 +113,9 @@ int fkd_serialize(fkd_tree_t tree, void* buf, uint64_t capacity, uint64_t* size,
  * and the search records derived from them (no build); the handle's points
  * are the tree's own, so fkd_build reproduces the same tree.  verify = 1
  * rebuilds from the points instead and fails with FKD_ERR_INVALID unless
- * every array comes out identical.  Synchronises `stream`. */
+ * every array comes out identical.  A buffer whose header, size or offsets
+ * disagree fails with FKD_ERR_INVALID before anything is allocated.
+ * Synchronises `stream`. */
 int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t stream, fkd_tree_t* out);
 
 /* Launch parameters of the handle (chosen by fkdtree_amd/tune.py; results
@@ -181,9 +185,14 @@ int fkd_search_status(fkd_tree_t tree, fkd_stream_t stream, uint64_t* total, int
  * Runs the sequence eagerly first (so every internal buffer has its final
  * size; FKD_ERR_CAPACITY when `capacity` is below the hits), then captures
  * it on `stream` (a created stream).  fkd_graph_launch replays it: a full
- * rebuild of every tree of the handle and the search.  The boxes, outputs and
- * the tree must stay alive while the graph exists.  No reference
- * counterpart (the reference launches kernel by kernel). */
+ * rebuild of every tree of the handle and the search, after which
+ * fkd_search_status reports on the replayed search.  The boxes and outputs
+ * must stay alive while the graph exists.  The graph holds the handle's
+ * build and search buffers: while it exists, fkd_trim and any call that
+ * would need larger buffers (a build or search of more boxes) fail with
+ * FKD_ERR_STATE; smaller eager searches work.  Destroying the tree first
+ * detaches its graphs: their fkd_graph_launch fails with FKD_ERR_STATE.
+ * No reference counterpart (the reference launches kernel by kernel). */
 int fkd_graph_create(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
                      const uint32_t* dev_query_events, uint64_t* dev_offsets, uint32_t* dev_ids_out,
                      uint64_t capacity, fkd_stream_t stream, fkd_graph_t* out);
@@ -194,7 +203,10 @@ void fkd_graph_destroy(fkd_graph_t graph);
  * dev_ids_out[dev_offsets[q] .. dev_offsets[q+1]) (slots >= capacity are
  * dropped; check dev_offsets[nq] <= capacity).  Within a query the order is
  * unspecified.  Must follow fkd_search_count on the same tree with the same
- * boxes (it reuses the processing order computed there).  Enqueues only. */
+ * boxes (it reuses the processing order computed there): a fill whose box
+ * pointers, event pointer or count differ from the last count's, or that
+ * follows a build or another search of the handle, fails with
+ * FKD_ERR_STATE.  Enqueues only. */
 int fkd_search_fill(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
                     const uint32_t* dev_query_events, const uint64_t* dev_offsets,
                     uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream);

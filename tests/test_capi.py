@@ -69,3 +69,40 @@ def test_deserialize_rejects_bad_buffers():
         buf = np.frombuffer(data + b"\0", dtype=np.uint8)
         rc = lib().fkd_deserialize(buf.ctypes.data, len(data), 0, None, C.byref(h))
         assert rc == -1 and h.value is None
+
+
+def _header(dim, n_events, n_points):
+    import numpy as np
+
+    return (b"FKDTREE1" + np.array([dim, n_events], np.uint32).tobytes() +
+            np.array([n_points], np.uint64).tobytes())
+
+
+@pytest.mark.parametrize("n_events,n_points,tail", [
+    (0xFFFFFFFF, 0, 0),            # n_events + 1 wraps to 0 in 32 bits
+    (0xFFFFFFFF, 0, 16),
+    ((1 << 30) + 1, 0, 0),         # above the event limit
+    (0xFFFFFFFE, 0, 8),            # a size check that wraps would accept 32 bytes
+    (2, 5, 24),                    # truncated: offsets present, arrays missing
+    (1, 1 << 31, 16),              # more points than a handle holds
+])
+def test_deserialize_rejects_wrapped_and_truncated_headers(n_events, n_points, tail):
+    """Malformed wire headers fail with FKD_ERR_INVALID before any
+    allocation, indexing or device call (no GPU needed)."""
+    import numpy as np
+
+    from fkdtree_amd._lib import lib
+
+    data = _header(3, n_events, n_points) + bytes(tail)
+    buf = np.frombuffer(data + b"\0" * 8, dtype=np.uint8)
+    h = C.c_void_p()
+    rc = lib().fkd_deserialize(buf.ctypes.data, len(data), 0, None, C.byref(h))
+    assert rc == -1 and h.value is None, lib().fkd_last_error()
+
+
+def test_create_rejects_too_many_events():
+    L = _lib.lib()
+    h = C.c_void_p()
+    offs = (C.c_uint64 * 1)(0)
+    assert L.fkd_create_device(3, (1 << 30) + 1, offs, None, None, None, C.byref(h)) == -1
+    assert b"events" in L.fkd_last_error()

@@ -111,6 +111,16 @@ def test_ring_ingestion_equals_soa(require_gpu):
         x, y = ra.results[r.event], rb.results[r.event]
         assert np.array_equal(x.offsets, y.offsets), r.event
         assert np.array_equal(sorted_sets(x.offsets, x.ids), sorted_sets(y.offsets, y.ids)), r.event
+    # ring-ingested results against the oracle (FKDTree_cpu.h:123-169) for a
+    # sample of events, the largest included
+    sample = sorted(soa, key=lambda r: -r.n)[:3] + soa[::9]
+    for r in sample:
+        tc, ti = oracle.build(r.coords, r.ids)
+        bx = gen.boxes_around(r.coords, 0.01)
+        ooff, oids = oracle.search(tc, ti, bx.mins, bx.maxs)
+        y = rb.results[r.event]
+        assert np.array_equal(y.offsets, ooff), r.event
+        assert np.array_equal(sorted_sets(y.offsets, y.ids), sorted_sets(ooff, oids)), r.event
     assert b.ring.used == 0
     b.close()
     # without keeping results only the event boundaries come back; totals agree
@@ -119,3 +129,60 @@ def test_ring_ingestion_equals_soa(require_gpu):
     for r in soa:
         assert rc.results[r.event].hits == int(ra.results[r.event].offsets[-1])
     c.close()
+
+
+def test_serving_full_size_events(require_gpu):
+    """The serving workload at its stated sizes: 48 events of 1,000-500,000
+    points (log-uniform, the uniform-plus-blob mix) arriving at 5,000 events/s
+    as FKDPoint records through the ingest ring.  Every event's result in the
+    mix equals its result alone, and eight events (the four largest and four
+    others) equal the oracle's exact id sets."""
+    soa = poisson_requests(48, rate_hz=5000.0)
+    rec = poisson_requests(48, rate_hz=5000.0, records=True)
+    pts = sum(r.n for r in rec)
+    print(f"[serve-full] events=48 points={pts} largest={max(r.n for r in rec)}", flush=True)
+    assert min(r.n for r in rec) >= 1000 and max(r.n for r in rec) <= 500_000
+    with EventServer(n_streams=3, max_batch_points=4_000_000, max_batch_events=256, keep_results=True,
+                     ring_points=max(16_000_000, 2 * max(r.n for r in rec))) as srv:
+        rep = srv.run(rec)
+        print(rep.summary(), flush=True)
+        assert len(rep.results) == 48
+        assert max(rep.batch_sizes) > 1           # events shared batches
+        for r in soa:
+            got = rep.results[r.event]
+            off, ids = alone(r)
+            assert np.array_equal(got.offsets, off), r.event
+            assert np.array_equal(sorted_sets(got.offsets, got.ids), sorted_sets(off, ids)), r.event
+        by_size = sorted(soa, key=lambda r: -r.n)
+        for r in by_size[:4] + by_size[10::10][:4]:
+            tc, ti = oracle.build(r.coords, r.ids)
+            b = gen.boxes_around(r.coords, 0.01)
+            ooff, oids = oracle.search(tc, ti, b.mins, b.maxs)
+            got = rep.results[r.event]
+            assert np.array_equal(got.offsets, ooff), r.event
+            assert np.array_equal(sorted_sets(got.offsets, got.ids), sorted_sets(ooff, oids)), r.event
+        print(f"[serve-full] hits={sum(r.hits for r in rep.results.values())}", flush=True)
+
+
+def test_servers_in_turn_release_resources(require_gpu):
+    """close() destroys a server's streams and frees its page-locked ingest
+    ring: forty servers created and closed in turn (each with the 256 MB ring
+    the benchmark uses) serve at the same rate at the end as at the start."""
+    import time
+
+    rec = poisson_requests(24, rate_hz=20_000.0, size_scale=0.05, records=True)
+    rates = []
+    for k in range(40):
+        with EventServer(n_streams=3, max_batch_points=400_000, max_batch_events=64,
+                         ring_points=16_000_000) as srv:
+            t0 = time.perf_counter()
+            rep = srv.run(rec)
+            dt = time.perf_counter() - t0
+        assert len(rep.results) == 24
+        rates.append(24 / dt)
+        if k % 10 == 9:
+            print(f"[servers] {k + 1} servers: last events/s {rates[-1]:.0f}", flush=True)
+    first = float(np.median(rates[1:6]))
+    last = float(np.median(rates[-5:]))
+    print(f"[servers] first {first:.0f} last {last:.0f} events/s", flush=True)
+    assert last * 1.5 >= first, (first, last, rates)

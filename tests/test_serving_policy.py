@@ -66,3 +66,23 @@ def test_contiguous_prefix_stops_at_the_wrap():
     assert contiguous_prefix(q, 2) == 2
     assert contiguous_prefix(q[3:], 2) == 2
     assert contiguous_prefix(q, 1) == 1
+
+
+def test_batches_hold_one_request_kind():
+    """A queue mixing SoA requests and FKDPoint-record requests is cut where
+    the kind changes, so a batch never mixes them."""
+    from fkdtree_amd.serving import Request, contiguous_prefix, same_kind_prefix
+
+    def rec(start, n):
+        r = Request(0, None, None, 0.0, records=_rec(n, 0))
+        r.ring_start = start
+        return r
+
+    soa = reqs([5, 6])
+    q = [soa[0], rec(0, 3), soa[1]]
+    assert same_kind_prefix(q, 3) == 1
+    q = [rec(0, 3), rec(3, 2), soa[0], rec(5, 1)]
+    assert same_kind_prefix(q, 4) == 2
+    assert contiguous_prefix(q, same_kind_prefix(q, 4)) == 2
+    q = soa + reqs([7])
+    assert same_kind_prefix(q, 3) == 3
