@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -968,6 +969,76 @@ int adopt_tree_impl(fkd_tree_s* t, hipStream_t s) {
   return FKD_OK;
 }
 
+// Scratch of fkd_search_noalloc: a file-scope device array, one copy per GPU,
+// held until the process ends.  It holds the zeroed look-back descriptors
+// of the offsets scan, then the per-box hit counts.  Host code never uses
+// the symbol itself as an address: noalloc_scratch() looks up each device's
+// copy once (hipGetSymbolAddress) and every kernel argument, fill and copy
+// uses that pointer.
+constexpr uint32_t kNoAllocMaxBoxes = 1u << 22;
+// descriptor words, rounded up to 256 bytes: the scan reads the counts that
+// follow as 16-byte vectors
+constexpr uint64_t kNoAllocDescWords =
+    ((kNoAllocMaxBoxes + 1 + fkd::kScanTile - 1) / fkd::kScanTile + fkd::kCtrWords + 31) & ~uint64_t(31);
+static_assert(kNoAllocDescWords * 8 % 256 == 0, "counts start 256-byte aligned");
+constexpr uint64_t kNoAllocBytes = kNoAllocDescWords * 8 + (uint64_t)kNoAllocMaxBoxes * 4;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (looked up on the first call on
+// each device, with that device current).
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  FKD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(FKD_ERR_INVALID, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    FKD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(FKD_ERR_HIP, "hipGetSymbolAddress returned no address for the search scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FKD_OK;
+}
+
+// fkd_search without the spatial box plan: count pass in the caller's box
+// order, scan, fill pass.  Scratch comes from `scr` (kNoAllocBytes); nothing
+// is allocated.
+template <int DIM>
+int search_noalloc_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                        uint64_t* qoff, uint32_t* out, uint64_t cap, void* scr, hipStream_t s) {
+  const uint64_t st = fkd::scan_tiles(nq);
+  uint64_t* desc = static_cast<uint64_t*>(scr);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(desc + kNoAllocDescWords);
+  FKD_TRY(fill_u64(desc, 0, st + fkd::kCtrWords, s));  // a kernel: no memset node under capture
+  fkd::SearchArgs a = search_args<DIM>(t, nq, qev);
+  a.mins = mins;
+  a.maxs = maxs;
+  if (nq > 0) {
+    a.counts = counts;
+    FKD_TRY((launch_search<DIM, fkd::kCount>(t, a, s, "search_count_noalloc")));
+  }
+  fkd::scan_exclusive_kernel<uint64_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
+      counts, qoff, nq, desc, reinterpret_cast<uint32_t*>(desc + st));
+  FKD_TRY(launch_check("search_scan_noalloc"));
+  if (nq > 0) {
+    a.counts = nullptr;
+    a.qoff = qoff;
+    a.out = out;
+    a.cap = cap;
+    a.out_aligned = (reinterpret_cast<uintptr_t>(out) & 63u) == 0;
+    FKD_TRY((launch_search<DIM, fkd::kFill>(t, a, s, "search_fill_noalloc")));
+  }
+  t->last_nq = nq;
+  t->last_capacity = cap;
+  t->last_offsets = qoff;
+  return FKD_OK;
+}
+
 constexpr char kWireMagic[8] = {'F', 'K', 'D', 'T', 'R', 'E', 'E', '1'};
 constexpr uint64_t kWireHeader = 24;  // magic, uint32 dim, uint32 n_events, uint64 n_points
 
@@ -1238,6 +1309,25 @@ int fkd_trace_events(fkd_tree_t t, fkd_event_t start, fkd_event_t stop) {
   t->trace_start = static_cast<hipEvent_t>(start);
   t->trace_stop = static_cast<hipEvent_t>(stop);
   return FKD_OK;
+}
+
+int fkd_search_noalloc(fkd_tree_t t, uint32_t nq, const float* mins, const float* maxs, const uint32_t* qev,
+                       uint64_t* dev_offsets, uint32_t* dev_ids_out, uint64_t capacity, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  if (!dev_offsets || (nq && (!mins || !maxs))) return fail(FKD_ERR_INVALID, "NULL search buffer");
+  if (!qev && t->n_events > 1) return fail(FKD_ERR_INVALID, "query events required for a batch of trees");
+  if (capacity && !dev_ids_out) return fail(FKD_ERR_INVALID, "dev_ids_out is NULL");
+  if (nq > kNoAllocMaxBoxes)
+    return fail(FKD_ERR_INVALID, "fkd_search_noalloc takes at most " + std::to_string(kNoAllocMaxBoxes) +
+                                     " boxes per call (got " + std::to_string(nq) + ")");
+  void* scr = nullptr;
+  FKD_TRY(noalloc_scratch(&scr));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dispatch_dim(t->dim, [&](auto D) {
+    return search_noalloc_impl<decltype(D)::value>(t, nq, mins, maxs, qev, dev_offsets, dev_ids_out, capacity,
+                                                   scr, s);
+  });
 }
 
 int fkd_search_status(fkd_tree_t t, fkd_stream_t stream, uint64_t* total, int* complete) {

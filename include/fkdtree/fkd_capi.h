@@ -168,6 +168,23 @@ int fkd_search(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float*
                const uint32_t* dev_query_events, uint64_t* dev_offsets, uint32_t* dev_ids_out,
                uint64_t capacity, fkd_stream_t stream);
 
+/* The search of fkd_search (same arguments, same offsets and id sets) for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  No reference
+ * counterpart.  It calls no hipMalloc / hipFree: its scratch (the per-box
+ * counts and the zeroed descriptors of the offsets scan) is a fixed device
+ * array of the library, 16.1 MiB on each GPU it runs on, held until the
+ * process ends; so it takes at most 4,194,304 boxes per call (more fail with
+ * FKD_ERR_INVALID) and walks the boxes in the caller's order (no spatial
+ * plan, which needs per-call buffers; sort boxes spatially for speed).  The
+ * tree must be built.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams, other threads, other
+ * handles) race and must be ordered by the caller.  Ids beyond `capacity`
+ * are dropped; fkd_search_status reports the total.  Enqueues only. */
+int fkd_search_noalloc(fkd_tree_t tree, uint32_t nq, const float* dev_mins, const float* dev_maxs,
+                       const uint32_t* dev_query_events, uint64_t* dev_offsets, uint32_t* dev_ids_out,
+                       uint64_t capacity, fkd_stream_t stream);
+
 /* Boxes [c - half_width, c + half_width] around every point of the handle,
  * in the order the points were given (SoA [dim][n_points], one float
  * rounding per bound), and the event of every box (dev_query_events may be
