@@ -39,8 +39,10 @@ def parse():
     ap.add_argument("--workload", default="mixed_1m", choices=["mixed_1m", "uniform_100k", "uniform_4m"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--events", type=int, default=256,
+    ap.add_argument("--events", type=int, default=2048,
                     help="events of the batched workload (first N of the 2,048-event set; 0 skips)")
+    ap.add_argument("--cpu-events", type=int, default=256,
+                    help="events of the batched CPU baseline (a bounded sample: the first N events)")
     ap.add_argument("--event-steps", type=int, default=3)
     ap.add_argument("--serve-events", type=int, default=512,
                     help="events of the serving stream (0 skips)")
@@ -48,7 +50,9 @@ def parse():
     ap.add_argument("--events-only", action="store_true",
                     help="time only the batched events (build + search per batch) and exit; for profiling")
     ap.add_argument("--calibrate", action="store_true",
-                    help="also run a 256 MiB copy kernel (counter calibration) and exit")
+                    help="run a plain copy kernel (counter calibration) three times and exit")
+    ap.add_argument("--calib-mib", type=int, default=2048,
+                    help="MiB read and written by one calibration copy (well past the 256 MB MALL)")
     return ap.parse_args()
 
 
@@ -73,7 +77,7 @@ def main():
     info = F.device_info()
     if args.calibrate:
         from fkdtree_amd._lib import check, lib
-        n = 64 << 20
+        n = args.calib_mib << 18
         a, b = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32)
         s = Stream()
         for _ in range(3):
@@ -160,6 +164,10 @@ def main():
     build_bytes = 36 * n * D
     achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.workload, "fkd::search_kernel<3, 1, true>")
+    # HBM bytes the counters saw per fill launch (calibrated FETCH_SIZE +
+    # WRITE_SIZE of the committed profile) over the launch time measured here
+    counter_gbs = traffic / (fill_ms * 1e-3) / 1e9 if traffic else None
+    search_gbs = search_bytes / (search_ms * 1e-3) / 1e9
 
     res = {
         "metric": "box queries/s + builds/s at 1M 3D pts; batched vs plain events/s; p99 ms",
@@ -188,7 +196,14 @@ def main():
         "build_algorithmic_GBs": build_bytes / (build_ms * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": "search_kernel<3,kFill> (fill traversal)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": traffic, "algorithmic_bytes": fill_bytes},
+                     "traffic": traffic, "algorithmic_bytes": fill_bytes,
+                     # the same launch by its counted HBM bytes: node re-reads hit L2 / MALL
+                     "hbm_counter_GBs": counter_gbs,
+                     "frac_counter": counter_gbs / PEAK_HBM_GBS if counter_gbs else None,
+                     # the whole timed search (plan + count + scan + fill) by the same model
+                     "search_model_GBs": search_gbs, "search_model_frac": search_gbs / PEAK_HBM_GBS,
+                     # what limits the traversal (SQ counters, profiles/kernels_mixed_1m.md)
+                     "limiter": "VALU issue of the dependent node walk (latency/issue bound, not HBM)"},
         "device": info["name"],
     }
     tree.close()
@@ -198,7 +213,7 @@ def main():
     if args.serve_events:
         res["serving"] = bench_serving(args.serve_events, [float(r) for r in args.serve_rates.split(",")])
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads, args.events)
+        res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads, args.cpu_events if args.events else 0)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -4,6 +4,7 @@
     python -m fkdtree_amd.profile_summary counter DIR [DIR...]  # PMC per kernel
     python -m fkdtree_amd.profile_summary traffic --fetch D --write D \
         --calib-fetch D --calib-write D --calib-bytes B --out F.json
+    python -m fkdtree_amd.profile_summary gaps DIR --split KERNEL  # idle time per batch
 
 `traffic` scales FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of every kernel
 by the ratio true bytes / counted bytes of a plain copy kernel of known size
@@ -48,6 +49,40 @@ def counters(path: str):
     return {key: sum(v) / len(v) for key, v in agg.items()}
 
 
+def dispatches(path: str):
+    """[(kernel, start_ns, end_ns)] of every kernel dispatch, by start time."""
+    rows = _db(path).execute("select name, start, end from kernels order by start")
+    return [(short(n), int(a), int(b)) for n, a, b in rows]
+
+
+def batch_gaps(path: str, split: str):
+    """Per batch of dispatches (a batch starts at each dispatch of `split`):
+    (span_us, busy_us, idle_us, dispatches), where busy is the union of the
+    batch's kernel intervals and idle the rest of its span up to the next
+    batch's first dispatch (or its own last end)."""
+    ds = dispatches(path)
+    starts = [i for i, d in enumerate(ds) if d[0] == split]
+    out = []
+    for j, i0 in enumerate(starts):
+        i1 = starts[j + 1] if j + 1 < len(starts) else len(ds)
+        seg = ds[i0:i1]
+        t0 = seg[0][1]
+        t1 = ds[i1][1] if i1 < len(ds) else max(d[2] for d in seg)
+        busy, cur_a, cur_b = 0, None, None
+        for _, a, b in seg:
+            b = min(b, t1)
+            if cur_b is None or a > cur_b:
+                if cur_b is not None:
+                    busy += cur_b - cur_a
+                cur_a, cur_b = a, b
+            else:
+                cur_b = max(cur_b, b)
+        if cur_b is not None:
+            busy += cur_b - cur_a
+        out.append(((t1 - t0) / 1e3, busy / 1e3, (t1 - t0 - busy) / 1e3, len(seg)))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -62,11 +97,18 @@ def main(argv=None):
     tr.add_argument("--calib-write", required=True)
     tr.add_argument("--calib-bytes", type=float, required=True)
     tr.add_argument("--out", required=True)
+    g = sub.add_parser("gaps")
+    g.add_argument("dir")
+    g.add_argument("--split", required=True, help="kernel whose dispatches start the batches")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         print(f"{'kernel':60s} {'calls':>6s} {'avg_us':>10s} {'total_ms':>9s} {'pct':>6s}")
         for k, n, avg, tot, pct in kernel_stats(a.dir):
             print(f"{k[:60]:60s} {n:6d} {avg:10.1f} {tot:9.3f} {pct:6.1f}")
+    elif a.cmd == "gaps":
+        print(f"{'batch':>5s} {'dispatches':>10s} {'span_us':>10s} {'busy_us':>10s} {'idle_us':>9s} {'idle%':>6s}")
+        for j, (span, busy, idle, n) in enumerate(batch_gaps(a.dir, a.split)):
+            print(f"{j:5d} {n:10d} {span:10.1f} {busy:10.1f} {idle:9.1f} {100 * idle / span:6.2f}")
     elif a.cmd == "counter":
         for d in a.dirs:
             for (k, cn), v in sorted(counters(d).items()):
