@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--serve-events", type=int, default=512,
                     help="events of the serving stream (0 skips)")
     ap.add_argument("--serve-rates", default="1000,5000,20000", help="Poisson arrival rates (events/s)")
+    ap.add_argument("--per-box-threads", default="16,256",
+                    help="host thread counts of the per-box search pattern (empty or 0 skips)")
     ap.add_argument("--events-only", action="store_true",
                     help="time only the batched events (build + search per batch) and exit; for profiling")
     ap.add_argument("--calibrate", action="store_true",
@@ -150,6 +152,10 @@ def main():
         return step_ms, wall, kms
 
     search_ms, wall_ms, fill_ms = timed(two_pass, True)
+    per_box = None
+    threads = [int(x) for x in str(args.per_box_threads).split(",") if x.strip() and int(x) > 0]
+    if threads:
+        per_box = bench_per_box(pts, boxes, d_off.download(stream), d_out, threads)
 
     V = int(visited.sum())
     H = total
@@ -206,6 +212,8 @@ def main():
                      "limiter": "VALU issue of the dependent node walk (latency/issue bound, not HBM)"},
         "device": info["name"],
     }
+    if per_box:
+        res["per_box"] = per_box
     tree.close()
     del d_out
     if args.events:
@@ -216,6 +224,48 @@ def main():
         res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads, args.cpu_events if args.events else 0)
     print(json.dumps(res), flush=True)
     return 0
+
+
+def bench_per_box(pts, boxes, off, d_out, thread_counts) -> list:
+    """The reference driver's per-box pattern (main.cpp:351-359, :372): host
+    threads call FKDTree_HIP::search_in_the_box once per box of the workload
+    (tests/cpp/fkdtree_perbox.cpp; concurrent calls are coalesced into shared
+    launches).  Every result's size is checked against the batched search's
+    offsets and the first 1,000 id sets against its ids."""
+    import subprocess
+    import tempfile
+
+    from fkdtree_amd import build_lib
+    if not os.path.exists(build_lib.CPP_PERBOX_BIN):
+        return [{"skipped": "tests/cpp/fkdtree_perbox.bin not built"}]
+    k = 1000
+    ids = d_out.download(count=int(off[k]))
+    sorted_ids = np.concatenate([np.sort(ids[int(off[i]):int(off[i + 1])]) for i in range(k)])
+    with tempfile.TemporaryDirectory() as d:
+        fin = os.path.join(d, "in.bin")
+        with open(fin, "wb") as f:
+            f.write(np.array([pts.n, boxes.q], np.uint32).tobytes())
+            f.write(np.ascontiguousarray(pts.coords.T).tobytes())
+            f.write(pts.ids.tobytes())
+            f.write(np.ascontiguousarray(boxes.mins.T).tobytes())
+            f.write(np.ascontiguousarray(boxes.maxs.T).tobytes())
+            f.write(np.diff(off).astype(np.uint32).tobytes())
+            f.write(np.array([k], np.uint32).tobytes())
+            f.write(np.ascontiguousarray(off[:k + 1], dtype=np.uint64).tobytes())
+            f.write(sorted_ids.astype(np.uint32).tobytes())
+        out = []
+        for threads in thread_counts:
+            r = subprocess.run([build_lib.CPP_PERBOX_BIN, fin, str(threads)], capture_output=True, text=True,
+                               timeout=600)
+            if r.returncode != 0:
+                out.append({"threads": threads, "failed": r.returncode, "stderr": r.stderr[-500:],
+                            "stdout": r.stdout[-500:]})
+                continue
+            res = json.loads(r.stdout.strip().splitlines()[-1])
+            res["unit"] = "box queries/s"
+            res["pattern"] = "search_in_the_box per box from host threads (main.cpp:372), FKDTree_HIP"
+            out.append(res)
+    return out
 
 
 def bench_serving(n_events: int, rates) -> list:

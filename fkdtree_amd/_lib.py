@@ -40,6 +40,7 @@ _SIGS = {
     "fkd_search_stats": (I32, [P, U32, P, P, P, P, P, P]),
     "fkd_search": (I32, [P, U32, P, P, P, P, P, U64, P]),
     "fkd_search_noalloc": (I32, [P, U32, P, P, P, P, P, U64, P]),
+    "fkd_search_small": (I32, [P, U32, P, U32, P, P, P]),
     "fkd_search_status": (I32, [P, P, C.POINTER(U64), C.POINTER(I32)]),
     "fkd_trace_events": (I32, [P, P, P]),
     "fkd_graph_create": (I32, [P, U32, P, P, P, P, P, U64, P, C.POINTER(P)]),

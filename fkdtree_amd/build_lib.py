@@ -46,17 +46,21 @@ def build(verbose: bool = False) -> str:
 
 CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "fkdtree_hip_test.cpp")
 CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "fkdtree_hip_test.bin")
+CPP_PERBOX_SRC = os.path.join(ROOT, "tests", "cpp", "fkdtree_perbox.cpp")
+CPP_PERBOX_BIN = os.path.join(ROOT, "tests", "cpp", "fkdtree_perbox.bin")
 
 
 def build_cpp_test(verbose: bool = False) -> str:
-    """The C++ drop-in test driver (host code only: g++ against the C ABI)."""
-    cmd = ["g++", "-std=c++14", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
-           "-L", HERE, "-lfkdtree_hip", "-Wl,-rpath,$ORIGIN/../../fkdtree_amd", "-pthread", "-o", CPP_TEST_BIN]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("g++ failed:\n" + res.stdout + res.stderr)
+    """The C++ drop-in drivers (host code only: g++ against the C ABI): the
+    parity driver and the per-box search driver."""
+    for src, out in ((CPP_TEST_SRC, CPP_TEST_BIN), (CPP_PERBOX_SRC, CPP_PERBOX_BIN)):
+        cmd = ["g++", "-std=c++14", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+               "-L", HERE, "-lfkdtree_hip", "-Wl,-rpath,$ORIGIN/../../fkdtree_amd", "-pthread", "-o", out]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("g++ failed:\n" + res.stdout + res.stderr)
     return CPP_TEST_BIN
 
 

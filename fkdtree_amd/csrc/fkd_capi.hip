@@ -119,6 +119,28 @@ struct DevBuf {
   }
 };
 
+// Page-locked host memory the kernels read and write directly (coherent,
+// mapped): the staging of fkd_search_small.
+struct PinnedBuf {
+  void* h = nullptr;  // host address
+  void* d = nullptr;  // the same memory as the device addresses it
+  size_t bytes = 0;
+  ~PinnedBuf() {
+    if (h) (void)hipHostFree(h);
+  }
+  int ensure(size_t want) {
+    if (want <= bytes) return FKD_OK;
+    if (h) FKD_HIP(hipHostFree(h));
+    h = d = nullptr;
+    bytes = 0;
+    want = std::max<size_t>(want, 4096);
+    FKD_HIP(hipHostMalloc(&h, want, hipHostMallocCoherent | hipHostMallocMapped));
+    FKD_HIP(hipHostGetDevicePointer(&d, h, 0));
+    bytes = want;
+    return FKD_OK;
+  }
+};
+
 int launch_check(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(FKD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -242,6 +264,7 @@ struct fkd_tree_s {
   const float* plan_mins = nullptr;
   const float* plan_maxs = nullptr;
   const uint32_t* plan_qev = nullptr;
+  PinnedBuf small_in, small_out;         // staging of fkd_search_small
   uint32_t graph_count = 0;              // live graphs captured on this handle
   std::vector<fkd_graph_s*> graph_list;  // ... detached when the handle goes first
   ~fkd_tree_s();
@@ -1328,6 +1351,49 @@ int fkd_search_noalloc(fkd_tree_t t, uint32_t nq, const float* mins, const float
     return search_noalloc_impl<decltype(D)::value>(t, nq, mins, maxs, qev, dev_offsets, dev_ids_out, capacity,
                                                    scr, s);
   });
+}
+
+int fkd_search_small(fkd_tree_t t, uint32_t nb, const float* host_boxes, uint32_t slot, uint32_t* host_counts,
+                     uint32_t* host_ids, fkd_stream_t stream) {
+  if (!t) return fail(FKD_ERR_INVALID, "tree is NULL");
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  if (t->n_events != 1) return fail(FKD_ERR_INVALID, "fkd_search_small searches a single tree (use fkd_search)");
+  if (nb > FKD_SMALL_MAX_BOXES)
+    return fail(FKD_ERR_INVALID, "fkd_search_small takes at most " + std::to_string(FKD_SMALL_MAX_BOXES) + " boxes");
+  if (slot == 0 || slot > FKD_SMALL_MAX_SLOT) return fail(FKD_ERR_INVALID, "slot must be 1..2^20 ids");
+  if ((uint64_t)nb * slot > (1ull << 28)) return fail(FKD_ERR_INVALID, "nb * slot above 2^28 ids");
+  if (nb && (!host_boxes || !host_counts || !host_ids)) return fail(FKD_ERR_INVALID, "NULL box or result buffer");
+  if (nb == 0) return FKD_OK;
+  const int dim = t->dim;
+  const size_t in_bytes = (size_t)nb * 2 * dim * 4;
+  FKD_TRY(t->small_in.ensure(in_bytes));
+  FKD_TRY(t->small_out.ensure(((size_t)nb + (size_t)nb * slot) * 4));
+  std::memcpy(t->small_in.h, host_boxes, in_bytes);
+  uint32_t* d_counts = static_cast<uint32_t*>(t->small_out.d);
+  uint32_t* d_ids = d_counts + nb;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FKD_TRY(dispatch_dim(dim, [&](auto D) {
+    constexpr int DIM = decltype(D)::value;
+    const uint64_t bytes = (uint64_t)t->P * fkd::NodeRec<DIM>::kFloats * 4;
+    const float* boxes = static_cast<const float*>(t->small_in.d);
+    if (bytes < (1ull << 32))
+      fkd::box_wave_kernel<DIM, true><<<nb, fkd::kWave, 0, s>>>(t->recs.as<float>(), (uint32_t)bytes, t->P, boxes,
+                                                                nb, slot, d_counts, d_ids);
+    else
+      fkd::box_wave_kernel<DIM, false><<<nb, fkd::kWave, 0, s>>>(t->recs.as<float>(), 0u, t->P, boxes, nb, slot,
+                                                                 d_counts, d_ids);
+    return launch_check("box_wave");
+  }));
+  FKD_HIP(hipStreamSynchronize(s));
+  const uint32_t* h_counts = static_cast<const uint32_t*>(t->small_out.h);
+  const uint32_t* h_ids = h_counts + nb;
+  std::memcpy(host_counts, h_counts, (size_t)nb * 4);
+  for (uint32_t b = 0; b < nb; ++b) {
+    const uint32_t c = h_counts[b];
+    if (c != fkd::kSmallOverflow && c) std::memcpy(host_ids + (size_t)b * slot, h_ids + (size_t)b * slot,
+                                                   (size_t)std::min(c, slot) * 4);
+  }
+  return FKD_OK;
 }
 
 int fkd_search_status(fkd_tree_t t, fkd_stream_t stream, uint64_t* total, int* complete) {

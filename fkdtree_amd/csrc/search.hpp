@@ -236,6 +236,124 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   if (MODE == kStats) a.visited[q] = vis;
 }
 
+// Low-latency search of a few boxes: one wave per box, breadth first.
+//
+// A lane-per-box walk is a chain of ~ visited-nodes dependent loads (280 on
+// 1M mixed points), hundreds of microseconds on a GPU, so a caller that
+// issues one box at a time (the reference's per-box search_in_the_box,
+// main.cpp:351-359) needs another shape: the 64 lanes of a wave take the
+// nodes of one depth of the box's frontier together, as the reference's
+// level-synchronous FQueue walk does (FKDTree_cpu.h:126-167), so the chain
+// is one load per depth.  The visited set and the reported ids are the
+// reference's (push rule FKDTree_cpu.h:150-163, closed box :145-148).  Hits
+// go to ids[b * slot + i] for i < slot; counts[b] is the true number of hits
+// (so counts[b] > slot means truncated), or kSmallOverflow when a depth's
+// frontier exceeds kWaveFrontier nodes.  Boxes are read as [nb][2 * DIM]
+// (mins, then maxs) and the outputs written straight to page-locked host
+// memory: no copies before or after the launch.
+// The records of the top kTopLevels depths (the first 2^kTopLevels - 1
+// nodes) are first copied to LDS with independent, coalesced loads, so the
+// walk's dependent global loads start below them.
+constexpr uint32_t kWaveFrontier = 2048;
+constexpr uint32_t kSmallOverflow = 0xffffffffu;
+constexpr uint32_t kTopLevels = 10;
+constexpr uint32_t kTopNodes = (1u << kTopLevels) - 1u;
+
+template <int DIM, bool BUF>
+__global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint32_t node_bytes, uint32_t n,
+                                                         const float* boxes, uint32_t nb, uint32_t slot,
+                                                         uint32_t* counts, uint32_t* ids) {
+  constexpr int RF = NodeRec<DIM>::kFloats;
+  __shared__ uint32_t fr[2][kWaveFrontier];
+  __shared__ __align__(16) float top[kTopNodes * RF];
+  const uint32_t b = blockIdx.x;
+  if (b >= nb) return;
+  const uint32_t lane = lane_id();
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t ntop = n < kTopNodes ? n : kTopNodes;
+  for (uint32_t i = lane; i < ntop * (uint32_t)(RF / 4); i += kWave)
+    reinterpret_cast<float4*>(top)[i] = reinterpret_cast<const float4*>(recs)[i];
+  float mn[DIM], mx[DIM];
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+    mn[m] = boxes[(uint64_t)b * 2 * DIM + m];
+    mx[m] = boxes[(uint64_t)b * 2 * DIM + DIM + m];
+    if (mn[m] != mn[m]) {  // a NaN lower bound: as in search_kernel
+      mn[m] = -__builtin_huge_valf();
+      mx[m] = __builtin_nanf("");
+    }
+  }
+  NodeSource src;
+  src.flat = recs;
+  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)node_bytes, 0x00020000);
+  const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
+  uint32_t nf = n ? 1u : 0u, cur = 0, hits = 0, depth = 0;
+  bool overflow = false;
+  if (lane == 0) fr[0][0] = 0;
+  __syncthreads();
+  while (nf) {
+    // the box rotated like this depth's records: slot j is dimension (depth + j) % DIM
+    const uint32_t dd = depth % (uint32_t)DIM;
+    float lo[DIM], hi[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+      uint32_t m = dd + (uint32_t)j;
+      m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
+      lo[j] = mn[0];
+      hi[j] = mx[0];
+#pragma unroll
+      for (int k = 1; k < DIM; ++k)
+        if (m == (uint32_t)k) lo[j] = mn[k], hi[j] = mx[k];
+    }
+    uint32_t nn = 0;
+    for (uint32_t base = 0; base < nf; base += kWave) {
+      const uint32_t i = base + lane;
+      const bool valid = i < nf;
+      const uint32_t node = valid ? fr[cur][i] : 0u;
+      float x[DIM];
+      uint32_t id = 0;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) x[j] = 0.f;
+      if (valid && depth < kTopLevels) {
+        const float* r = top + (uint64_t)node * RF;
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) x[j] = r[j];
+        id = __float_as_uint(r[NodeRec<DIM>::kId]);
+      } else if (valid) {
+        load_rec<DIM, BUF>(src, node, x, id);
+      }
+      const bool ge = x[0] >= lo[0], le = x[0] <= hi[0], lower = x[0] < lo[0];
+      bool in = valid & ge & le;
+#pragma unroll
+      for (int j = 1; j < DIM; ++j) in = in & (x[j] <= hi[j]) & (x[j] >= lo[j]);
+      const uint64_t hm = __ballot(in);
+      if (in) {
+        const uint32_t pos = hits + (uint32_t)__popcll(hm & below);
+        if (pos < slot) ids[(uint64_t)b * slot + pos] = id;
+      }
+      hits += (uint32_t)__popcll(hm);
+      const bool go_l = valid && ge && node < tl;
+      const bool go_r = valid && (lower || le) && node < tr;
+      const uint64_t lm = __ballot(go_l), rm = __ballot(go_r);
+      const uint32_t at = nn + (uint32_t)__popcll(lm & below) + (uint32_t)__popcll(rm & below);
+      const uint32_t nxt = cur ^ 1u;
+      if (go_l && at < kWaveFrontier) fr[nxt][at] = 2u * node + 1u;
+      const uint32_t atr = at + (go_l ? 1u : 0u);
+      if (go_r && atr < kWaveFrontier) fr[nxt][atr] = 2u * node + 2u;
+      nn += (uint32_t)(__popcll(lm) + __popcll(rm));
+    }
+    __syncthreads();
+    if (nn > kWaveFrontier) {
+      overflow = true;
+      break;
+    }
+    cur ^= 1u;
+    nf = nn;
+    ++depth;
+  }
+  if (lane == 0) counts[b] = overflow ? kSmallOverflow : hits;
+}
+
 // Work estimate of every search block for the fill pass: the most nodes any
 // of its queries visited in the count pass (visits in processing order).
 __global__ __launch_bounds__(kThreads) void block_work_kernel(const uint32_t* __restrict__ work_slot, uint32_t nq,

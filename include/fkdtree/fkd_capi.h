@@ -185,6 +185,29 @@ int fkd_search_noalloc(fkd_tree_t tree, uint32_t nq, const float* dev_mins, cons
                        const uint32_t* dev_query_events, uint64_t* dev_offsets, uint32_t* dev_ids_out,
                        uint64_t capacity, fkd_stream_t stream);
 
+/* Low-latency search of a few boxes given in host memory, for callers that
+ * issue one box (or a handful) at a time and wait for it: the reference's
+ * per-box search_in_the_box called from many threads (main.cpp:351-359,
+ * FKDTree_cpu.h:123-169).  One wave walks each box breadth first (one
+ * dependent load per depth instead of one per visited node), reading the
+ * boxes from and writing the results to page-locked host memory, so a call
+ * is one launch and one synchronisation.  Single trees only (n_events 1).
+ *   host_boxes   [nb][2 * dim] floats: the box's mins, then its maxs;
+ *   host_counts  [nb]: hits of box b, or FKD_SMALL_OVERFLOW when one depth
+ *                of its walk holds more than 2,048 nodes (search it with
+ *                fkd_search instead);
+ *   host_ids     [nb][slot]: the first min(count, slot) ids of box b in
+ *                unspecified order; a count above `slot` means the ids were
+ *                cut off (search the box again with a larger slot).
+ * At most FKD_SMALL_MAX_BOXES boxes, FKD_SMALL_MAX_SLOT ids per box and
+ * 2^28 ids in all (nb * slot).
+ * Synchronous; calls on one tree must not overlap. */
+#define FKD_SMALL_MAX_BOXES 65536u
+#define FKD_SMALL_MAX_SLOT (1u << 20)
+#define FKD_SMALL_OVERFLOW 0xffffffffu
+int fkd_search_small(fkd_tree_t tree, uint32_t nb, const float* host_boxes, uint32_t slot,
+                     uint32_t* host_counts, uint32_t* host_ids, fkd_stream_t stream);
+
 /* Boxes [c - half_width, c + half_width] around every point of the handle,
  * in the order the points were given (SoA [dim][n_points], one float
  * rounding per bound), and the event of every box (dev_query_events may be
