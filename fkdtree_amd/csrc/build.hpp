@@ -134,20 +134,23 @@ __host__ __device__ inline uint64_t plane_stride(uint64_t P) { return (P + 4u + 
 constexpr int kLevelItems = 8;
 constexpr int kLevelTile = kThreads * kLevelItems;
 
-// Depths whose subtrees have at most kFinCap points are built by one
+// Depths whose subtrees have at most 2^fin_log - 1 points are built by one
 // workgroup per subtree in LDS (finish_kernel); the global level loop stops
 // above them.  In an event of depth D every subtree of depth L has at most
-// 2^(D-L+1) - 1 points, so the finisher starts at depth fin_depth(D).
-constexpr uint32_t kFinLog = 10;
-constexpr uint32_t kFinCap = (1u << kFinLog) - 1u;
-constexpr int kFinItems = 4;  // consecutive positions per thread in the finisher
-static_assert(kFinCap <= (uint32_t)(kThreads * kFinItems), "a finisher subtree fits one pass of the workgroup");
-__host__ __device__ inline uint32_t fin_depth(uint32_t D) { return D + 1u > kFinLog ? D + 1u - kFinLog : 0u; }
+// 2^(D-L+1) - 1 points, so the finisher starts at depth fin_depth(D, fin_log).
+// fin_log is a launch parameter (kFinLogMin..kFinLogMax); the finisher's
+// threads take 2^(fin_log - 8) consecutive positions each.
+constexpr uint32_t kFinLogDefault = 10;
+constexpr uint32_t kFinLogMin = 9;
+constexpr uint32_t kFinLogMax = 11;
+__host__ __device__ inline uint32_t fin_depth(uint32_t D, uint32_t fin_log) {
+  return D + 1u > fin_log ? D + 1u - fin_log : 0u;
+}
 // Largest finisher subtree of an event of n >= 1 points: H - 1 + min(R, H)
 // with H = 2^(D - fin_depth(D)) and R the nodes on the last depth D.
-__host__ __device__ inline uint32_t fin_subtree_max(uint32_t n) {
+__host__ __device__ inline uint32_t fin_subtree_max(uint32_t n, uint32_t fin_log) {
   const uint32_t D = floor_log2(n);
-  const uint32_t H = 1u << (D - fin_depth(D));
+  const uint32_t H = 1u << (D - fin_depth(D, fin_log));
   const uint32_t R = n - ((1u << D) - 1u);
   return H - 1u + (R < H ? R : H);
 }
@@ -163,6 +166,7 @@ template <int DIM>
 struct LevelArgs {
   uint32_t L;                 // depth whose intervals are split
   uint32_t d;                 // split dimension L % DIM
+  uint32_t fin_log;           // depths from fin_depth(D, fin_log) on are finish_kernel's
   const uint32_t* Ad;         // rank planes of array d (not moved at this depth)
   const uint32_t* src[3];     // rank planes of the DIM-1 other arrays ...
   uint32_t* dst[3];           // ... and where their partitioned copies go
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   const LevelTile lti = level_tile(a.tm, t);
   const uint32_t lt = lti.lt, o = lti.o, n = lti.n;
   const Shape sh = shape_of(n);
-  const bool split = a.L < fin_depth(sh.D);  // deeper levels: finish_kernel
+  const bool split = a.L < fin_depth(sh.D, a.fin_log);  // deeper levels: finish_kernel
   const bool root_only = (n == 1 && a.L == 0);
   const uint64_t P = a.PS;  // plane stride
   if (!split) {
@@ -375,8 +379,9 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   }
   // Phase 2: classification and the thread's segmented counts.  cnt packs
   // the thread's unsegmented counts of left (bits 0-11), right (12-23) and
-  // median (24-31) elements: a tile of 2,048 holds at most three intervals
-  // (every interval here has at least 1,023 points), so no field overflows.
+  // median (24-31) elements: a tile of 2,048 holds at most six intervals
+  // (every interval here has at least 2^fin_log - 1 >= 511 points), so no
+  // field overflows.
   SegCount agg{0u, 0u, 0u};
   uint32_t cnt = 0;
 #pragma unroll
@@ -597,6 +602,7 @@ struct FinArgs {
   TileMap fm;                 // workgroup -> event; local index = subtree of depth fin_depth(D)
   uint64_t P;                 // plane stride, plane_stride(points of the batch)
   uint32_t cap;               // most points of one subtree (fin_subtree_max over the events)
+  uint32_t fin_log;           // the subtrees start at depth fin_depth(D, fin_log)
 };
 
 // The levels fin_depth(D) .. D-1 of one subtree in LDS: the same medians and
@@ -604,11 +610,14 @@ struct FinArgs {
 // subtree's points named by their index in array 0 (sorted by dimension 0)
 // and their ranks kept in a table, so only 16-bit names move.  Writes F
 // (heap index -> rank) for the subtree's nodes, leaves included.
-template <int DIM>
+// ITEMS consecutive positions per thread: subtrees of up to 256 * ITEMS - 1
+// points (fin_log = 8 + log2 ITEMS).
+template <int DIM, int ITEMS>
 // Eight waves per SIMD (at most 64 registers) keep a whole launch of
-// half-full subtrees resident (2,048 workgroups on 1M points); dim 4 would
-// spill at that bound and keeps the default.
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DIM <= 3 ? 8 : 1))) void finish_kernel(FinArgs<DIM> a) {
+// half-full subtrees resident (2,048 workgroups on 1M points); dim 4 and
+// 8 items per thread would spill at that bound and keep the default.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && ITEMS <= 4 ? 8 : 1))) void finish_kernel(FinArgs<DIM> a) {
+  constexpr int kFinItems = ITEMS;
   constexpr int MA = DIM > 1 ? DIM - 1 : 1;  // arrays moved per depth
   __shared__ SegCountN<MA> s_wave[kThreads / kWave];
   extern __shared__ __align__(16) uint32_t fin_lds[];
@@ -623,7 +632,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DIM <=
   const uint32_t o = a.fm.off[e];
   const uint32_t n = a.fm.off[e + 1] - o;
   const Shape sh = shape_of(n);
-  const uint32_t L0 = fin_depth(sh.D);
+  const uint32_t L0 = fin_depth(sh.D, a.fin_log);
   uint32_t start0, size0;
   subtree_range(sh, L0, t0, start0, size0);
   const uint64_t P = a.P;

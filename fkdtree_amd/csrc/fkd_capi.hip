@@ -227,6 +227,8 @@ struct fkd_tuning {
   uint32_t fill_window = 4096;  // fill pass: boxes reordered by work inside windows of this many (0: off)
   uint32_t count_window = 0;    // count pass: boxes reordered by volume inside windows of this many (0: off)
   uint32_t key_volume_bits = 0; // box order: bits of a box-volume class below the top Morton regions
+  uint32_t fin_log = fkd::kFinLogDefault;  // build: subtrees of < 2^fin_log points are finished in LDS
+  uint32_t build_radix_items = fkd::kRadixItems;  // build: keys per thread of the digit passes (8 or 16)
 };
 
 struct fkd_tree_s {
@@ -238,6 +240,8 @@ struct fkd_tree_s {
   uint32_t Dmax = 0;
   uint32_t global_levels = 0;  // depths built by level_kernel; the rest by finish_kernel
   uint32_t fin_cap = 1;        // most points of one finisher subtree over all events
+  uint32_t lay_fin_log = 0;    // fin_log and radix items the tilings were laid out for
+  uint32_t lay_radix_items = 0;
   bool ids_in_order = true;
   bool built = false;
   std::vector<uint64_t> h_off;
@@ -321,49 +325,58 @@ fkd_tree_s::~fkd_tree_s() {
 namespace {
 
 int ensure_build_scratch(fkd_tree_s* t);
+int layout(fkd_tree_s* t, hipStream_t s);
 
 // Zeroed words for the decoupled look-back scans of one build or search:
 // sized up front, cleared by a single launch, handed out in slices.
 struct ZeroArena {
   uint64_t* base = nullptr;
   uint64_t used = 0;
+  uint64_t cap = 0;  // words cleared by zero_arena
+  // The next `words` words, or null when they are not all inside the
+  // cleared arena (callers fail instead of launching on such memory).
   uint64_t* take(uint64_t words) {
+    if (!base || used + words > cap) return nullptr;
     uint64_t* p = base + used;
     used += words;
     return p;
   }
 };
 
+int arena_exhausted() { return fail(FKD_ERR_STATE, "internal: zeroed scratch arena too small for this launch"); }
+
 int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za);
 
 // Words one radix pass takes from the arena: scan descriptors + counter.
 uint64_t radix_pass_words(uint32_t M, uint32_t T) { return fkd::scan_tiles((uint64_t)M * 256u * T) + fkd::kCtrWords; }
 
-// One stable LSD digit pass over M key arrays (count, scan, scatter); desc
-// (scan_tiles of the digit counts) and ctr must be zero.
+// One stable LSD digit pass over M key arrays (count, scan, scatter) with
+// ITEMS keys per thread; desc (scan_tiles of the digit counts) and ctr must
+// be zero.
+template <int ITEMS>
 int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts, uint32_t* offsets,
                uint64_t* desc, uint32_t* ctr, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
                uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift, bool write_keys,
                bool write_ranks, hipStream_t s) {
   const uint64_t nc = (uint64_t)M * 256u * tm.T;
   dim3 grid(tm.T, M);
-  fkd::radix_count_kernel<<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts);
+  fkd::radix_count_kernel<ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts);
   FKD_TRY(launch_check("radix_count"));
   const uint64_t st = fkd::scan_tiles(nc);
   fkd::scan_exclusive_kernel<uint32_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(counts, offsets, nc, desc, ctr);
   FKD_TRY(launch_check("radix_scan"));
   if (write_keys && !write_ranks)
-    fkd::radix_scatter_kernel<true, false><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks, P,
-                                                                          shift, tm, offsets);
+    fkd::radix_scatter_kernel<true, false, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
+                                                                                 P, shift, tm, offsets);
   else if (!write_keys && write_ranks)
-    fkd::radix_scatter_kernel<false, true><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks, P,
-                                                                          shift, tm, offsets);
+    fkd::radix_scatter_kernel<false, true, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
+                                                                                 P, shift, tm, offsets);
   else if (write_keys && write_ranks)
-    fkd::radix_scatter_kernel<true, true><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks, P,
-                                                                         shift, tm, offsets);
+    fkd::radix_scatter_kernel<true, true, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
+                                                                                P, shift, tm, offsets);
   else
-    fkd::radix_scatter_kernel<false, false><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks, P,
-                                                                           shift, tm, offsets);
+    fkd::radix_scatter_kernel<false, false, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride,
+                                                                                  ranks, P, shift, tm, offsets);
   return launch_check("radix_scatter");
 }
 
@@ -373,9 +386,13 @@ int run_radix_pass(fkd_tree_s* t, ZeroArena& za, uint32_t M, const uint32_t* kin
                    bool write_keys, bool write_ranks, hipStream_t s) {
   const fkd::TileMap tm = t->radix_tiles.map(t->off.as<uint32_t>());
   uint64_t* desc = za.take(radix_pass_words(M, tm.T));
+  if (!desc) return arena_exhausted();
   uint32_t* ctr = reinterpret_cast<uint32_t*>(desc + fkd::scan_tiles((uint64_t)M * 256u * tm.T));
-  return radix_pass(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, kin, kout, vin,
-                    vout, stride, ranks, shift, write_keys, write_ranks, s);
+  if (t->lay_radix_items == 8)
+    return radix_pass<8>(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, kin, kout, vin,
+                         vout, stride, ranks, shift, write_keys, write_ranks, s);
+  return radix_pass<16>(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, kin, kout, vin,
+                        vout, stride, ranks, shift, write_keys, write_ranks, s);
 }
 
 // Level tiles of the handle; one tree needs no tile table (its kernels take
@@ -395,6 +412,10 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   if (P == 0) {
     t->built = true;
     return FKD_OK;
+  }
+  if (t->lay_fin_log != t->tune.fin_log || t->lay_radix_items != t->tune.build_radix_items) {
+    if (t->graph_count) return fail(FKD_ERR_STATE, "the build's tilings cannot change while a HIP graph exists");
+    FKD_TRY(layout(t, s));
   }
   FKD_TRY(ensure_build_scratch(t));
   const uint32_t levels = t->global_levels;
@@ -463,6 +484,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     fkd::LevelArgs<DIM> a;
     a.L = L;
     a.d = L % DIM;
+    a.fin_log = t->lay_fin_log;
     a.Ad = Pbuf(a.d, cur[a.d]);
     int k = 0;
     int moved[3] = {0, 0, 0};
@@ -488,6 +510,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     a.Tid = t->Tid.as<uint32_t>();
     a.recs = t->recs.as<float>();
     a.desc = za.take((uint64_t)M * tm.T + fkd::kCtrWords);
+    if (!a.desc) return arena_exhausted();
     a.ctr = reinterpret_cast<uint32_t*>(a.desc + (uint64_t)M * tm.T);
     const uint32_t blocks = (M ? M : 1u) * tm.T;
     fkd::level_kernel<DIM><<<blocks, fkd::kThreads, 0, s>>>(a);
@@ -505,7 +528,14 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     f.fm = t->fin_tiles.map(t->off.as<uint32_t>());
     f.P = PS;
     f.cap = t->fin_cap;
-    fkd::finish_kernel<DIM><<<t->fin_tiles.T, fkd::kThreads, fkd::fin_lds_bytes(DIM, f.cap), s>>>(f);
+    f.fin_log = t->lay_fin_log;
+    const uint32_t lds = fkd::fin_lds_bytes(DIM, f.cap);
+    if (t->lay_fin_log == 9)
+      fkd::finish_kernel<DIM, 2><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
+    else if (t->lay_fin_log == 10)
+      fkd::finish_kernel<DIM, 4><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
+    else
+      fkd::finish_kernel<DIM, 8><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
     FKD_TRY(launch_check("finish"));
   }
   // 7. Tree arrays and search records from F.
@@ -606,8 +636,9 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   for (int pass = 0; pass < passes; ++pass) {
     const bool lastp = pass == passes - 1;
     uint64_t* desc = za.take(radix_pass_words(1, T));
+    if (!desc) return arena_exhausted();
     uint32_t* ctr = reinterpret_cast<uint32_t*>(desc + fkd::scan_tiles(nc));
-    FKD_TRY(radix_pass(tm, nq, 1, t->q_rcounts.as<uint32_t>(), t->q_roffsets.as<uint32_t>(), desc, ctr,
+    FKD_TRY(radix_pass<fkd::kRadixItems>(tm, nq, 1, t->q_rcounts.as<uint32_t>(), t->q_roffsets.as<uint32_t>(), desc, ctr,
                        keys + (uint64_t)c * n,
                        keys + (uint64_t)(1 - c) * n, perm + (uint64_t)c * n, perm + (uint64_t)(1 - c) * n, n,
                        nullptr, 8u * pass, !lastp, false, s));
@@ -615,6 +646,7 @@ int plan_queries(fkd_tree_s* t, uint32_t nq, const float* mins, const float* max
   }
   t->q_perm_half = (uint32_t)c;
   t->q_scan_desc = za.take(fkd::scan_tiles(nq) + fkd::kCtrWords);
+  if (!t->q_scan_desc) return arena_exhausted();
   float* smins = t->q_sbox.as<float>();
   float* smaxs = smins + (uint64_t)DIM * n;
   fkd::permute_boxes_kernel<DIM><<<g, 256, 0, s>>>(t->q_aos.as<float4>(), perm + (uint64_t)c * n, nq, smins,
@@ -676,6 +708,7 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
     ZeroArena za;
     FKD_TRY(zero_arena(t, st + fkd::kCtrWords, s, za));
     desc = za.take(st + fkd::kCtrWords);
+    if (!desc) return arena_exhausted();
   }
   fkd::scan_exclusive_kernel<uint64_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(
       t->q_counts.as<uint32_t>(), qoff, nq, desc, reinterpret_cast<uint32_t*>(desc + st));
@@ -808,6 +841,7 @@ int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za) {
   FKD_TRY(t->zarena.ensure(std::max<uint64_t>(words, 1) * 8));
   za.base = t->zarena.as<uint64_t>();
   za.used = 0;
+  za.cap = words;
   return fill_u64(za.base, 0, words, s);
 }
 
@@ -844,10 +878,8 @@ int check_unique_ids(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(launch_check("id_keys"));
   int c = 0;
   for (int pass = 0; pass < 4; ++pass) {
-    uint64_t* desc = za.take(radix_pass_words(1, tm.T));
-    uint32_t* ctr = reinterpret_cast<uint32_t*>(desc + fkd::scan_tiles(256ull * tm.T));
-    FKD_TRY(radix_pass(tm, P, 1, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, K + c * S,
-                       K + (1 - c) * S, A + c * S, A + (1 - c) * S, S, nullptr, 8u * pass, true, false, s));
+    FKD_TRY(run_radix_pass<1>(t, za, 1, K + c * S, K + (1 - c) * S, A + c * S, A + (1 - c) * S, S, nullptr, 8u * pass,
+                              true, false, s));
     c = 1 - c;
   }
   uint32_t* flag = t->err.as<uint32_t>() + 1;
@@ -861,11 +893,11 @@ int check_unique_ids(fkd_tree_s* t, hipStream_t s) {
   return FKD_OK;
 }
 
-int finish_create(fkd_tree_s* t, hipStream_t s) {
-  const uint32_t P = t->P, E = t->n_events;
-  const int dim = t->dim;
-  std::vector<uint32_t> off32((size_t)E + 1);
-  for (size_t e = 0; e <= E; ++e) off32[e] = (uint32_t)t->h_off[e];
+// Tilings that depend on the build's launch parameters (finisher capacity,
+// digit-pass tile): the global depths, the finisher's subtrees and the radix
+// tiles.  Laid out at creation and again by a build after those change.
+int layout(fkd_tree_s* t, hipStream_t s) {
+  const uint32_t E = t->n_events, fl = t->tune.fin_log;
   t->Dmax = 0;
   t->global_levels = 0;
   t->fin_cap = 1;
@@ -875,15 +907,26 @@ int finish_create(fkd_tree_s* t, hipStream_t s) {
     if (!n) continue;
     const uint32_t D = fkd::floor_log2((uint32_t)n);
     t->Dmax = std::max(t->Dmax, D);
-    t->global_levels = std::max(t->global_levels, fkd::fin_depth(D));
-    fin_count[e] = 1u << fkd::fin_depth(D);  // subtrees the finisher builds
-    t->fin_cap = std::max(t->fin_cap, fkd::fin_subtree_max((uint32_t)n));
+    t->global_levels = std::max(t->global_levels, fkd::fin_depth(D, fl));
+    fin_count[e] = 1u << fkd::fin_depth(D, fl);  // subtrees the finisher builds
+    t->fin_cap = std::max(t->fin_cap, fkd::fin_subtree_max((uint32_t)n, fl));
   }
+  FKD_TRY(t->radix_tiles.init(t->h_off, fkd::kThreads * t->tune.build_radix_items, s));
+  FKD_TRY(t->fin_tiles.init_counts(fin_count, s));
+  t->lay_fin_log = fl;
+  t->lay_radix_items = t->tune.build_radix_items;
+  return FKD_OK;
+}
+
+int finish_create(fkd_tree_s* t, hipStream_t s) {
+  const uint32_t P = t->P, E = t->n_events;
+  const int dim = t->dim;
+  std::vector<uint32_t> off32((size_t)E + 1);
+  for (size_t e = 0; e <= E; ++e) off32[e] = (uint32_t)t->h_off[e];
   FKD_TRY(t->off.ensure(((size_t)E + 1) * 4));
   FKD_HIP(hipMemcpyAsync(t->off.p, off32.data(), ((size_t)E + 1) * 4, hipMemcpyHostToDevice, s));
-  FKD_TRY(t->radix_tiles.init(t->h_off, fkd::kRadixTile, s));
   FKD_TRY(t->level_tiles.init(t->h_off, fkd::kLevelTile, s));
-  FKD_TRY(t->fin_tiles.init_counts(fin_count, s));
+  FKD_TRY(layout(t, s));
   const size_t S = std::max<uint32_t>(P, 1);
   const int rf = dim <= 3 ? 4 : 8;
   FKD_TRY(t->Tc.ensure(S * dim * 4));
@@ -1505,6 +1548,14 @@ int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
   } else if (k == "key_volume_bits") {
     if (value > 6) return fail(FKD_ERR_INVALID, "key_volume_bits must be <= 6");
     t->tune.key_volume_bits = value;
+  } else if (k == "fin_log" || k == "build_radix_items") {
+    const bool fl = k == "fin_log";
+    if (fl && (value < fkd::kFinLogMin || value > fkd::kFinLogMax)) return fail(FKD_ERR_INVALID, "fin_log must be 9..11");
+    if (!fl && value != 8 && value != 16) return fail(FKD_ERR_INVALID, "build_radix_items must be 8 or 16");
+    // a captured graph launches with the tilings these lay out
+    if (t->graph_count && value != (fl ? t->tune.fin_log : t->tune.build_radix_items))
+      return fail(FKD_ERR_STATE, k + " cannot change while a HIP graph of this handle exists");
+    (fl ? t->tune.fin_log : t->tune.build_radix_items) = value;
   } else if (k == "fill_window" || k == "count_window") {
     if (value != 0 && value != 256 && value != 1024 && value != 4096)
       return fail(FKD_ERR_INVALID, k + " must be 0, 256, 1024 or 4096");
@@ -1523,6 +1574,8 @@ int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
   else if (k == "fill_window") *value = t->tune.fill_window;
   else if (k == "count_window") *value = t->tune.count_window;
   else if (k == "key_volume_bits") *value = t->tune.key_volume_bits;
+  else if (k == "fin_log") *value = t->tune.fin_log;
+  else if (k == "build_radix_items") *value = t->tune.build_radix_items;
   else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;
 }

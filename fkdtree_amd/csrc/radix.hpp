@@ -10,9 +10,10 @@
 
 namespace fkd {
 
+// Keys per thread of the digit passes: a launch parameter of the build
+// (8 or 16: tiles of 2,048 or 4,096 keys); the query plan uses kRadixItems.
 constexpr int kRadixItems = 16;
 constexpr int kRadixTile = kThreads * kRadixItems;
-constexpr int kRadixPerWave = kRadixTile / (kThreads / kWave);
 
 struct TileMap {
   const uint32_t* tile_ev;  // tile -> event; null: one event of single_n keys
@@ -49,26 +50,29 @@ __device__ inline TileInfo tile_info(const TileMap& tm, uint32_t t) {
 
 // counts[m][256*tb[e] + digit*nt + lt] = number of keys of tile t in array m
 // whose digit (key >> shift) & 255 equals digit.
+template <int ITEMS>
 __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* __restrict__ keys,
                                                                uint64_t kstride, uint32_t shift,
                                                                TileMap tm, uint32_t* counts) {
+  constexpr int kItems = ITEMS;
+  constexpr int kTile = kThreads * ITEMS;
   __shared__ uint32_t hist[4][256];
   const uint32_t t = blockIdx.x, m = blockIdx.y;
   const int wave = threadIdx.x / kWave;
 #pragma unroll
   for (int w = 0; w < 4; ++w) hist[w][threadIdx.x] = 0;
   __syncthreads();
-  const TileInfo ti = tile_info<kRadixTile>(tm, t);
+  const TileInfo ti = tile_info<kTile>(tm, t);
   const uint32_t* K = keys + m * kstride;
   // all loads of the thread first, so they are in flight together
-  uint32_t key[kRadixItems];
+  uint32_t key[kItems];
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
+  for (int r = 0; r < kItems; ++r) {
     const uint32_t i = ti.base + threadIdx.x + (uint32_t)r * kThreads;
     key[r] = i < ti.end ? K[i] : 0u;
   }
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r)
+  for (int r = 0; r < kItems; ++r)
     if (ti.base + threadIdx.x + (uint32_t)r * kThreads < ti.end) atomicAdd(&hist[wave][(key[r] >> shift) & 255u], 1u);
   __syncthreads();
   const uint32_t c = hist[0][threadIdx.x] + hist[1][threadIdx.x] + hist[2][threadIdx.x] +
@@ -84,42 +88,45 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
 // item, lane by lane), then written out in that order, so consecutive threads
 // write consecutive slots of one digit's run instead of up to 64 buckets per
 // store.
-template <bool WRITE_KEYS, bool WRITE_RANKS>
+template <bool WRITE_KEYS, bool WRITE_RANKS, int ITEMS>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, uint32_t* __restrict__ keys_out,
     const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out, uint64_t stride,
     uint32_t* __restrict__ ranks, uint32_t P, uint32_t shift, TileMap tm,
     const uint32_t* __restrict__ offsets) {
+  constexpr int kItems = ITEMS;
+  constexpr int kTile = kThreads * ITEMS;
+  constexpr int kPerWave = kTile / (kThreads / kWave);
   __shared__ uint32_t cnt[4][256];
   __shared__ uint32_t goff[256];
   __shared__ uint32_t tstart[256];
   __shared__ uint32_t s_wsum[kThreads / kWave];
-  __shared__ uint32_t s_key[kRadixTile];
-  __shared__ uint32_t s_val[kRadixTile];
+  __shared__ uint32_t s_key[kTile];
+  __shared__ uint32_t s_val[kTile];
   const uint32_t t = blockIdx.x, m = blockIdx.y;
   const int wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
 #pragma unroll
   for (int w = 0; w < 4; ++w) cnt[w][threadIdx.x] = 0;
-  const TileInfo ti = tile_info<kRadixTile>(tm, t);
+  const TileInfo ti = tile_info<kTile>(tm, t);
   goff[threadIdx.x] =
       offsets[(uint64_t)m * 256u * tm.T + 256ull * (t - ti.lt) + (uint64_t)threadIdx.x * ti.nt + ti.lt] -
       m * P;
   __syncthreads();
   const uint32_t* Ki = keys_in + m * stride;
   const uint32_t* Vi = vals_in + m * stride;
-  uint32_t key[kRadixItems], val[kRadixItems], rank[kRadixItems];
+  uint32_t key[kItems], val[kItems], rank[kItems];
   const uint64_t lt_mask = (1ull << lane) - 1ull;
-  const uint32_t wbase = ti.base + (uint32_t)wave * kRadixPerWave + lane;
+  const uint32_t wbase = ti.base + (uint32_t)wave * kPerWave + lane;
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
+  for (int r = 0; r < kItems; ++r) {
     const uint32_t pos = wbase + (uint32_t)r * kWave;
     const bool valid = pos < ti.end;
     key[r] = valid ? Ki[pos] : 0u;
     val[r] = valid ? Vi[pos] : 0u;
   }
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
+  for (int r = 0; r < kItems; ++r) {
     const uint32_t pos = wbase + (uint32_t)r * kWave;
     const bool valid = pos < ti.end;
     const uint32_t dig = (key[r] >> shift) & 255u;
@@ -156,7 +163,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
+  for (int r = 0; r < kItems; ++r) {
     const uint32_t pos = wbase + (uint32_t)r * kWave;
     if (pos < ti.end) {
       const uint32_t dig = (key[r] >> shift) & 255u;
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   uint32_t* Vo = vals_out + m * stride;
   const uint32_t tn = ti.end - ti.base;
 #pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
+  for (int r = 0; r < kItems; ++r) {
     const uint32_t i = threadIdx.x + (uint32_t)r * kThreads;
     if (i < tn) {
       const uint32_t k = s_key[i], v = s_val[i];

@@ -1,16 +1,24 @@
-"""Autotuner of the search launch parameters.
+"""Autotuner of the build and search launch parameters.
 
     python -m fkdtree_amd.tune [--workload mixed_1m] [--reps 3]
 
-Sweeps `query_key_bits` (Morton bits per dimension of the box order; 0
-keeps the caller's order), `fill_order` (heaviest search blocks first in
-the fill pass) and `fill_window` (boxes reordered by work inside windows of
-that many boxes for the fill pass) on a workload, timing the two-pass search with HIP events,
-and stores the
-fastest setting in `tuning.json` next to this file, keyed by GPU
-architecture, dimension and log2 of the point count.  Trees created by this
-package apply the nearest stored entry (`apply`).  Results never depend on
-these parameters: only time does.
+On a workload it times, with HIP events:
+
+* the search, sweeping `query_key_bits` (Morton bits per dimension of the
+  box order; 0 keeps the caller's order), `fill_order` (heaviest search
+  blocks first in the fill pass), `fill_window` (boxes reordered by work
+  inside windows of that many boxes for the fill pass), `count_window` and
+  `key_volume_bits`;
+* the build, sweeping `fin_log` (subtrees of fewer than 2^fin_log points
+  are finished by one workgroup in LDS: 511, 1,023 or 2,047 points; fewer
+  global level launches against bigger finisher workgroups) and
+  `build_radix_items` (keys per thread of the presort's digit passes: tiles
+  of 2,048 or 4,096 keys).
+
+The fastest settings go to `tuning.json` next to this file, keyed by GPU
+architecture, dimension, log2 of the (per-event) point count and kind of
+handle.  Trees created by this package apply the nearest stored entry
+(`apply`).  Results never depend on these parameters: only time does.
 """
 from __future__ import annotations
 
@@ -28,6 +36,7 @@ TABLE = os.path.join(HERE, "tuning.json")
 
 CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill_window": (0, 256, 1024, 4096),
               "count_window": (0, 1024, 4096), "key_volume_bits": (0, 3, 4)}
+BUILD_CANDIDATES = {"fin_log": (9, 10, 11), "build_radix_items": (8, 16)}
 
 
 _ARCH: Optional[str] = None
@@ -132,6 +141,28 @@ def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=Non
     return out
 
 
+def sweep_build(tree, reps: int = 5, candidates=None):
+    """Time the build for every candidate geometry; [(params, ms)] sorted."""
+    from . import Stream, Timer
+
+    cands = candidates or BUILD_CANDIDATES
+    s = tree.stream if tree.stream is not None else Stream()
+    tm = Timer()
+    out = []
+    keys = list(cands)
+    for values in itertools.product(*(cands[k] for k in keys)):
+        params = dict(zip(keys, values))
+        tree.set_tuning(**params)
+        tree.build()             # lays the tilings out for the new geometry
+        tm.start(s)
+        for _ in range(reps):
+            tree.build_async()
+        tm.stop(s)
+        out.append((params, tm.ms() / reps))
+    out.sort(key=lambda x: x[1])
+    return out
+
+
 KEEP_SWEEP = 16  # sweep rows kept per table entry
 
 
@@ -165,11 +196,13 @@ def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path
         key = _key(pts.dim, pts.n)
     tree.build()
     res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=d_q.ptr if d_q is not None else None)
+    bres = sweep_build(tree, reps=max(reps, 5))
     tree.close()
     arch = _arch()
-    # the table keeps the fastest settings of the sweep; the whole sweep is returned
-    entry = {"params": res[0][0], "ms": res[0][1], "workload": workload,
-             "sweep": [{"params": p, "ms": ms} for p, ms in res[:KEEP_SWEEP]]}
+    # the table keeps the fastest settings of the sweeps; the whole sweeps are returned
+    entry = {"params": {**res[0][0], **bres[0][0]}, "ms": res[0][1], "build_ms": bres[0][1], "workload": workload,
+             "sweep": [{"params": p, "ms": ms} for p, ms in res[:KEEP_SWEEP]],
+             "build_sweep": [{"params": p, "ms": ms} for p, ms in bres]}
     if write:
         table = load_table(path)
         table.setdefault(arch, {})[key] = entry
@@ -186,9 +219,11 @@ def main(argv=None):
     ap.add_argument("--out", default=TABLE, help="tuning table to update")
     a = ap.parse_args(argv)
     r = autotune(a.workload, a.reps, write=not a.no_write, path=a.out)
-    print(json.dumps({k: r[k] for k in ("arch", "key", "params", "ms")}))
+    print(json.dumps({k: r[k] for k in ("arch", "key", "params", "ms", "build_ms")}))
     for row in r["sweep"]:
         print(json.dumps(row))
+    for row in r["build_sweep"]:
+        print(json.dumps({"build": row}))
 
 
 if __name__ == "__main__":

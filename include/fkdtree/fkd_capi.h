@@ -136,7 +136,18 @@ int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t str
  *   "key_volume_bits" bits of a box-volume class inserted in the box order
  *                     below the top two Morton bits per dimension (0..6,
  *                     default 0), so boxes of similar size are neighbours;
- *                     it pays when box sizes vary widely. */
+ *                     it pays when box sizes vary widely;
+ *   "fin_log"         build: subtrees of fewer than 2^fin_log points are
+ *                     finished by one workgroup in LDS (9, 10 or 11:
+ *                     511 / 1,023 / 2,047 points; default 10), the depths
+ *                     above them by one launch per depth;
+ *   "build_radix_items" build: keys per thread of the presort's digit
+ *                     passes (8 or 16: tiles of 2,048 or 4,096 keys;
+ *                     default 16).
+ * The two build keys cannot change while a graph of the handle exists
+ * (FKD_ERR_STATE); the next build lays the tilings out for new values.
+ * They replace the reference's fixed launch geometry
+ * (FKDTree_opencl.h:13-15, 288-289). */
 int fkd_set_tuning(fkd_tree_t tree, const char* key, uint32_t value);
 int fkd_get_tuning(fkd_tree_t tree, const char* key, uint32_t* value);
 

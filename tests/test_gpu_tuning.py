@@ -97,3 +97,60 @@ def test_event_batch_tuning_keeps_results(require_gpu):
         assert np.array_equal(off, ooff), f"{tag}: counts differ"
         assert np.array_equal(sorted_sets(off, ids), want), f"{tag}: ids differ"
     t.close()
+
+
+GEOMETRIES = [(fl, ri) for fl in (9, 10, 11) for ri in (8, 16)]
+
+
+@pytest.mark.parametrize("kind,n,dim", [("mixed", 300_000, 3), ("shuffled_ids", 70_000, 3), ("grid", 40_000, 2),
+                                        ("uniform", 50_000, 4), ("zeros", 9_000, 1)])
+def test_build_geometry_keeps_trees(require_gpu, kind, n, dim):
+    """Finisher capacity (fin_log: subtrees of < 2^fin_log points built in
+    LDS) and digit-pass tile (build_radix_items) change time only: every
+    setting builds the oracle's tree bit for bit (FKDTree_cpu.h:221-277), on
+    one handle re-laid out between builds, and searches it exactly."""
+    from tests.helpers import bits
+    p = adversarial_points(kind, n, dim=dim)
+    boxes = self_boxes(p, 0.02)
+    print(f"[geometry] {kind} n={n} dim={dim}", flush=True)
+    oc, oi = oracle.build(p.coords, p.ids)
+    q = min(n, 5000)
+    ooff, oids = oracle.search(oc, oi, boxes.mins[:, :q], boxes.maxs[:, :q])
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    for fl, ri in GEOMETRIES:
+        t.set_tuning(fin_log=fl, build_radix_items=ri)
+        assert t.get_tuning("fin_log") == fl and t.get_tuning("build_radix_items") == ri
+        t.build()
+        c, i = t.tree_arrays()
+        assert np.array_equal(bits(c), bits(oc)) and np.array_equal(i, oi), (fl, ri)
+        off, ids = t.search_in_the_box_multiple_csr(boxes.mins[:, :q], boxes.maxs[:, :q])
+        assert np.array_equal(off, ooff), (fl, ri)
+        assert np.array_equal(sorted_sets(off, ids), sorted_sets(ooff, oids)), (fl, ri)
+    t.close()
+
+
+def test_build_geometry_events(require_gpu):
+    """The same for a batch of events of every size class: empty, one point,
+    below and above each finisher capacity (511, 1,023, 2,047 points)."""
+    from fkdtree_amd import FKDEvents, gen
+    from tests.helpers import bits
+    sizes = [0, 1, 2, 510, 511, 512, 1022, 1023, 1024, 2046, 2047, 2048, 4097, 30_000, 0, 70_001, 3]
+    evs = [gen.mixed_event(s, 77, e) for e, s in enumerate(sizes)]
+    b = gen.concat_events(evs)
+    oc, oi = oracle.build_events(b.points.coords, b.points.ids, b.offsets)
+    t = FKDEvents(b.points.coords, b.points.ids, b.offsets)
+    for fl, ri in GEOMETRIES[::-1]:
+        t.set_tuning(fin_log=fl, build_radix_items=ri)
+        t.build()
+        c, i = t.tree_arrays()
+        assert np.array_equal(bits(c), bits(oc)) and np.array_equal(i, oi), (fl, ri)
+    t.close()
+
+
+def test_build_geometry_rejects_bad_values(require_gpu):
+    p = adversarial_points("mixed", 100)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    for k, v in (("fin_log", 8), ("fin_log", 12), ("build_radix_items", 4), ("build_radix_items", 32)):
+        with pytest.raises(RuntimeError):
+            t.set_tuning(**{k: v})
+    t.close()

@@ -44,6 +44,9 @@ def test_candidates_are_valid():
     assert set(tune.CANDIDATES["fill_window"]) <= {0, 256, 1024, 4096}
     assert set(tune.CANDIDATES["count_window"]) <= {0, 256, 1024, 4096}
     assert all(0 <= v <= 6 for v in tune.CANDIDATES["key_volume_bits"])
+    # build geometry: finisher capacities the library instantiates, digit-pass tiles
+    assert set(tune.BUILD_CANDIDATES["fin_log"]) <= {9, 10, 11}
+    assert set(tune.BUILD_CANDIDATES["build_radix_items"]) <= {8, 16}
 
 
 class _FakeTree:
@@ -79,3 +82,14 @@ def test_table_is_reread_when_the_file_changes(tmp_path):
     st = os.stat(p)
     os.utime(p, ns=(st.st_atime_ns, st.st_mtime_ns + 1_000_000))
     assert tune.lookup("gfx950", 3, 1 << 20, tune.load_table(str(p))) == {"query_key_bits": 7}
+
+
+def test_library_accepts_every_candidate_key():
+    """Every key the tuner sweeps is one the library's fkd_set_tuning knows:
+    its error message for an unknown key differs from a bad value's (no GPU
+    call is needed to reject a NULL handle, so the names are checked by
+    the header instead)."""
+    import os
+    hdr = open(os.path.join(os.path.dirname(tune.HERE), "include", "fkdtree", "fkd_capi.h")).read()
+    for k in list(tune.CANDIDATES) + list(tune.BUILD_CANDIDATES):
+        assert f'"{k}"' in hdr, k
