@@ -28,7 +28,7 @@ from . import gen  # noqa: F401
 from ._lib import FKDError, check, lib
 
 __all__ = ["FKDTree", "FKDEvents", "DeviceArray", "Stream", "Timer", "Event", "Graph", "FKDError",
-           "device_info"]
+           "device_info", "release_cached"]
 
 
 def device_info() -> dict:
@@ -40,10 +40,16 @@ def device_info() -> dict:
     res, used = C.c_uint64(), C.c_uint64()
     check(lib().fkd_pool_info(C.byref(res), C.byref(used)))
     cached = max(res.value - used.value, 0)
-    # free_bytes: what the device reports free plus what the allocator's pool
-    # holds unused (reusable by stream-ordered allocations)
+    # free_bytes: what the device reports free plus what the library's
+    # device-memory cache holds unused (reusable by its allocations)
     return {"name": name.value.decode(), "compute_units": cus.value, "free_bytes": fr.value + cached,
             "device_free_bytes": fr.value, "pool_cached_bytes": cached, "total_bytes": tot.value}
+
+
+def release_cached() -> None:
+    """Give the device memory the library caches (freed buffers kept for
+    reuse) back to the driver, where their last use has completed."""
+    check(lib().fkd_release_cached())
 
 
 class Stream:
@@ -148,8 +154,10 @@ class DeviceArray:
 
     @classmethod
     def from_host(cls, a: np.ndarray, stream=None) -> "DeviceArray":
+        """A copy of host array `a`; with `stream`, allocated (and later
+        freed) in stream order on it."""
         a = np.ascontiguousarray(a)
-        d = cls(a.shape, a.dtype)
+        d = cls(a.shape, a.dtype, stream=stream)
         d.upload(a, stream)
         return d
 

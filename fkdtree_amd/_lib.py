@@ -54,6 +54,7 @@ _SIGS = {
     "fkd_malloc_async": (I32, [C.POINTER(P), SZ, P]),
     "fkd_free_async": (I32, [P, P]),
     "fkd_pool_info": (I32, [C.POINTER(U64), C.POINTER(U64)]),
+    "fkd_release_cached": (I32, []),
     "fkd_host_alloc": (I32, [C.POINTER(P), SZ]),
     "fkd_host_free": (I32, [P]),
     "fkd_memcpy_htod": (I32, [P, P, SZ, P]),

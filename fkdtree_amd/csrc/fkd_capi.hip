@@ -5,8 +5,10 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <type_traits>
 #include <vector>
 
@@ -61,27 +63,192 @@ struct StreamScope {
   }
 };
 
-// The device's default pool keeps freed memory for reuse instead of handing
-// it back at every synchronisation.
-void keep_pool_memory() {
-  static bool done[64] = {};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || done[dev]) return;
-  hipMemPool_t pool;
-  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-    uint64_t keep = UINT64_MAX;
-    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+// Device memory cache of the library's stream-ordered allocations.
+//
+// A freed block stays with the library, with an event recorded on the stream
+// that freed it, and is handed out again (to any stream) once that event has
+// completed; a new block comes from the driver only when no cached block of
+// the size class is free.  Blocks go back to the driver only when an
+// allocation finds the device full, or on fkd_release_cached.  The runtime's
+// stream-ordered pool cannot serve here: any synchronous free in the process
+// (hipFree, hipHostFree, the runtime's staging of pageable copies) empties
+// it, and a long-running server then mapped multi-gigabyte buffers from the
+// driver for every batch and stalled for up to seconds.  Sizes are rounded
+// up to four classes per power of two (at most 25% slack).  Each device has
+// its own cache; the caches live until the process ends.
+// Which runtime call of the last failed cache operation on this thread.
+thread_local const char* tl_cache_call = "";
+
+class DeviceCache {
+ public:
+  hipError_t alloc(void** out, size_t bytes) {
+    const size_t cls = size_class(bytes);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto range = free_.equal_range(cls);
+      for (auto it = range.first; it != range.second; ++it) {
+        const Block b = it->second;
+        if (b.done) {
+          const hipError_t q = hipEventQuery(b.done);
+          if (q == hipErrorNotReady) {
+            (void)hipGetLastError();  // "not ready" is an answer, not an error of a later call
+            continue;
+          }
+          if (q != hipSuccess) {
+            // An event whose stream was destroyed outside fkd_stream_destroy
+            // can no longer answer: once the whole device is idle the block
+            // is free regardless.
+            (void)hipGetLastError();
+            const hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) {
+              tl_cache_call = "hipDeviceSynchronize after an unanswerable cached-block event";
+              return e;
+            }
+          }
+          events_.push_back(b.done);
+        }
+        free_.erase(it);
+        free_bytes_ -= cls;
+        live_[b.p] = cls;
+        live_bytes_ += cls;
+        *out = b.p;
+        return hipSuccess;
+      }
+    }
+    hipError_t e = hipMalloc(out, cls);
+    if (e == hipErrorOutOfMemory) {  // give the completed cached blocks back and try once more
+      (void)hipGetLastError();
+      (void)release();
+      e = hipMalloc(out, cls);
+    }
+    if (e != hipSuccess) {
+      tl_cache_call = "hipMalloc";
+      return e;
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    live_[*out] = cls;
+    live_bytes_ += cls;
+    return hipSuccess;
   }
-  done[dev] = true;
+
+  // Stream-ordered free on `s` (ordered), or after the whole device is idle.
+  hipError_t free(void* p, hipStream_t s, bool ordered) {
+    if (!p) return hipSuccess;
+    size_t cls = 0;
+    hipEvent_t ev = nullptr;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto it = live_.find(p);
+      if (it == live_.end()) return hipErrorInvalidValue;
+      cls = it->second;
+      live_.erase(it);
+      live_bytes_ -= cls;
+      if (ordered && !events_.empty()) {
+        ev = events_.back();
+        events_.pop_back();
+      }
+    }
+    hipError_t e = hipSuccess;
+    if (ordered) {
+      if (!ev) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(ev, s);
+      if (e != hipSuccess) tl_cache_call = "hipEventRecord at a stream-ordered free";
+    } else {
+      e = hipDeviceSynchronize();  // as hipFree would
+      if (e != hipSuccess) tl_cache_call = "hipDeviceSynchronize at a free";
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    if (e != hipSuccess && ev) {  // the block is kept out of the cache: nothing tells when it is free
+      events_.push_back(ev);
+      return e;
+    }
+    free_.emplace(cls, Block{p, ev, ordered ? s : nullptr});
+    free_bytes_ += cls;
+    return e;
+  }
+
+  // The blocks freed on stream s need no event once s has drained (its
+  // caller synchronised it): they are free, and no cached event refers to s
+  // after it is destroyed.
+  void retire_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& kv : free_) {
+      Block& b = kv.second;
+      if (b.done && b.stream == s) {
+        events_.push_back(b.done);
+        b.done = nullptr;
+        b.stream = nullptr;
+      }
+    }
+  }
+
+  // Cached blocks whose last use has completed go back to the driver.
+  hipError_t release() {
+    std::vector<void*> give;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      for (auto it = free_.begin(); it != free_.end();) {
+        const Block b = it->second;
+        if (b.done) {
+          const hipError_t q = hipEventQuery(b.done);
+          if (q == hipErrorNotReady) {
+            (void)hipGetLastError();
+            ++it;
+            continue;
+          }
+          events_.push_back(b.done);
+        }
+        free_bytes_ -= it->first;
+        give.push_back(b.p);
+        it = free_.erase(it);
+      }
+    }
+    hipError_t r = hipSuccess;
+    for (void* p : give) {
+      const hipError_t e = hipFree(p);
+      if (e != hipSuccess) r = e;
+    }
+    return r;
+  }
+
+  void info(uint64_t* held, uint64_t* used) {
+    std::lock_guard<std::mutex> g(mu_);
+    if (held) *held = live_bytes_ + free_bytes_;
+    if (used) *used = live_bytes_;
+  }
+
+ private:
+  struct Block {
+    void* p;
+    hipEvent_t done;     // recorded on the freeing stream; null: free at once
+    hipStream_t stream;  // the freeing stream (while `done` is pending)
+  };
+  static size_t size_class(size_t bytes) {
+    bytes = std::max<size_t>(bytes, 512);
+    const unsigned k = 63u - (unsigned)__builtin_clzll(bytes);
+    const size_t step = (size_t)1 << (k - 2);
+    return (bytes + step - 1) & ~(step - 1);
+  }
+  std::mutex mu_;
+  std::unordered_map<void*, size_t> live_;  // block -> its size class
+  std::multimap<size_t, Block> free_;       // size class -> free blocks
+  std::vector<hipEvent_t> events_;          // spare events
+  uint64_t live_bytes_ = 0, free_bytes_ = 0;
+};
+
+DeviceCache* device_cache() {
+  static std::mutex mu;
+  static DeviceCache* caches[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  std::lock_guard<std::mutex> g(mu);
+  if (!caches[dev]) caches[dev] = new DeviceCache();  // kept until the process ends
+  return caches[dev];
 }
 
-hipError_t dev_alloc(void** p, size_t bytes) {
-  if (!tl_ordered) return hipMalloc(p, bytes);
-  keep_pool_memory();
-  return hipMallocAsync(p, bytes, tl_stream);
-}
+hipError_t dev_alloc(void** p, size_t bytes) { return device_cache()->alloc(p, bytes); }
 
-hipError_t dev_free(void* p) { return tl_ordered ? hipFreeAsync(p, tl_stream) : hipFree(p); }
+hipError_t dev_free(void* p) { return device_cache()->free(p, tl_stream, tl_ordered); }
 
 struct DevBuf {
   void* p = nullptr;
@@ -104,7 +271,9 @@ struct DevBuf {
       bytes = 0;
     }
     want = std::max<size_t>(want, 256);
-    FKD_HIP(dev_alloc(&p, want));
+    const hipError_t e = dev_alloc(&p, want);
+    if (e != hipSuccess)
+      return fail(FKD_ERR_HIP, std::string("device allocation (") + tl_cache_call + "): " + hipGetErrorString(e));
     bytes = want;
     return FKD_OK;
   }
@@ -1660,15 +1829,12 @@ int fkd_host_free(void* ptr) {
 }
 
 int fkd_pool_info(uint64_t* reserved, uint64_t* used) {
-  int dev = 0;
-  FKD_HIP(hipGetDevice(&dev));
-  hipMemPool_t pool;
-  FKD_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-  uint64_t r = 0, u = 0;
-  FKD_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &r));
-  FKD_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &u));
-  if (reserved) *reserved = r;
-  if (used) *used = u;
+  device_cache()->info(reserved, used);
+  return FKD_OK;
+}
+
+int fkd_release_cached(void) {
+  FKD_HIP(device_cache()->release());
   return FKD_OK;
 }
 
@@ -1720,7 +1886,10 @@ int fkd_stream_create(fkd_stream_t* stream) {
   return FKD_OK;
 }
 int fkd_stream_destroy(fkd_stream_t stream) {
-  FKD_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FKD_HIP(hipStreamSynchronize(s));
+  device_cache()->retire_stream(s);
+  FKD_HIP(hipStreamDestroy(s));
   return FKD_OK;
 }
 int fkd_stream_sync(fkd_stream_t stream) {
@@ -1744,7 +1913,10 @@ int fkd_event_record(fkd_event_t ev, fkd_stream_t stream) {
 int fkd_event_query(fkd_event_t ev) {
   const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(ev));
   if (e == hipSuccess) return 1;
-  if (e == hipErrorNotReady) return 0;
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();  // an answer, not an error for the next launch check
+    return 0;
+  }
   return fail(FKD_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
 }
 int fkd_event_elapsed_ms(fkd_event_t a, fkd_event_t b, float* ms) {

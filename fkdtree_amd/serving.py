@@ -29,7 +29,9 @@ per stream that creates batches (creation synchronises the batch's stream):
 
 * Ingestion: events that arrive as FKDPoint records (uint32 [n, dim+1],
   `Request.records`) are written into a page-locked ring (`IngestRing`) as
-  they arrive; a batch of consecutive events is then one contiguous range of
+  they arrive: the serving thread reserves each event's slots in arrival
+  order and receiver threads copy the records in, so the copying never
+  holds up admission; a batch of consecutive events is then one contiguous range of
   the ring, handed to fkd_create_events as it lies (one asynchronous copy at
   full link rate, transposed on the device).  Events given as SoA arrays are
   copied per event into the batch's device arrays instead.
@@ -58,6 +60,7 @@ class Request:
     arrival: float = 0.0            # seconds after the start of the stream
     records: Optional[np.ndarray] = None  # FKDPoint records uint32 [n, dim+1] (coordinate bits, id)
     ring_start: int = -1            # slot of the records in the ingest ring, once received
+    ingested: Optional[concurrent.futures.Future] = None  # the copy into the ring, once started
 
     @property
     def n(self) -> int:
@@ -66,6 +69,27 @@ class Request:
     @property
     def dim(self) -> int:
         return int(self.records.shape[1] - 1 if self.records is not None else self.coords.shape[0])
+
+
+# Page-locked ring buffers no longer used by a server, by (capacity, dim),
+# handed to the next server instead of being freed: freeing page-locked
+# memory empties the device's stream-ordered memory pool, and afterwards the
+# pool no longer keeps what is freed (measured: 17 GB released at one
+# hipHostFree, then every later synchronisation released the freed
+# buffers), so every batch of every later server would map its buffers from
+# the driver anew; runs then stalled for up to seconds.  The buffers live
+# until the process ends.
+_FREE_RINGS: Dict[Tuple[int, int], List[np.ndarray]] = {}
+
+
+def _ring_buffer(capacity: int, dim: int) -> np.ndarray:
+    free = _FREE_RINGS.get((capacity, dim))
+    if free:
+        return free.pop()
+    import fkdtree_amd as F
+    buf = F.pinned_empty((capacity, dim + 1), np.uint32)
+    buf.fill(0)  # touch every page once, before events arrive
+    return buf
 
 
 class IngestRing:
@@ -78,10 +102,9 @@ class IngestRing:
     host (FKDTree_opencl.h:22-33)."""
 
     def __init__(self, capacity: int, dim: int = 3, buffer: Optional[np.ndarray] = None):
+        self.pooled = buffer is None
         if buffer is None:
-            import fkdtree_amd as F
-            buffer = F.pinned_empty((capacity, dim + 1), np.uint32)
-            buffer.fill(0)  # touch every page once, before events arrive
+            buffer = _ring_buffer(capacity, dim)
         assert buffer.shape == (capacity, dim + 1) and buffer.dtype == np.uint32
         self.buf = buffer
         self.capacity = capacity
@@ -90,9 +113,9 @@ class IngestRing:
         self.used = 0     # slots held, skipped tail ends included
         self.spans: "collections.deque[Tuple[int, int]]" = collections.deque()  # (slots held, n)
 
-    def put(self, records: np.ndarray) -> Optional[int]:
-        """Copy an event's records in; its first slot, or None when full."""
-        n = int(records.shape[0])
+    def reserve(self, n: int) -> Optional[int]:
+        """Claim the slots of an event of n points in arrival order; its
+        first slot, or None when full.  write() fills them."""
         if n > self.capacity:
             raise ValueError(f"event of {n} points exceeds the ring ({self.capacity})")
         start, skip = self.tail, 0
@@ -100,10 +123,20 @@ class IngestRing:
             start, skip = 0, self.capacity - self.tail
         if self.used + skip + n > self.capacity:
             return None
-        self.buf[start:start + n] = records
         self.tail = (start + n) % self.capacity
         self.used += skip + n
         self.spans.append((skip + n, n))
+        return start
+
+    def write(self, start: int, records: np.ndarray) -> None:
+        """Copy an event's records into its reserved slots (any thread)."""
+        self.buf[start:start + records.shape[0]] = records
+
+    def put(self, records: np.ndarray) -> Optional[int]:
+        """Reserve and copy in one step; the first slot, or None when full."""
+        start = self.reserve(int(records.shape[0]))
+        if start is not None:
+            self.write(start, records)
         return start
 
     def release(self, count: int) -> None:
@@ -117,6 +150,12 @@ class IngestRing:
 
     def view(self, start: int, n: int) -> np.ndarray:
         return self.buf[start:start + n]
+
+    def close(self) -> None:
+        """Hand a page-locked buffer back for the next ring (it is not freed)."""
+        if self.pooled and self.buf is not None:
+            _FREE_RINGS.setdefault((self.capacity, self.dim), []).append(self.buf)
+        self.buf = None
 
 
 def same_kind_prefix(queue: Sequence[Request], k: int) -> int:
@@ -237,6 +276,9 @@ class _Batch:
 
         offs, P, stream = self.offsets, self.P, self.stream
         dim = self.members[0].dim
+        for r in self.members:          # the members' copies into the ring
+            if r.ingested is not None:
+                r.ingested.result()
         if self.recs is not None:
             self.handle = F.FKDEvents.from_records(self.recs, offs, stream)  # synchronises the stream
         else:
@@ -246,6 +288,8 @@ class _Batch:
         self.d_qev = F.DeviceArray(max(P, 1), np.uint32, stream=stream)
         self.handle.boxes_around(server.half_width, self.d_mins.ptr, self.d_maxs.ptr, self.d_qev.ptr, stream)
         self.d_off = F.DeviceArray(P + 1, np.uint64, stream=stream)
+        # CSR offsets at the event boundaries are gathered on completion
+        self.d_bidx = F.DeviceArray.from_host(self.offsets, stream)
         self.cap = max(1, int(P * server.hits_per_point))
         self.d_out = F.DeviceArray(self.cap, np.uint32, stream=stream)
         self.done_ev = F.Event()
@@ -307,20 +351,20 @@ class _Batch:
         import fkdtree_amd as F
         from fkdtree_amd._lib import check, lib
 
-        idx = F.DeviceArray.from_host(self.offsets, self.stream)
         out = F.DeviceArray(self.offsets.shape[0], np.uint64, stream=self.stream)
-        check(lib().fkd_gather_u64(out.ptr, self.d_off.ptr, idx.ptr, self.offsets.shape[0], self.stream.handle))
+        check(lib().fkd_gather_u64(out.ptr, self.d_off.ptr, self.d_bidx.ptr, self.offsets.shape[0],
+                                   self.stream.handle))
         return out.download(self.stream)
 
     def free_search_buffers(self):
-        self.d_mins = self.d_maxs = self.d_qev = self.d_off = self.d_out = None
+        self.d_mins = self.d_maxs = self.d_qev = self.d_off = self.d_out = self.d_bidx = None
 
 
 class EventServer:
     def __init__(self, n_streams: int = 3, max_batch_points: int = 8_000_000, max_batch_events: int = 512,
                  hbm_fraction: float = 0.8, hbm_budget: Optional[int] = None, half_width: float = 0.01,
                  hits_per_point: float = 256.0, keep_results: bool = False, keep_resident: bool = True,
-                 poll_sleep_s: float = 50e-6, ring_points: int = 0, dim: int = 3):
+                 poll_sleep_s: float = 50e-6, ring_points: int = 0, dim: int = 3, ingest_threads: int = 4):
         import fkdtree_amd as F
 
         self.streams = [F.Stream() for _ in range(n_streams)]
@@ -343,6 +387,10 @@ class EventServer:
         # batches are created (upload, input check, launch) on worker threads,
         # one per stream, so the serving thread keeps admitting and completing
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=n_streams)
+        # arriving records are copied into the ring by receiver threads; the
+        # serving thread only reserves their slots
+        self.ingest = (concurrent.futures.ThreadPoolExecutor(max_workers=max(1, ingest_threads))
+                       if ring_points else None)
 
     # -- memory ---------------------------------------------------------------
     def _budget(self) -> int:
@@ -401,10 +449,11 @@ class EventServer:
                 if r.records is not None:
                     if self.ring is None:
                         raise ValueError("events given as records need an ingest ring (ring_points)")
-                    start = self.ring.put(r.records)
+                    start = self.ring.reserve(r.n)
                     if start is None:
                         break       # ring full: later arrivals wait for batches to take their events
                     r.ring_start = start
+                    r.ingested = self.ingest.submit(self.ring.write, start, r.records)
                 queue.append(pending.popleft())
             progressed = False
             # created batches (launched by their worker); the ring frees
@@ -491,16 +540,21 @@ class EventServer:
         return b.handle.search(mins, maxs, qev)
 
     def close(self):
-        """Evict every resident tree, stop the workers, and release the
-        streams and the page-locked ingest ring (a server holds about
-        256 MB of page-locked memory and three streams until then)."""
+        """Evict every resident tree, stop the workers, destroy the streams
+        and hand the page-locked ingest ring back for the next server (a
+        server holds about 256 MB of page-locked memory and three streams
+        until then)."""
         while self._evict_one():
             pass
         self.pool.shutdown(wait=True)
+        if self.ingest is not None:
+            self.ingest.shutdown(wait=True)
         for s in self.streams:
             s.sync()
             s.close()
         self.streams = []
+        if self.ring is not None:
+            self.ring.close()
         self.ring = None
 
     def __enter__(self) -> "EventServer":

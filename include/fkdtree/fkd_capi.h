@@ -70,9 +70,10 @@ int fkd_create_device(int dim, uint32_t n_events, const uint64_t* event_offsets,
 
 void fkd_destroy(fkd_tree_t tree);
 
-/* As fkd_destroy, but the device memory returns to the allocator in stream
- * order on `stream` instead of after a device-wide synchronisation: every
- * call on the handle must be ordered before it on `stream` (or complete). */
+/* As fkd_destroy, but the device memory returns to the library's cache in
+ * stream order on `stream` instead of after a device-wide synchronisation:
+ * every call on the handle must be ordered before it on `stream` (or
+ * complete). */
 int fkd_destroy_async(fkd_tree_t tree, fkd_stream_t stream);
 
 /* Build every tree of the handle.  Enqueues only.
@@ -282,18 +283,24 @@ int fkd_device_info(char* name, size_t name_len, int* compute_units, size_t* fre
                     size_t* total_bytes);
 int fkd_malloc(void** ptr, size_t bytes);
 int fkd_free(void* ptr);
-/* Stream-ordered device allocation (hipMallocAsync / hipFreeAsync on the
- * device's default pool, which keeps freed memory for reuse). */
+/* Stream-ordered device allocation from the library's device-memory cache,
+ * which also holds every handle's buffers: a block freed on a stream is
+ * reused once the work queued on that stream before the free has completed;
+ * blocks return to the driver only when the device is full or on
+ * fkd_release_cached (the runtime's own stream-ordered pool is emptied by
+ * any synchronous free in the process, so it cannot be relied on). */
 int fkd_malloc_async(void** ptr, size_t bytes, fkd_stream_t stream);
 /* Page-locked host memory (hipHostMalloc): copies from it to the device run
  * asynchronously at full link rate.  For receiving FKDPoint records
  * (FKDPoint.h:13-104) that go to fkd_create_events without a staging copy. */
 int fkd_host_alloc(void** ptr, size_t bytes);
 int fkd_host_free(void* ptr);
-/* Bytes the default pool holds from the device, and of those in use: the
- * difference is free for stream-ordered allocations though hipMemGetInfo
- * counts it as used. */
+/* Bytes the library's device-memory cache holds from the device, and of
+ * those in use: the difference is free for the library's allocations though
+ * hipMemGetInfo counts it as used. */
 int fkd_pool_info(uint64_t* reserved, uint64_t* used);
+/* Give the cached blocks whose last use has completed back to the driver. */
+int fkd_release_cached(void);
 int fkd_free_async(void* ptr, fkd_stream_t stream);
 int fkd_memcpy_htod(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
 int fkd_memcpy_dtoh(void* dst, const void* src, size_t bytes, fkd_stream_t stream);
@@ -303,6 +310,10 @@ int fkd_copy_u32(uint32_t* dst, const uint32_t* src, uint64_t n, fkd_stream_t st
  * event boundaries of a batch).  Enqueues only. */
 int fkd_gather_u64(uint64_t* dst, const uint64_t* src, const uint64_t* idx, uint64_t n, fkd_stream_t stream);
 int fkd_stream_create(fkd_stream_t* stream);
+/* Synchronises the stream, marks the cached memory freed on it as free, and
+ * destroys it.  Streams of the caller's own may be destroyed with
+ * hipStreamDestroy; cached blocks freed on them then become reusable after
+ * a device-wide synchronisation at the next allocation that needs them. */
 int fkd_stream_destroy(fkd_stream_t stream);
 int fkd_stream_sync(fkd_stream_t stream);
 typedef void* fkd_event_t;

@@ -86,3 +86,18 @@ def test_batches_hold_one_request_kind():
     assert contiguous_prefix(q, same_kind_prefix(q, 4)) == 2
     q = soa + reqs([7])
     assert same_kind_prefix(q, 3) == 3
+
+
+def test_ingest_ring_reserve_then_write():
+    """Slots are claimed in arrival order by reserve() and filled later by
+    write() (from receiver threads), in any order."""
+    from fkdtree_amd.serving import IngestRing
+
+    ring = IngestRing(10, 3, buffer=np.zeros((10, 4), dtype=np.uint32))
+    a, b = ring.reserve(4), ring.reserve(5)
+    assert (a, b) == (0, 4) and ring.reserve(2) is None
+    ring.write(b, _rec(5, 2))
+    ring.write(a, _rec(4, 1))
+    assert np.all(ring.view(0, 4)[:, 3] == 1) and np.all(ring.view(4, 5)[:, 3] == 2)
+    ring.release(1)
+    assert ring.reserve(3) == 0            # wraps past the 1-slot tail end
