@@ -161,6 +161,11 @@ __host__ __device__ inline uint32_t fin_subtree_max(uint32_t n, uint32_t fin_log
 __host__ __device__ inline uint32_t fin_lds_bytes(uint32_t dim, uint32_t cap) {
   return (dim * cap * 4u + dim * 2u * (cap + 1u) * 2u + 15u) & ~15u;
 }
+// Positions per finisher thread for subtrees of at most cap points: the
+// smallest of 1, 2, 4, 8 with 256 * items >= cap (cap < 2^kFinLogMax).
+__host__ __device__ inline uint32_t fin_items(uint32_t cap) {
+  return cap <= 256u ? 1u : cap <= 512u ? 2u : cap <= 1024u ? 4u : 8u;
+}
 
 template <int DIM>
 struct LevelArgs {
@@ -525,6 +530,373 @@ __global__ __launch_bounds__(kThreads) void level_kernel(LevelArgs<DIM> a) {
   }
 }
 
+// Two depths for one array at a time (dim 3).  An array of dim 3 moves on two
+// consecutive depths and then splits the next one, so its two moves can be
+// one: array m = (L + 2) % 3 goes from depth-L intervals straight to its
+// place under depth L + 1, a stable partition of every depth-L interval into
+// four runs (left-left, left-right, right-left, right-right) and three medians
+// (of the interval on dimension dA = L % 3, of its two children on dB =
+// (L + 1) % 3).  The medians' ranks come from the arrays that already hold
+// depth L (array dA, moved through depth L - 1, is split on dA) and depth
+// L + 1 (array dB, moved through depth L by the launch before).  One launch per
+// depth then moves one array instead of two, half the bytes of level_kernel;
+// the first launch of a build moves array 1 through depth 0 alone (single),
+// and an event whose depth L + 1 is the finisher's gets depth L alone.
+struct PairArgs {
+  uint32_t L;            // first depth of the pair
+  uint32_t dA, dB;       // split dimensions of depths L and L + 1
+  uint32_t single;       // 1: depth L only, for every event
+  uint32_t write_f;      // store the depth-L nodes in F
+  uint32_t fin_log;
+  const uint32_t* Aa;    // rank planes of array dA (holds depth L)
+  const uint32_t* Ab;    // rank planes of array dB (holds depth L + 1)
+  const uint32_t* src;   // rank planes of the moved array ...
+  uint32_t* dst;         // ... and where its partitioned copy goes
+  uint32_t* F;
+  TileMap tm;            // tiles of kThreads * ITEMS positions, event aligned
+  uint64_t PS;           // plane stride
+  uint64_t* desc;        // two look-back words per block, zero at launch
+  uint32_t* ctr;         // dynamic block counter, zero at launch
+};
+
+// Segmented counts of the four runs (reset at depth-L interval starts).
+struct SegCount4 {
+  uint32_t f, c[4];
+};
+__device__ inline SegCount4 seg4_op(const SegCount4& a, const SegCount4& b) {
+  SegCount4 r;
+  r.f = a.f | b.f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.c[k] = b.f ? b.c[k] : a.c[k] + b.c[k];
+  return r;
+}
+__device__ inline SegCount4 seg4_shfl_up(const SegCount4& v, int ofs) {
+  SegCount4 u;
+  u.f = __shfl_up(v.f, ofs);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) u.c[k] = __shfl_up(v.c[k], ofs);
+  return u;
+}
+__device__ inline SegCount4 seg4_shfl_down(const SegCount4& v, int ofs) {
+  SegCount4 u;
+  u.f = __shfl_down(v.f, ofs);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) u.c[k] = __shfl_down(v.c[k], ofs);
+  return u;
+}
+// Look-back words: word 0 = status | f << 60 | c0 << 30 | c1, word 1 =
+// status | c2 << 30 | c3 (counts below 2^30: events hold fewer points).  A
+// reader takes a pair only when both words carry the same status.
+__device__ inline void seg4_pack(const SegCount4& s, uint64_t st, uint64_t& w0, uint64_t& w1) {
+  w0 = st | ((uint64_t)s.f << 60) | ((uint64_t)s.c[0] << 30) | (uint64_t)s.c[1];
+  w1 = st | ((uint64_t)s.c[2] << 30) | (uint64_t)s.c[3];
+}
+__device__ inline SegCount4 seg4_unpack(uint64_t w0, uint64_t w1) {
+  SegCount4 s;
+  s.f = (uint32_t)(w0 >> 60) & 1u;
+  s.c[0] = (uint32_t)(w0 >> 30) & 0x3fffffffu;
+  s.c[1] = (uint32_t)w0 & 0x3fffffffu;
+  s.c[2] = (uint32_t)(w1 >> 30) & 0x3fffffffu;
+  s.c[3] = (uint32_t)w1 & 0x3fffffffu;
+  return s;
+}
+
+enum : uint32_t { kLL = 0, kLR = 1, kRL = 2, kRR = 3, kMedian = 4, kOutside = 5 };
+
+// The depth-L interval that holds position j (or the one just before the
+// separator j), its median positions, the medians' ranks and the F entry of
+// its node.
+struct PairIv {
+  uint32_t st, sz, mp, mpl, mpr, hn, rma, rml, rmr, fv;
+};
+__device__ inline PairIv pair_interval(const Shape& sh, uint32_t L, uint32_t j, bool two, const uint32_t* Aa0,
+                                       const uint32_t* Aa_d, const uint32_t* Ab_d, uint32_t o) {
+  PairIv v;
+  const uint32_t ts = subtree_at(sh, L, j);
+  subtree_range(sh, L, ts, v.st, v.sz);
+  const uint32_t k = left_size(v.sz);
+  v.mp = v.st + k;
+  v.hn = (1u << L) - 1u + ts;
+  v.rma = Aa_d[o + v.mp];
+  v.fv = Aa0[o + v.mp];  // rank in dimension 0 of the median: its entry in F
+  v.mpl = v.mpr = v.rml = v.rmr = 0u;
+  if (two) {  // children of at least 2^fin_log - 1 points: both exist
+    v.mpl = v.st + left_size(k);
+    v.mpr = v.mp + 1u + left_size(v.sz - k - 1u);
+    v.rml = Ab_d[o + v.mpl];
+    v.rmr = Ab_d[o + v.mpr];
+  }
+  return v;
+}
+
+// DA = a.dA, the split dimension of depth L (a template argument, so the
+// ranks of the item on dA and dB = (DA + 1) % 3 are registers, not selects).
+template <int ITEMS, int DA>
+__global__ __launch_bounds__(kThreads) void level_pair_kernel(PairArgs a) {
+  constexpr int DIM = 3;
+  constexpr uint32_t kTile = kThreads * ITEMS;
+  static_assert(ITEMS == 4 || ITEMS == 8, "one or two 16-byte loads per plane");
+  __shared__ uint32_t s_blk;
+  __shared__ uint64_t s_wave[2][kThreads / kWave];
+  __shared__ SegCount4 s_carry;
+  // staging slot q at q + q / 8: a thread's items of one run are 8 slots
+  // apart, and the padding spreads a wave's writes over all banks
+  __shared__ uint32_t s_out[DIM + 1][kTile + kTile / 8 + 1];  // + a slot for positions that stay
+  const uint32_t bid = dynamic_block(a.ctr, &s_blk);
+  const LevelTile lti = level_tile(a.tm, bid);
+  const uint32_t lt = lti.lt, o = lti.o, n = lti.n;
+  const Shape sh = shape_of(n);
+  const uint32_t fd = fin_depth(sh.D, a.fin_log);
+  const uint64_t P = a.PS;
+  if (a.L >= fd) {  // depth L is the finisher's for this event
+    if (threadIdx.x == 0) {
+      uint64_t w0, w1;
+      seg4_pack(SegCount4{1u, {0u, 0u, 0u, 0u}}, kStPrefix, w0, w1);
+      publish(&a.desc[2 * bid + 1], w1);
+      publish(&a.desc[2 * bid], w0);
+      if (n == 1 && a.L == 0 && a.write_f) a.F[o] = a.Aa[o];
+    }
+    return;
+  }
+  const bool two = !a.single && a.L + 1u < fd;
+  const uint32_t first = lt * kTile + threadIdx.x * (uint32_t)ITEMS;
+  const uint32_t* Aa_d = a.Aa + (uint64_t)a.dA * P;  // plane dA of array dA
+  const uint32_t* Ab_d = a.Ab + (uint64_t)a.dB * P;  // plane dB of array dB
+
+  // Phase 1: the ranks of the thread's items and the medians of their
+  // intervals, all loads before any use.
+  uint32_t rk[ITEMS][DIM];
+  bool in[ITEMS], second[ITEMS];  // inside an interval; that interval is v1
+  const bool vec = first + (uint32_t)ITEMS <= n;
+  const uint32_t sh4 = o & 3u;
+  if (vec) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const uint4* q = reinterpret_cast<const uint4*>(a.src + (uint64_t)k * P + o + first - sh4);
+      uint32_t e[ITEMS + 4];
+#pragma unroll
+      for (int v = 0; v < ITEMS / 4; ++v) {
+        const uint4 x = q[v];
+        e[4 * v] = x.x;
+        e[4 * v + 1] = x.y;
+        e[4 * v + 2] = x.z;
+        e[4 * v + 3] = x.w;
+      }
+      if (sh4) {
+        const uint4 x = q[ITEMS / 4];
+        e[ITEMS] = x.x;
+        e[ITEMS + 1] = x.y;
+        e[ITEMS + 2] = x.z;
+        e[ITEMS + 3] = x.w;
+      } else {
+        e[ITEMS] = e[ITEMS + 1] = e[ITEMS + 2] = e[ITEMS + 3] = 0u;
+      }
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        rk[i][k] = sh4 == 0u ? e[i] : sh4 == 1u ? e[i + 1] : sh4 == 2u ? e[i + 2] : e[i + 3];
+    }
+  }
+  // The thread's ITEMS positions lie in at most two depth-L intervals (every
+  // interval has at least 2^fin_log - 1 > ITEMS points) and a separator:
+  // the intervals and medians are found for the first and the last position
+  // only.
+  PairIv v0{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u}, v1 = v0;
+  if (first < n) {
+    const uint32_t jlast = (first + (uint32_t)ITEMS <= n ? first + (uint32_t)ITEMS : n) - 1u;
+    v0 = pair_interval(sh, a.L, first, two, a.Aa, Aa_d, Ab_d, o);
+    v1 = pair_interval(sh, a.L, jlast, two, a.Aa, Aa_d, Ab_d, o);
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t jl = first + i;
+    const bool in0 = jl < n && jl - v0.st < v0.sz;
+    const bool in1 = !in0 && jl < n && jl - v1.st < v1.sz;
+    in[i] = in0 || in1;
+    second[i] = !in0;
+    if (!vec) {
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) rk[i][k] = in[i] ? a.src[(uint64_t)k * P + o + jl] : 0u;
+    }
+  }
+  // Phase 2: classes, destinations relative to the run starts, and the
+  // thread's segmented run counts.  cnt0 packs the thread's unsegmented
+  // counts of left-left (bits 0-11), left-right (12-23) and median (24-31)
+  // elements, cnt1 right-left and right-right: a tile holds at most six
+  // depth-L intervals (each of at least 2^fin_log - 1 >= 511 points), so
+  // none overflows.
+  // Selects and arithmetic only, no branches: the lanes of a wave take every
+  // class.  The thread's counts are packed 12 bits a class (a tile holds at
+  // most 2,048 elements): seg = the segmented run counts (LL, LR, RL, RR in
+  // bits 0-47, bit 48: an interval starts here), cnt = the unsegmented counts
+  // of the four runs and the medians (bits 48-59).
+  constexpr int DB = (DA + 1) % 3;
+  constexpr uint64_t kSegF = 1ull << 48;
+  uint32_t cls[ITEMS], base[ITEMS];
+  bool starts[ITEMS];
+  uint64_t seg = 0, cnt = 0;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t jl = first + i;
+    const uint32_t ra = rk[i][DA], rb = rk[i][DB];
+    const bool s1 = second[i];
+    const uint32_t st = s1 ? v1.st : v0.st, mp = s1 ? v1.mp : v0.mp, rma = s1 ? v1.rma : v0.rma;
+    const bool start = in[i] && jl == st;
+    const bool med = ra == rma, left = ra < rma;
+    const uint32_t rm2 = left ? (s1 ? v1.rml : v0.rml) : (s1 ? v1.rmr : v0.rmr);
+    const uint32_t mpl = s1 ? v1.mpl : v0.mpl, mpr = s1 ? v1.mpr : v0.mpr, mp2 = left ? mpl : mpr;
+    const bool med2 = two & !med & (rb == rm2), hi2 = two & (rb > rm2);
+    uint32_t c = (left ? kLL : kRL) + (hi2 ? 1u : 0u);  // kLR = kLL + 1, kRR = kRL + 1
+    c = (med | med2) ? kMedian : c;
+    c = in[i] ? c : kOutside;
+    const uint32_t run = left ? (hi2 ? mpl + 1u : st) : (hi2 ? mpr + 1u : mp + 1u);
+    base[i] = o + (med ? mp : med2 ? mp2 : run);
+    cls[i] = c;
+    starts[i] = start;
+    const uint64_t one = c < kOutside ? 1ull << (12u * c) : 0ull;  // the class's field
+    seg = (start ? kSegF : seg) + (c < kMedian ? one : 0ull);
+    cnt += one;
+  }
+  if (a.write_f) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if (in[i] && first + i == (second[i] ? v1.mp : v0.mp))
+        a.F[o + (second[i] ? v1.hn : v0.hn)] = second[i] ? v1.fv : v0.fv;  // the depth-L node
+  }
+
+  // Block-wide scans of the packed counts: segmented (a flagged element
+  // restarts the sum) for the destinations, plain for the staging slots.
+  const auto seg_add = [](uint64_t x, uint64_t y) { return (y & kSegF) ? y : x + y; };
+  const auto add = [](uint64_t x, uint64_t y) { return x + y; };
+  const uint32_t lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  const uint64_t seg_incl = wave_scan64(seg, seg_add), cnt_incl = wave_scan64(cnt, add);
+  uint64_t seg_excl = ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(seg_incl >> 32), 1) << 32) |
+                      (uint32_t)__shfl_up((int)(uint32_t)seg_incl, 1);
+  if (lane == 0) seg_excl = 0;
+  if (lane == kWave - 1) {
+    s_wave[0][wave] = seg_incl;
+    s_wave[1][wave] = cnt_incl;
+  }
+  __syncthreads();
+  uint64_t wcarry = 0, tile_seg = 0, cnt_excl = cnt_incl - cnt, cnt_tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / kWave; ++w) {
+    if (w < wave) {
+      wcarry = seg_add(wcarry, s_wave[0][w]);
+      cnt_excl += s_wave[1][w];
+    }
+    tile_seg = seg_add(tile_seg, s_wave[0][w]);
+    cnt_tot += s_wave[1][w];
+  }
+  seg_excl = seg_add(wcarry, seg_excl);
+  const auto unpack = [](uint64_t x) {
+    return SegCount4{(uint32_t)(x >> 48) & 1u,
+                     {(uint32_t)x & 0xfffu, (uint32_t)(x >> 12) & 0xfffu, (uint32_t)(x >> 24) & 0xfffu,
+                      (uint32_t)(x >> 36) & 0xfffu}};
+  };
+  const SegCount4 tile_agg = unpack(tile_seg), excl = unpack(seg_excl);
+
+  if (wave == 0) {
+    // Windowed look-back over the tiles of the same event (as level_kernel),
+    // two words per tile.
+    SegCount4 carry{0u, {0u, 0u, 0u, 0u}};
+    uint64_t w0, w1;
+    if (lt == 0) {
+      if (lane == 0) {
+        seg4_pack(tile_agg, kStPrefix, w0, w1);
+        publish(&a.desc[2 * bid + 1], w1);
+        publish(&a.desc[2 * bid], w0);
+      }
+    } else {
+      if (lane == 0) {
+        seg4_pack(tile_agg, kStAggregate, w0, w1);
+        publish(&a.desc[2 * bid + 1], w1);
+        publish(&a.desc[2 * bid], w0);
+      }
+      for (uint32_t b0 = 0;; b0 += kWave) {
+        const int64_t j = (int64_t)lt - 1 - (int64_t)b0 - (int64_t)lane;
+        uint64_t* p = j >= 0 ? &a.desc[2 * (bid - (b0 + lane + 1u))] : nullptr;
+        uint64_t x0 = kStPrefix, x1 = kStPrefix;
+        if (p) {
+          x0 = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          x1 = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        uint32_t fst;
+        while (true) {
+          const bool valid = (x0 & kStMask) != kStInvalid && (x0 & kStMask) == (x1 & kStMask);
+          const bool stop = valid && ((x0 & kStMask) == kStPrefix || ((x0 >> 60) & 1ull));
+          const uint64_t sm = __ballot(stop);
+          fst = sm ? (uint32_t)__builtin_ctzll(sm) : (uint32_t)kWave;
+          const uint64_t need = fst >= (uint32_t)kWave - 1u ? ~0ull : (2ull << fst) - 1ull;
+          if ((__ballot(!valid) & need) == 0ull) break;
+          __builtin_amdgcn_s_sleep(1);
+          if (!valid) {
+            x0 = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            x1 = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        SegCount4 v = seg4_unpack(x0, x1);
+        if (lane > fst) v = SegCount4{0u, {0u, 0u, 0u, 0u}};
+#pragma unroll
+        for (int ofs = 1; ofs < kWave; ofs <<= 1) {
+          const SegCount4 u = seg4_shfl_down(v, ofs);
+          if (lane + ofs < (uint32_t)kWave) v = seg4_op(u, v);
+        }
+        carry = seg4_op(v, carry);
+        if (fst < (uint32_t)kWave) break;
+      }
+      if (lane == 0) {
+        seg4_pack(seg4_op(carry, tile_agg), kStPrefix, w0, w1);
+        publish(&a.desc[2 * bid + 1], w1);
+        publish(&a.desc[2 * bid], w0);
+      }
+    }
+    if (lane == 0) s_carry = carry;
+  }
+  __syncthreads();
+  const SegCount4 run0 = seg4_op(s_carry, excl);
+  uint32_t nLL = run0.c[0], nLR = run0.c[1], nRL = run0.c[2], nRR = run0.c[3];
+  // Staging in write-out order: the tile's four runs, then its medians; in
+  // tile order the destinations of each run rise, jumping only between
+  // intervals.
+  const uint32_t tLL = (uint32_t)cnt_tot & 0xfffu, tLR = (uint32_t)(cnt_tot >> 12) & 0xfffu;
+  const uint32_t tRL = (uint32_t)(cnt_tot >> 24) & 0xfffu, tRR = (uint32_t)(cnt_tot >> 36) & 0xfffu;
+  const uint32_t tMD = (uint32_t)(cnt_tot >> 48) & 0xfffu;
+  uint32_t sLL = (uint32_t)cnt_excl & 0xfffu, sLR = tLL + ((uint32_t)(cnt_excl >> 12) & 0xfffu);
+  uint32_t sRL = tLL + tLR + ((uint32_t)(cnt_excl >> 24) & 0xfffu);
+  uint32_t sRR = tLL + tLR + tRL + ((uint32_t)(cnt_excl >> 36) & 0xfffu);
+  uint32_t sMD = tLL + tLR + tRL + tRR + ((uint32_t)(cnt_excl >> 48) & 0xfffu);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t c = cls[i];
+    if (starts[i]) nLL = nLR = nRL = nRR = 0;
+    const bool ll = c == kLL, lr = c == kLR, rl = c == kRL, rr = c == kRR, md = c == kMedian;
+    const uint32_t dest = base[i] + (ll ? nLL : lr ? nLR : rl ? nRL : rr ? nRR : 0u);
+    const uint32_t sl = ll ? sLL : lr ? sLR : rl ? sRL : rr ? sRR : sMD;
+    nLL += ll ? 1u : 0u;
+    nLR += lr ? 1u : 0u;
+    nRL += rl ? 1u : 0u;
+    nRR += rr ? 1u : 0u;
+    sLL += ll ? 1u : 0u;
+    sLR += lr ? 1u : 0u;
+    sRL += rl ? 1u : 0u;
+    sRR += rr ? 1u : 0u;
+    sMD += md ? 1u : 0u;
+    const uint32_t sp = c == kOutside ? kTile + kTile / 8 : sl + (sl >> 3);
+    s_out[DIM][sp] = dest;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) s_out[k][sp] = rk[i][k];
+  }
+  __syncthreads();
+  const uint32_t tn = tLL + tLR + tRL + tRR + tMD;
+  for (uint32_t q = threadIdx.x; q < tn; q += kThreads) {
+    const uint32_t qp = q + (q >> 3);
+    const uint32_t dest = s_out[DIM][qp];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) a.dst[(uint64_t)k * P + dest] = s_out[k][qp];
+  }
+}
+
 // Segmented counts of N arrays at once: one interval-start flag, and the
 // left / right counts of every array.
 template <int N>
@@ -563,9 +935,9 @@ __device__ inline SegCountN<N> seg_shfl_up(const SegCountN<N>& v, int ofs) {
 }
 
 // Block-wide exclusive segmented scan of the threads' aggregates (every
-// thread calls it); returns the thread's exclusive prefix.  s_wave holds
-// kThreads / kWave entries; the caller syncs before reusing it.
-template <int N>
+// thread of an NT-thread block calls it); returns the thread's exclusive
+// prefix.  s_wave holds NT / kWave entries; the caller syncs before reusing it.
+template <int N, int NT = kThreads>
 __device__ inline SegCountN<N> block_seg_excl(SegCountN<N> agg, SegCountN<N>* s_wave) {
   SegCountN<N> incl = agg;
   const uint32_t lane = lane_id();
@@ -581,7 +953,7 @@ __device__ inline SegCountN<N> block_seg_excl(SegCountN<N> agg, SegCountN<N>* s_
   __syncthreads();
   SegCountN<N> wcarry = seg_zero<N>();
 #pragma unroll
-  for (int w = 0; w < kThreads / kWave; ++w)
+  for (int w = 0; w < NT / kWave; ++w)
     if (w < wave) wcarry = seg_op(wcarry, s_wave[w]);
   return seg_op(wcarry, excl);
 }
@@ -595,6 +967,14 @@ __host__ __device__ inline uint32_t moves_before(uint32_t L0, uint32_t m, uint32
   return L0 - same;
 }
 
+// The same count under level_pair_kernel (dim 3): array 1 in the first
+// launch (when depth 0 is built globally), and array m in launch L for every
+// L < L0 with (L + 2) % 3 == m, i.e. L = m + 1 (mod 3).
+__host__ __device__ inline uint32_t pair_moves_before(uint32_t L0, uint32_t m) {
+  const uint32_t r = (m + 1u) % 3u;
+  return ((m == 1u && L0 >= 1u) ? 1u : 0u) + (L0 > r ? (L0 - 1u - r) / 3u + 1u : 0u);
+}
+
 template <int DIM>
 struct FinArgs {
   const uint32_t* planes[4][2];  // array m, buffer c: plane k (rank in dimension k) at planes[m][c] + k * P
@@ -603,6 +983,7 @@ struct FinArgs {
   uint64_t P;                 // plane stride, plane_stride(points of the batch)
   uint32_t cap;               // most points of one subtree (fin_subtree_max over the events)
   uint32_t fin_log;           // the subtrees start at depth fin_depth(D, fin_log)
+  uint32_t pairs;             // the depths above were built by level_pair_kernel
 };
 
 // The levels fin_depth(D) .. D-1 of one subtree in LDS: the same medians and
@@ -610,16 +991,17 @@ struct FinArgs {
 // subtree's points named by their index in array 0 (sorted by dimension 0)
 // and their ranks kept in a table, so only 16-bit names move.  Writes F
 // (heap index -> rank) for the subtree's nodes, leaves included.
-// ITEMS consecutive positions per thread: subtrees of up to 256 * ITEMS - 1
-// points (fin_log = 8 + log2 ITEMS).
-template <int DIM, int ITEMS>
+// ITEMS consecutive positions per thread: subtrees of up to NT * ITEMS
+// points.
+// NT threads per workgroup (one to four waves).
+template <int DIM, int ITEMS, int NT>
 // Eight waves per SIMD (at most 64 registers) keep a whole launch of
 // half-full subtrees resident (2,048 workgroups on 1M points); dim 4 and
 // 8 items per thread would spill at that bound and keep the default.
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && ITEMS <= 4 ? 8 : 1))) void finish_kernel(FinArgs<DIM> a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && ITEMS <= 4 ? 8 : 1))) void finish_kernel(FinArgs<DIM> a) {
   constexpr int kFinItems = ITEMS;
   constexpr int MA = DIM > 1 ? DIM - 1 : 1;  // arrays moved per depth
-  __shared__ SegCountN<MA> s_wave[kThreads / kWave];
+  __shared__ SegCountN<MA> s_wave[NT / kWave];
   extern __shared__ __align__(16) uint32_t fin_lds[];
   const uint32_t cap = a.cap;
   // s_rank(k, j): rank in dimension k of name j; s_arr(m, c, j): buffer c of array m
@@ -638,8 +1020,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DIM <=
   const uint64_t P = a.P;
   const uint32_t* pl[DIM];
 #pragma unroll
-  for (int m = 0; m < DIM; ++m) pl[m] = a.planes[m][moves_before(L0, (uint32_t)m, (uint32_t)DIM) & 1u];
-  for (uint32_t j = threadIdx.x; j < size0; j += kThreads) {
+  for (int m = 0; m < DIM; ++m)
+    pl[m] = a.planes[m][(a.pairs ? pair_moves_before(L0, (uint32_t)m) : moves_before(L0, (uint32_t)m, (uint32_t)DIM)) & 1u];
+  for (uint32_t j = threadIdx.x; j < size0; j += NT) {
 #pragma unroll
     for (int k = 0; k < DIM; ++k) s_rank(k)[j] = pl[0][(uint64_t)k * P + o + start0 + j];
     s_arr(0, 0)[j] = (uint16_t)j;
@@ -652,7 +1035,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DIM <=
   // names of the other arrays' points: index of their dimension-0 rank in array 0
 #pragma unroll
   for (int m = 1; m < DIM; ++m)
-    for (uint32_t j = threadIdx.x; j < size0; j += kThreads) {
+    for (uint32_t j = threadIdx.x; j < size0; j += NT) {
       const uint32_t r0 = pl[m][o + start0 + j];
       uint32_t lo = 0, hi = size0;  // first index with s_rank(0)[idx] >= r0
       while (lo < hi) {
@@ -732,7 +1115,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DIM <=
         cls[k][i] = c;
       }
     }
-    const SegCountN<MA> run0 = block_seg_excl<MA>(agg, s_wave);
+    const SegCountN<MA> run0 = block_seg_excl<MA, NT>(agg, s_wave);
 #pragma unroll
     for (int k = 0; k < MA; ++k) {
       uint32_t m = d + 1u + (uint32_t)k;

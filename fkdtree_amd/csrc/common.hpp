@@ -84,6 +84,29 @@ __device__ inline T wave_incl_sum(T v) {
   return v;
 }
 
+// Wave-wide inclusive scan of a 64-bit value under an associative op (first
+// argument: the earlier lanes), by DPP moves: row_shr:1/2/4/8 inside rows of
+// 16 lanes (lanes without a source read 0, which op must treat as its
+// identity), then row_bcast:15 (lane 15 of rows 0 and 2 into rows 1 and 3)
+// and row_bcast:31 (lane 31 into rows 2 and 3).  VALU moves instead of the
+// LDS round trip of a shuffle.
+template <int CTRL, int ROW_MASK>
+__device__ inline uint64_t dpp_mov64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROW_MASK, 0xf, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, ROW_MASK, 0xf, true);
+  return ((uint64_t)hi << 32) | lo;
+}
+template <class Op>
+__device__ inline uint64_t wave_scan64(uint64_t v, Op op) {
+  v = op(dpp_mov64<0x111, 0xf>(v), v);  // row_shr:1
+  v = op(dpp_mov64<0x112, 0xf>(v), v);  // row_shr:2
+  v = op(dpp_mov64<0x114, 0xf>(v), v);  // row_shr:4
+  v = op(dpp_mov64<0x118, 0xf>(v), v);  // row_shr:8
+  v = op(dpp_mov64<0x142, 0xa>(v), v);  // row_bcast:15
+  v = op(dpp_mov64<0x143, 0xc>(v), v);  // row_bcast:31
+  return v;
+}
+
 // Publication of decoupled look-back descriptors: status and value share one
 // 64-bit word (status in the top two bits), so relaxed device-scope atomics
 // suffice -- no other data is published through them.  Release/acquire

@@ -398,6 +398,7 @@ struct fkd_tuning {
   uint32_t key_volume_bits = 0; // box order: bits of a box-volume class below the top Morton regions
   uint32_t fin_log = fkd::kFinLogDefault;  // build: subtrees of < 2^fin_log points are finished in LDS
   uint32_t build_radix_items = fkd::kRadixItems;  // build: keys per thread of the digit passes (8 or 16)
+  uint32_t level_pairs = 1;     // build, dim 3: one array moved through two depths per launch
 };
 
 struct fkd_tree_s {
@@ -591,10 +592,14 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   const fkd::TileMap tm = level_map(t);
   constexpr uint32_t M = DIM - 1;
   const uint32_t RT = t->radix_tiles.T;
+  // dim 3: one launch per depth moves one array through two depths, plus a
+  // first launch for array 1's lone move at depth 0 (level_pair_kernel)
+  const bool pairs = DIM == 3 && t->tune.level_pairs && levels > 0;
+  const uint64_t level_words =
+      pairs ? (uint64_t)(levels + 1) * (2ull * tm.T + fkd::kCtrWords) : (uint64_t)levels * ((uint64_t)M * tm.T + fkd::kCtrWords);
   ZeroArena za;
   FKD_TRY(zero_arena(t,
-                     (t->ids_in_order ? 0 : 4 * radix_pass_words(1, RT)) + 4 * radix_pass_words(DIM, RT) +
-                         (uint64_t)levels * ((uint64_t)M * tm.T + fkd::kCtrWords),
+                     (t->ids_in_order ? 0 : 4 * radix_pass_words(1, RT)) + 4 * radix_pass_words(DIM, RT) + level_words,
                      s, za));
   const uint64_t S = P;  // stride of one array
   uint32_t* A = t->A.as<uint32_t>();  // A[m][2][P]
@@ -649,7 +654,39 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   FKD_TRY(launch_check("planes"));
   // 5. One partition launch per depth.
   int cur[4] = {0, 0, 0, 0};
-  for (uint32_t L = 0; L < levels; ++L) {
+  if (pairs) {
+    // launch -1 moves array 1 through depth 0; launch L moves array (L + 2) % 3
+    // through depths L and L + 1 (L alone where L + 1 is the finisher's)
+    for (int64_t L = -1; L < (int64_t)levels; ++L) {
+      fkd::PairArgs a;
+      const uint32_t m = L < 0 ? 1u : (uint32_t)(L + 2) % 3u;
+      a.L = L < 0 ? 0u : (uint32_t)L;
+      a.dA = a.L % 3u;
+      a.dB = (a.L + 1u) % 3u;
+      a.single = L < 0 ? 1u : 0u;
+      a.write_f = L < 0 ? 0u : 1u;
+      a.fin_log = t->lay_fin_log;
+      a.Aa = Pbuf((int)a.dA, cur[a.dA]);
+      a.Ab = L < 0 ? a.Aa : Pbuf((int)a.dB, cur[a.dB]);
+      a.src = Pbuf((int)m, cur[m]);
+      a.dst = Pbuf((int)m, 1 - cur[m]);
+      a.F = t->F.as<uint32_t>();
+      a.tm = tm;
+      a.PS = PS;
+      a.desc = za.take(2ull * tm.T + fkd::kCtrWords);
+      if (!a.desc) return arena_exhausted();
+      a.ctr = reinterpret_cast<uint32_t*>(a.desc + 2ull * tm.T);
+      if (a.dA == 0)
+        fkd::level_pair_kernel<fkd::kLevelItems, 0><<<tm.T, fkd::kThreads, 0, s>>>(a);
+      else if (a.dA == 1)
+        fkd::level_pair_kernel<fkd::kLevelItems, 1><<<tm.T, fkd::kThreads, 0, s>>>(a);
+      else
+        fkd::level_pair_kernel<fkd::kLevelItems, 2><<<tm.T, fkd::kThreads, 0, s>>>(a);
+      FKD_TRY(launch_check("level_pair"));
+      cur[m] ^= 1;
+    }
+  }
+  for (uint32_t L = 0; L < (pairs ? 0u : levels); ++L) {
     fkd::LevelArgs<DIM> a;
     a.L = L;
     a.d = L % DIM;
@@ -698,13 +735,17 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     f.P = PS;
     f.cap = t->fin_cap;
     f.fin_log = t->lay_fin_log;
+    f.pairs = pairs ? 1u : 0u;
     const uint32_t lds = fkd::fin_lds_bytes(DIM, f.cap);
-    if (t->lay_fin_log == 9)
-      fkd::finish_kernel<DIM, 2><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
-    else if (t->lay_fin_log == 10)
-      fkd::finish_kernel<DIM, 4><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
+    const uint32_t items = fkd::fin_items(f.cap);
+    if (items == 1)
+      fkd::finish_kernel<DIM, 1, fkd::kThreads><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
+    else if (items == 2)
+      fkd::finish_kernel<DIM, 2, fkd::kThreads><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
+    else if (items == 4)
+      fkd::finish_kernel<DIM, 4, fkd::kThreads><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
     else
-      fkd::finish_kernel<DIM, 8><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
+      fkd::finish_kernel<DIM, 8, fkd::kThreads><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
     FKD_TRY(launch_check("finish"));
   }
   // 7. Tree arrays and search records from F.
@@ -1725,6 +1766,9 @@ int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
     if (t->graph_count && value != (fl ? t->tune.fin_log : t->tune.build_radix_items))
       return fail(FKD_ERR_STATE, k + " cannot change while a HIP graph of this handle exists");
     (fl ? t->tune.fin_log : t->tune.build_radix_items) = value;
+  } else if (k == "level_pairs") {
+    if (value > 1) return fail(FKD_ERR_INVALID, "level_pairs must be 0 or 1");
+    t->tune.level_pairs = value;
   } else if (k == "fill_window" || k == "count_window") {
     if (value != 0 && value != 256 && value != 1024 && value != 4096)
       return fail(FKD_ERR_INVALID, k + " must be 0, 256, 1024 or 4096");
@@ -1745,6 +1789,7 @@ int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
   else if (k == "key_volume_bits") *value = t->tune.key_volume_bits;
   else if (k == "fin_log") *value = t->tune.fin_log;
   else if (k == "build_radix_items") *value = t->tune.build_radix_items;
+  else if (k == "level_pairs") *value = t->tune.level_pairs;
   else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;
 }

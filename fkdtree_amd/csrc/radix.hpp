@@ -71,9 +71,27 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
     const uint32_t i = ti.base + threadIdx.x + (uint32_t)r * kThreads;
     key[r] = i < ti.end ? K[i] : 0u;
   }
+  // Lanes that share a digit would serialise on one LDS address (a wave of
+  // coordinates that differ only in their upper bits has a handful of
+  // low digits).  Up to 4 times, the digit of the first lane not yet counted
+  // is counted for all its lanes with one add when at least 8 lanes have it;
+  // the remaining lanes add one each.
+  const uint32_t lane = lane_id();
 #pragma unroll
-  for (int r = 0; r < kItems; ++r)
-    if (ti.base + threadIdx.x + (uint32_t)r * kThreads < ti.end) atomicAdd(&hist[wave][(key[r] >> shift) & 255u], 1u);
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t dig = (key[r] >> shift) & 255u;
+    uint64_t act = __ballot(ti.base + threadIdx.x + (uint32_t)r * kThreads < ti.end);
+    for (int it = 0; it < 4 && act != 0ull; ++it) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(act);
+      const uint32_t dl = (uint32_t)__builtin_amdgcn_readlane((int)dig, (int)leader);
+      const uint64_t same = __ballot(dig == dl) & act;
+      const uint32_t c = (uint32_t)__popcll(same);
+      if (c < 8u) break;
+      if (lane == leader) atomicAdd(&hist[wave][dl], c);
+      act &= ~same;
+    }
+    if ((act >> lane) & 1ull) atomicAdd(&hist[wave][dig], 1u);
+  }
   __syncthreads();
   const uint32_t c = hist[0][threadIdx.x] + hist[1][threadIdx.x] + hist[2][threadIdx.x] +
                      hist[3][threadIdx.x];

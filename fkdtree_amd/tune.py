@@ -13,7 +13,8 @@ On a workload it times, with HIP events:
   are finished by one workgroup in LDS: 511, 1,023 or 2,047 points; fewer
   global level launches against bigger finisher workgroups) and
   `build_radix_items` (keys per thread of the presort's digit passes: tiles
-  of 2,048 or 4,096 keys).
+  of 2,048 or 4,096 keys) and, at dim 3, `level_pairs` (one array moved
+  through two depths per launch, or two arrays through one).
 
 The fastest settings go to `tuning.json` next to this file, keyed by GPU
 architecture, dimension, log2 of the (per-event) point count and kind of
@@ -36,7 +37,7 @@ TABLE = os.path.join(HERE, "tuning.json")
 
 CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill_window": (0, 256, 1024, 4096),
               "count_window": (0, 1024, 4096), "key_volume_bits": (0, 3, 4)}
-BUILD_CANDIDATES = {"fin_log": (9, 10, 11), "build_radix_items": (8, 16)}
+BUILD_CANDIDATES = {"fin_log": (9, 10, 11), "build_radix_items": (8, 16), "level_pairs": (1, 0)}
 
 
 _ARCH: Optional[str] = None

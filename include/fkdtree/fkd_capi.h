@@ -144,9 +144,13 @@ int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t str
  *                     above them by one launch per depth;
  *   "build_radix_items" build: keys per thread of the presort's digit
  *                     passes (8 or 16: tiles of 2,048 or 4,096 keys;
- *                     default 16).
- * The two build keys cannot change while a graph of the handle exists
- * (FKD_ERR_STATE); the next build lays the tilings out for new values.
+ *                     default 16);
+ *   "level_pairs"     build, dim 3: 1 (default) moves one point array
+ *                     through two depths per launch (half the bytes of
+ *                     moving two arrays through one), 0 two arrays through
+ *                     one depth.
+ * fin_log and build_radix_items cannot change while a graph of the handle
+ * exists (FKD_ERR_STATE); the next build lays the tilings out for new values.
  * They replace the reference's fixed launch geometry
  * (FKDTree_opencl.h:13-15, 288-289). */
 int fkd_set_tuning(fkd_tree_t tree, const char* key, uint32_t value);

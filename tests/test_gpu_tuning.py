@@ -99,14 +99,15 @@ def test_event_batch_tuning_keeps_results(require_gpu):
     t.close()
 
 
-GEOMETRIES = [(fl, ri) for fl in (9, 10, 11) for ri in (8, 16)]
+GEOMETRIES = [(fl, ri, lp) for fl in (9, 10, 11) for ri in (8, 16) for lp in (1, 0)]
 
 
 @pytest.mark.parametrize("kind,n,dim", [("mixed", 300_000, 3), ("shuffled_ids", 70_000, 3), ("grid", 40_000, 2),
                                         ("uniform", 50_000, 4), ("zeros", 9_000, 1)])
 def test_build_geometry_keeps_trees(require_gpu, kind, n, dim):
     """Finisher capacity (fin_log: subtrees of < 2^fin_log points built in
-    LDS) and digit-pass tile (build_radix_items) change time only: every
+    LDS), digit-pass tile (build_radix_items) and the two-depth partition of
+    dim 3 (level_pairs) change time only: every
     setting builds the oracle's tree bit for bit (FKDTree_cpu.h:221-277), on
     one handle re-laid out between builds, and searches it exactly."""
     from tests.helpers import bits
@@ -117,15 +118,16 @@ def test_build_geometry_keeps_trees(require_gpu, kind, n, dim):
     q = min(n, 5000)
     ooff, oids = oracle.search(oc, oi, boxes.mins[:, :q], boxes.maxs[:, :q])
     t = FKDTree(coords=p.coords, ids=p.ids)
-    for fl, ri in GEOMETRIES:
-        t.set_tuning(fin_log=fl, build_radix_items=ri)
+    for fl, ri, lp in GEOMETRIES:
+        t.set_tuning(fin_log=fl, build_radix_items=ri, level_pairs=lp)
         assert t.get_tuning("fin_log") == fl and t.get_tuning("build_radix_items") == ri
+        assert t.get_tuning("level_pairs") == lp
         t.build()
         c, i = t.tree_arrays()
-        assert np.array_equal(bits(c), bits(oc)) and np.array_equal(i, oi), (fl, ri)
+        assert np.array_equal(bits(c), bits(oc)) and np.array_equal(i, oi), (fl, ri, lp)
         off, ids = t.search_in_the_box_multiple_csr(boxes.mins[:, :q], boxes.maxs[:, :q])
-        assert np.array_equal(off, ooff), (fl, ri)
-        assert np.array_equal(sorted_sets(off, ids), sorted_sets(ooff, oids)), (fl, ri)
+        assert np.array_equal(off, ooff), (fl, ri, lp)
+        assert np.array_equal(sorted_sets(off, ids), sorted_sets(ooff, oids)), (fl, ri, lp)
     t.close()
 
 
@@ -139,18 +141,19 @@ def test_build_geometry_events(require_gpu):
     b = gen.concat_events(evs)
     oc, oi = oracle.build_events(b.points.coords, b.points.ids, b.offsets)
     t = FKDEvents(b.points.coords, b.points.ids, b.offsets)
-    for fl, ri in GEOMETRIES[::-1]:
-        t.set_tuning(fin_log=fl, build_radix_items=ri)
+    for fl, ri, lp in GEOMETRIES[::-1]:
+        t.set_tuning(fin_log=fl, build_radix_items=ri, level_pairs=lp)
         t.build()
         c, i = t.tree_arrays()
-        assert np.array_equal(bits(c), bits(oc)) and np.array_equal(i, oi), (fl, ri)
+        assert np.array_equal(bits(c), bits(oc)) and np.array_equal(i, oi), (fl, ri, lp)
     t.close()
 
 
 def test_build_geometry_rejects_bad_values(require_gpu):
     p = adversarial_points("mixed", 100)
     t = FKDTree(coords=p.coords, ids=p.ids)
-    for k, v in (("fin_log", 8), ("fin_log", 12), ("build_radix_items", 4), ("build_radix_items", 32)):
+    for k, v in (("fin_log", 8), ("fin_log", 12), ("build_radix_items", 4), ("build_radix_items", 32),
+                 ("level_pairs", 2)):
         with pytest.raises(RuntimeError):
             t.set_tuning(**{k: v})
     t.close()

@@ -47,6 +47,7 @@ def test_candidates_are_valid():
     # build geometry: finisher capacities the library instantiates, digit-pass tiles
     assert set(tune.BUILD_CANDIDATES["fin_log"]) <= {9, 10, 11}
     assert set(tune.BUILD_CANDIDATES["build_radix_items"]) <= {8, 16}
+    assert set(tune.BUILD_CANDIDATES["level_pairs"]) <= {0, 1}
 
 
 class _FakeTree:
