@@ -996,9 +996,10 @@ struct FinArgs {
 // NT threads per workgroup (one to four waves).
 template <int DIM, int ITEMS, int NT>
 // Eight waves per SIMD (at most 64 registers) keep a whole launch of
-// half-full subtrees resident (2,048 workgroups on 1M points); dim 4 and
-// 8 items per thread would spill at that bound and keep the default.
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && ITEMS <= 4 ? 8 : 1))) void finish_kernel(FinArgs<DIM> a) {
+// half-full subtrees resident (2,048 workgroups of 2 items per thread on 1M
+// points); more items per thread would spill at that bound and keep the
+// default (1,024 workgroups of 4 items need four waves per SIMD).
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && ITEMS <= 2 ? 8 : 1))) void finish_kernel(FinArgs<DIM> a) {
   constexpr int kFinItems = ITEMS;
   constexpr int MA = DIM > 1 ? DIM - 1 : 1;  // arrays moved per depth
   __shared__ SegCountN<MA> s_wave[NT / kWave];
@@ -1022,63 +1023,178 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && 
 #pragma unroll
   for (int m = 0; m < DIM; ++m)
     pl[m] = a.planes[m][(a.pairs ? pair_moves_before(L0, (uint32_t)m) : moves_before(L0, (uint32_t)m, (uint32_t)DIM)) & 1u];
-  for (uint32_t j = threadIdx.x; j < size0; j += NT) {
+  // All global loads of the subtree at once: array 0's ranks in every
+  // dimension, and the other arrays' ranks in dimension 0 (for their names).
+  constexpr int kS = ITEMS * (DIM - 1);
+  uint32_t r0[kS > 0 ? kS : 1], lo[kS > 0 ? kS : 1];
+  {
+    uint32_t rr[ITEMS][DIM];
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) s_rank(k)[j] = pl[0][(uint64_t)k * P + o + start0 + j];
-    s_arr(0, 0)[j] = (uint16_t)j;
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t j = threadIdx.x + (uint32_t)i * NT;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) rr[i][k] = j < size0 ? pl[0][(uint64_t)k * P + o + start0 + j] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < kS; ++q) {
+      const uint32_t j = threadIdx.x + (uint32_t)(q % ITEMS) * NT;
+      r0[q] = j < size0 ? pl[1 + q / ITEMS][o + start0 + j] : 0u;
+      lo[q] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t j = threadIdx.x + (uint32_t)i * NT;
+      if (j < size0) {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s_rank(k)[j] = rr[i][k];
+        s_arr(0, 0)[j] = (uint16_t)j;
+      }
+    }
   }
   if (sh.D == 0) {  // a single point
     if (threadIdx.x == 0 && n == 1) a.F[o] = pl[0][o];
     return;
   }
   __syncthreads();
-  // names of the other arrays' points: index of their dimension-0 rank in array 0
+  // names of the other arrays' points: index of their dimension-0 rank in
+  // array 0, by a binary search of fixed length (the largest power of two
+  // not above size0 halved each step), all of a thread's searches in step
+  // together so that their LDS reads are in flight at once
+  {
+    for (uint32_t step = 1u << floor_log2(size0); step > 0u; step >>= 1) {
+      uint32_t v[kS > 0 ? kS : 1];
 #pragma unroll
-  for (int m = 1; m < DIM; ++m)
-    for (uint32_t j = threadIdx.x; j < size0; j += NT) {
-      const uint32_t r0 = pl[m][o + start0 + j];
-      uint32_t lo = 0, hi = size0;  // first index with s_rank(0)[idx] >= r0
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_rank(0)[mid] < r0) lo = mid + 1u; else hi = mid;
+      for (int q = 0; q < kS; ++q) {  // every read issued before any is used
+        const uint32_t idx = lo[q] + step - 1u;
+        v[q] = s_rank(0)[idx < size0 ? idx : size0 - 1u];
       }
-      s_arr(m, 0)[j] = (uint16_t)(lo < size0 ? lo : size0 - 1u);  // always found; kept in range
+#pragma unroll
+      for (int q = 0; q < kS; ++q)  // lo = number of entries below r0 seen so far
+        lo[q] += (lo[q] + step <= size0 && v[q] < r0[q]) ? step : 0u;
     }
+#pragma unroll
+    for (int q = 0; q < kS; ++q) {
+      const uint32_t j = threadIdx.x + (uint32_t)(q % ITEMS) * NT;
+      if (j < size0) s_arr(1 + q / ITEMS, 0)[j] = (uint16_t)(lo[q] < size0 ? lo[q] : size0 - 1u);  // always found
+    }
+  }
   __syncthreads();
   uint32_t curbits = 0;  // bit m: the buffer of s_arr(m, .) that holds array m
   const uint32_t first = threadIdx.x * (uint32_t)kFinItems;
+  // The interval of every item, followed down the depths instead of found
+  // from its position each time: at depth L0 it is the whole subtree; below,
+  // the left (right) son's range when the item lies before (after) the
+  // median, none once the item is a node.  ts is the interval's index among
+  // the nodes of its depth.
+  uint32_t st[kFinItems], sz[kFinItems], ts[kFinItems], mp[kFinItems];
+  // Every position becomes a node (or, at the last split, a leaf) at one
+  // depth: its heap index is kept, and F is written once at the end from the
+  // array that held the node at that depth.
+  uint32_t fh[kFinItems];
+  bool inside[kFinItems];
+#pragma unroll
+  for (int i = 0; i < kFinItems; ++i) {
+    st[i] = start0;
+    sz[i] = size0;
+    ts[i] = t0;
+    inside[i] = first + i < size0;
+    fh[i] = 0;
+  }
   for (uint32_t L = L0; L < sh.D; ++L) {
     const uint32_t d = L % (uint32_t)DIM;
     const bool last = L + 1u == sh.D;
     const uint16_t* med = s_arr(d, (curbits >> d) & 1u);
-    uint32_t st[kFinItems], mp[kFinItems];
-    bool inside[kFinItems];
 #pragma unroll
     for (int i = 0; i < kFinItems; ++i) {
-      const uint32_t j = first + i;
-      inside[i] = false;
-      st[i] = mp[i] = 0;
-      if (j < size0) {
-        const uint32_t jl = start0 + j;
-        const uint32_t ts = subtree_at(sh, L, jl);
-        uint32_t sz;
-        subtree_range(sh, L, ts, st[i], sz);
-        inside[i] = jl - st[i] < sz;
-        mp[i] = st[i] + left_size(sz);
-        // node of depth L, or (last split) a leaf of depth D
-        if (inside[i] && (jl == mp[i] || last)) {
-          const uint32_t h = jl == mp[i] ? (1u << L) - 1u + ts : (1u << sh.D) - 1u + subtree_at(sh, sh.D, jl);
-          a.F[o + h] = s_rank(0)[med[j]];
-        }
-      }
+      const uint32_t jl = start0 + first + i;
+      mp[i] = st[i] + left_size(sz[i]);
+      // node of depth L, or (last split) a leaf of depth D: the son on the
+      // item's side of the median; array d holds it at this depth
+      const bool placed = inside[i] && (jl == mp[i] || last);
+      const uint32_t h = jl == mp[i] ? (1u << L) - 1u + ts[i] : (2u << L) - 1u + 2u * ts[i] + (jl > mp[i] ? 1u : 0u);
+      fh[i] = placed ? h : fh[i];
     }
-    if (last || DIM == 1) continue;
+    const auto advance = [&]() {
+#pragma unroll
+      for (int i = 0; i < kFinItems; ++i) {
+        const uint32_t jl = start0 + first + i;
+        const bool right = jl > mp[i];
+        inside[i] = inside[i] && jl != mp[i];
+        sz[i] = right ? st[i] + sz[i] - mp[i] - 1u : mp[i] - st[i];
+        st[i] = right ? mp[i] + 1u : st[i];
+        ts[i] = 2u * ts[i] + (right ? 1u : 0u);
+      }
+    };
+    if (last) continue;
+    if (DIM == 1) {  // one array, split by position only
+      advance();
+      continue;
+    }
     // The DIM-1 arrays other than d (array (d + 1 + k) % DIM for slot k) are
     // partitioned together: one segmented scan of all their counts.
     uint32_t rmed[kFinItems];  // rank in dimension d of the item's interval median
 #pragma unroll
     for (int i = 0; i < kFinItems; ++i) rmed[i] = inside[i] ? s_rank(d)[med[mp[i] - start0]] : 0u;
     uint32_t cls[MA][kFinItems], dest0[MA][kFinItems], who[MA][kFinItems];
+    if constexpr (MA <= 2) {
+      // Packed counts, 12 bits each (a subtree holds fewer than 4,096
+      // points): left / right of array slot k in bits 24k and 24k + 12, bit
+      // 48 an interval start; classes by selects, the scan by DPP moves.
+      constexpr uint64_t kSegF = 1ull << 48;
+      uint64_t seg = 0;
+#pragma unroll
+      for (int i = 0; i < kFinItems; ++i) {
+        const uint32_t jl = start0 + first + i;
+        const bool start = inside[i] && jl == st[i];
+        seg = start ? kSegF : seg;
+#pragma unroll
+        for (int k = 0; k < MA; ++k) {
+          uint32_t m = d + 1u + (uint32_t)k;
+          m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
+          const uint32_t w = inside[i] ? s_arr(m, (curbits >> m) & 1u)[first + i] : 0u;
+          const uint32_t r = s_rank(d)[w];
+          const bool lo = r < rmed[i], hi = r > rmed[i];
+          who[k][i] = w;
+          cls[k][i] = !inside[i] ? kNone : (lo ? kLeft : hi ? kRight : kMed) | (start ? 4u : 0u);
+          dest0[k][i] = lo ? st[i] : hi ? mp[i] + 1u : mp[i];
+          seg += inside[i] ? ((lo ? 1ull : 0ull) << (24 * k)) + ((hi ? 1ull : 0ull) << (24 * k + 12)) : 0ull;
+        }
+      }
+      const auto seg_add = [](uint64_t x, uint64_t y) { return (y & kSegF) ? y : x + y; };
+      __shared__ uint64_t s_w64[NT / kWave];
+      const uint32_t lane = lane_id();
+      const int wave = threadIdx.x / kWave;
+      const uint64_t incl = wave_scan64(seg, seg_add);
+      uint64_t excl = ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(incl >> 32), 1) << 32) |
+                      (uint32_t)__shfl_up((int)(uint32_t)incl, 1);
+      if (lane == 0) excl = 0;
+      if (lane == kWave - 1) s_w64[wave] = incl;
+      __syncthreads();
+      uint64_t wcarry = 0;
+#pragma unroll
+      for (int w = 0; w < NT / kWave; ++w)
+        if (w < wave) wcarry = seg_add(wcarry, s_w64[w]);
+      excl = seg_add(wcarry, excl);
+#pragma unroll
+      for (int k = 0; k < MA; ++k) {
+        uint32_t m = d + 1u + (uint32_t)k;
+        m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
+        uint16_t* dst = s_arr(m, ((curbits >> m) & 1u) ^ 1u);
+        uint32_t nl = (uint32_t)(excl >> (24 * k)) & 0xfffu, nr = (uint32_t)(excl >> (24 * k + 12)) & 0xfffu;
+#pragma unroll
+        for (int i = 0; i < kFinItems; ++i) {
+          const uint32_t c = cls[k][i];
+          nl = (c & 4u) ? 0u : nl;
+          nr = (c & 4u) ? 0u : nr;
+          const bool l = (c & 3u) == kLeft, r = (c & 3u) == kRight;
+          const uint32_t dest = dest0[k][i] + (l ? nl : r ? nr : 0u);
+          nl += l ? 1u : 0u;
+          nr += r ? 1u : 0u;
+          // elements outside every interval write the spare slot at index cap
+          dst[(c & 3u) != kNone ? dest - start0 : cap] = (uint16_t)who[k][i];
+        }
+      }
+    } else {
     SegCountN<MA> agg = seg_zero<MA>();
 #pragma unroll
     for (int i = 0; i < kFinItems; ++i) {
@@ -1138,6 +1254,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && 
         if ((c & 3u) != kNone) dst[dest - start0] = (uint16_t)who[k][i];
       }
     }
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < MA; ++k) {
@@ -1145,6 +1262,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && 
       m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
       curbits ^= 1u << m;
     }
+    advance();
+  }
+  // F from the array split at the depth where each position was placed (the
+  // depth of its heap index; D - 1 for the leaves of depth D), in the buffer
+  // it held then: array m moves on every depth of [L0, L) except those with
+  // L % DIM == m, and starts in buffer 0.
+  const auto same = [](uint32_t x, uint32_t m) { return (x + (uint32_t)DIM - 1u - m) / (uint32_t)DIM; };
+#pragma unroll
+  for (int i = 0; i < kFinItems; ++i) {
+    const uint32_t j = first + i;
+    if (j >= size0) continue;
+    const uint32_t hd = floor_log2(fh[i] + 1u), L = hd < sh.D ? hd : sh.D - 1u, m = L % (uint32_t)DIM;
+    const uint32_t moves = (L - L0) - (same(L, m) - same(L0, m));
+    a.F[o + fh[i]] = s_rank(0)[s_arr(m, moves & 1u)[j]];
   }
 }
 

@@ -14,9 +14,11 @@ __host__ __device__ inline uint32_t floor_log2(uint32_t x) { return 31u - (uint3
 // `len` nodes: the rank of the median the reference selects for an interval
 // of that length (partition_complete_kdtree, FKDTree_cpu.h:287-296).
 __host__ __device__ inline uint32_t left_size(uint32_t len) {
-  if (len <= 1) return 0;
-  const uint32_t p = 1u << floor_log2(len);
-  return (p / 2 - 1 <= len - p) ? p - 1 : len - p / 2;
+  // selects only (no branch in the build's per-item loops); for len <= 1 the
+  // formula's value is discarded
+  const uint32_t p = 1u << floor_log2(len), h = p >> 1;
+  const uint32_t v = (h - 1u <= len - p) ? p - 1u : len - h;
+  return len <= 1u ? 0u : v;
 }
 
 // Shape of the heap-ordered left-complete tree of n nodes: deepest depth D and

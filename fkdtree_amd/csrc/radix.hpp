@@ -127,9 +127,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
 #pragma unroll
   for (int w = 0; w < 4; ++w) cnt[w][threadIdx.x] = 0;
   const TileInfo ti = tile_info<kTile>(tm, t);
-  goff[threadIdx.x] =
-      offsets[(uint64_t)m * 256u * tm.T + 256ull * (t - ti.lt) + (uint64_t)threadIdx.x * ti.nt + ti.lt] -
-      m * P;
+  // the tile's first slot of digit threadIdx.x, needed only for the write-out:
+  // loaded now, stored to LDS with the staging
+  const uint32_t my_goff =
+      offsets[(uint64_t)m * 256u * tm.T + 256ull * (t - ti.lt) + (uint64_t)threadIdx.x * ti.nt + ti.lt] - m * P;
   __syncthreads();
   const uint32_t* Ki = keys_in + m * stride;
   const uint32_t* Vi = vals_in + m * stride;
@@ -174,6 +175,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
 #pragma unroll
     for (int w = 0; w < kThreads / kWave; ++w) ts += (w < wave) ? s_wsum[w] : 0u;
     tstart[d] = ts;
+    goff[d] = my_goff;
     cnt[0][d] = ts;
     cnt[1][d] = ts + c0;
     cnt[2][d] = ts + c0 + c1;
