@@ -42,41 +42,32 @@ __global__ void aos_to_soa_kernel(const uint32_t* __restrict__ aos, float* __res
 
 // Keys and values for the coordinate sorts.  order == nullptr means point
 // order already is (event, id) order.  Sets bit 0 of *err on a NaN.  Also
-// sets the bounding box that bbox_kernel narrows to its starting values
-// (min keys all ones, max keys zero).
+// sets the bounding box that the last digit pass narrows to its starting
+// values (min keys all ones, max keys zero), and, when rec is given (dim <= 3,
+// point order), the point records of records_kernel from the same loads.
 template <int DIM>
 __global__ void keys_kernel(const float* __restrict__ X, const uint32_t* __restrict__ order,
                             uint32_t P, uint32_t* __restrict__ K, uint32_t* __restrict__ V,
-                            uint64_t stride, uint32_t* err, uint32_t* bbox) {
+                            uint64_t stride, uint32_t* err, uint32_t* bbox, const uint32_t* __restrict__ ids,
+                            float4* __restrict__ rec) {
   if (blockIdx.x == 0 && threadIdx.x < 2u * DIM) bbox[threadIdx.x] = threadIdx.x < (uint32_t)DIM ? 0xffffffffu : 0u;
   bool nan = false;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
     const uint32_t p = order ? order[i] : i;
+    float c[DIM];
 #pragma unroll
     for (int m = 0; m < DIM; ++m) {
       const float x = X[(uint64_t)m * P + p];
+      c[m] = x;
       nan |= (x != x);
       K[m * stride + i] = fkey(x);
       V[m * stride + i] = p;
     }
+    if (DIM <= 3 && rec)
+      rec[i] = make_float4(c[0], DIM > 1 ? c[DIM > 1 ? 1 : 0] : 0.f, DIM > 2 ? c[DIM > 2 ? 2 : 0] : 0.f,
+                           __uint_as_float(ids[i]));
   }
   if (nan) atomicOr(err, 1u);
-}
-
-// Key-space bounding box of all events from the sorted arrays (first / last
-// element of every event): bbox[m] = min key, bbox[DIM + m] = max key.
-template <int DIM>
-__global__ void bbox_kernel(const float* __restrict__ X, const uint32_t* __restrict__ A, uint64_t stride,
-                            const uint32_t* __restrict__ off, uint32_t n_events, uint32_t P, uint32_t* bbox) {
-  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n_events; e += gridDim.x * blockDim.x) {
-    const uint32_t a = off[e], b = off[e + 1];
-    if (a == b) continue;
-#pragma unroll
-    for (int m = 0; m < DIM; ++m) {
-      atomicMin(&bbox[m], fkey(X[(uint64_t)m * P + A[m * stride + a]]));
-      atomicMax(&bbox[DIM + m], fkey(X[(uint64_t)m * P + A[m * stride + b - 1]]));
-    }
-  }
 }
 
 // Rank planes of the level arrays: array m at position j holds the ranks of

@@ -527,7 +527,7 @@ template <int ITEMS>
 int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts, uint32_t* offsets,
                uint64_t* desc, uint32_t* ctr, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
                uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift, bool write_keys,
-               bool write_ranks, hipStream_t s) {
+               bool write_ranks, hipStream_t s, uint32_t* bbox = nullptr) {
   const uint64_t nc = (uint64_t)M * 256u * tm.T;
   dim3 grid(tm.T, M);
   fkd::radix_count_kernel<ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts);
@@ -537,32 +537,32 @@ int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts,
   FKD_TRY(launch_check("radix_scan"));
   if (write_keys && !write_ranks)
     fkd::radix_scatter_kernel<true, false, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
-                                                                                 P, shift, tm, offsets);
+                                                                                 P, shift, tm, offsets, bbox);
   else if (!write_keys && write_ranks)
     fkd::radix_scatter_kernel<false, true, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
-                                                                                 P, shift, tm, offsets);
+                                                                                 P, shift, tm, offsets, bbox);
   else if (write_keys && write_ranks)
     fkd::radix_scatter_kernel<true, true, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
-                                                                                P, shift, tm, offsets);
+                                                                                P, shift, tm, offsets, bbox);
   else
     fkd::radix_scatter_kernel<false, false, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride,
-                                                                                  ranks, P, shift, tm, offsets);
+                                                                                  ranks, P, shift, tm, offsets, bbox);
   return launch_check("radix_scatter");
 }
 
 template <int DIM>
 int run_radix_pass(fkd_tree_s* t, ZeroArena& za, uint32_t M, const uint32_t* kin, uint32_t* kout,
                    const uint32_t* vin, uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift,
-                   bool write_keys, bool write_ranks, hipStream_t s) {
+                   bool write_keys, bool write_ranks, hipStream_t s, uint32_t* bbox = nullptr) {
   const fkd::TileMap tm = t->radix_tiles.map(t->off.as<uint32_t>());
   uint64_t* desc = za.take(radix_pass_words(M, tm.T));
   if (!desc) return arena_exhausted();
   uint32_t* ctr = reinterpret_cast<uint32_t*>(desc + fkd::scan_tiles((uint64_t)M * 256u * tm.T));
   if (t->lay_radix_items == 8)
     return radix_pass<8>(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, kin, kout, vin,
-                         vout, stride, ranks, shift, write_keys, write_ranks, s);
+                         vout, stride, ranks, shift, write_keys, write_ranks, s, bbox);
   return radix_pass<16>(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, kin, kout, vin,
-                        vout, stride, ranks, shift, write_keys, write_ranks, s);
+                        vout, stride, ranks, shift, write_keys, write_ranks, s, bbox);
 }
 
 // Level tiles of the handle; one tree needs no tile table (its kernels take
@@ -626,25 +626,26 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     order = R;  // R holds the (event, id) order until the last key pass writes the ranks
   }
   // 2. Keys of every coordinate, in tie order.
+  // point records (search nodes' payload) with the keys when point order is
+  // the tie order; otherwise records_kernel below
+  float4* fused_rec = (DIM <= 3 && !order) ? t->Xr.as<float4>() : nullptr;
   fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>(),
-                                             t->bbox.as<uint32_t>());
+                                             t->bbox.as<uint32_t>(), t->ids.as<uint32_t>(), fused_rec);
   FKD_TRY(launch_check("keys"));
   // 3. Stable LSD sort of every coordinate inside every event; the last pass
-  //    stores each point's rank R[p][m] (the values end in buffer 0).
+  //    stores each point's rank R[p][m] (the values end in buffer 0) and
+  //    narrows the key-space bounding box (used to order search queries)
+  //    from the starting values keys_kernel set.
   {
     int c = 0;
     for (int pass = 0; pass < 4; ++pass) {
       const bool lastp = pass == 3;
       FKD_TRY(run_radix_pass<DIM>(t, za, DIM, Kbuf(0, c), Kbuf(0, 1 - c), Abuf(0, c), Abuf(0, 1 - c), 2 * S,
-                                  lastp ? R : nullptr, 8u * pass, !lastp, lastp, s));
+                                  lastp ? R : nullptr, 8u * pass, !lastp, lastp, s,
+                                  lastp ? t->bbox.as<uint32_t>() : nullptr));
       c = 1 - c;
     }
   }
-  // Bounding box (used to order search queries) from the sorted arrays;
-  // keys_kernel set its starting values.
-  fkd::bbox_kernel<DIM><<<grid_for(t->n_events), 256, 0, s>>>(X, Abuf(0, 0), 2 * S, t->off.as<uint32_t>(),
-                                                               t->n_events, P, t->bbox.as<uint32_t>());
-  FKD_TRY(launch_check("bbox"));
   // 4. Rank planes: every array element carries its ranks in all dimensions.
   uint32_t* PL = t->planes.as<uint32_t>();  // PL[m][c][k][PS]
   const uint64_t PS = fkd::plane_stride(P);
@@ -750,7 +751,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   }
   // 7. Tree arrays and search records from F.
   {
-    if (DIM <= 3) {
+    if (DIM <= 3 && !fused_rec) {
       fkd::records_kernel<DIM><<<grid, 256, 0, s>>>(X, t->ids.as<uint32_t>(), P, t->Xr.as<float4>());
       FKD_TRY(launch_check("records"));
     }

@@ -101,7 +101,9 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
 // Stable scatter of one digit pass.  offsets = exclusive scan of the counts of
 // all arrays (array m's entries start at m * P).  Keys are written only when
 // WRITE_KEYS; when WRITE_RANKS, ranks[4 val + m] = final slot of val (the
-// ranks of a point in all dimensions form one 16-byte record).
+// ranks of a point in all dimensions form one 16-byte record), and, when
+// bbox is given, the keys that land first and last in their event narrow
+// bbox[m] (min) and bbox[M + m] (max), M = gridDim.y.
 // The tile is first ordered by digit in LDS (stable: wave by wave, item by
 // item, lane by lane), then written out in that order, so consecutive threads
 // write consecutive slots of one digit's run instead of up to 64 buckets per
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, uint32_t* __restrict__ keys_out,
     const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out, uint64_t stride,
     uint32_t* __restrict__ ranks, uint32_t P, uint32_t shift, TileMap tm,
-    const uint32_t* __restrict__ offsets) {
+    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ bbox) {
   constexpr int kItems = ITEMS;
   constexpr int kTile = kThreads * ITEMS;
   constexpr int kPerWave = kTile / (kThreads / kWave);
@@ -196,6 +198,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   uint32_t* Ko = keys_out + m * stride;
   uint32_t* Vo = vals_out + m * stride;
   const uint32_t tn = ti.end - ti.base;
+  // event range of the tile (bounding box of the sorted keys)
+  const uint32_t eo = tm.tile_ev ? tm.off[ti.e] : 0u, eo1 = tm.tile_ev ? tm.off[ti.e + 1] : tm.single_n;
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const uint32_t i = threadIdx.x + (uint32_t)r * kThreads;
@@ -205,7 +209,11 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       const uint32_t dest = goff[dig] + (i - tstart[dig]);
       Vo[dest] = v;
       if (WRITE_KEYS) Ko[dest] = k;
-      if (WRITE_RANKS) ranks[4 * (uint64_t)v + m] = dest;
+      if (WRITE_RANKS) {
+        ranks[4 * (uint64_t)v + m] = dest;
+        if (bbox && dest == eo) atomicMin(&bbox[m], k);
+        if (bbox && dest + 1u == eo1) atomicMax(&bbox[gridDim.y + m], k);
+      }
     }
   }
 }
