@@ -151,12 +151,15 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t pos = wbase + (uint32_t)r * kWave;
     const bool valid = pos < ti.end;
     const uint32_t dig = (key[r] >> shift) & 255u;
+    // lanes with the same digit: for every bit, the ballot of the lanes that
+    // have it set, complemented where this lane's bit is clear (s is all ones
+    // for a set bit, so ~(ballot ^ s) selects without a per-lane branch)
     uint64_t match = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      const bool bit = (dig >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      match &= bit ? bal : ~bal;
+      const uint32_t sb = (uint32_t)__builtin_amdgcn_sbfe((int)dig, (uint32_t)b, 1u);
+      const uint64_t bal = __ballot(sb != 0u);
+      match &= ~(bal ^ (((uint64_t)sb << 32) | sb));
     }
     uint32_t pre = 0;
     if (valid) pre = cnt[wave][dig];
