@@ -73,35 +73,48 @@ __global__ void keys_kernel(const float* __restrict__ X, const uint32_t* __restr
 // Rank planes of the level arrays: array m at position j holds the ranks of
 // its point in every dimension k: (k == m) ? j : R[A_m[j]][k], where the
 // ranks of a point form one 16-byte record (one gather per array element).
-// Plane k of an array starts at k * ps; ps = plane_stride(P).
+// Plane k of an array starts at k * ps; ps = plane_stride(P).  Work items
+// are (radix tile, array) pairs, XCD-chunked as in the last digit pass that
+// wrote R, so that an XCD gathers the records of the events it wrote.
 template <int DIM>
-__global__ void planes_kernel(const uint32_t* __restrict__ A, uint64_t astride, const uint4* __restrict__ R,
-                              uint32_t P, uint32_t* __restrict__ planes, uint64_t ps, uint64_t pstride) {
+__global__ __launch_bounds__(kThreads) void planes_kernel(const uint32_t* __restrict__ A, uint64_t astride,
+                                                          const uint4* __restrict__ R, TileMap tm, uint32_t tile,
+                                                          uint32_t* __restrict__ planes, uint64_t ps,
+                                                          uint64_t pstride) {
   static_assert(DIM <= 4, "rank records hold 4 dimensions");
-  constexpr int U = 4;  // elements per thread and round, their gathers in flight together
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t j0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j0 < P; j0 += U * stride) {
-    uint32_t pt[U][DIM];
-    uint4 rr[U][DIM];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int m = 0; m < DIM; ++m) pt[u][m] = j0 + u * stride < P ? A[m * astride + j0 + u * stride] : 0u;
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int m = 0; m < DIM; ++m) rr[u][m] = j0 + u * stride < P ? R[pt[u][m]] : make_uint4(0u, 0u, 0u, 0u);
+  constexpr int U = 16;  // elements per thread, their gathers in flight together
+  const uint32_t w = xcd_item(blockIdx.x, gridDim.x);
+  const uint32_t t = w / (uint32_t)DIM, m = w % (uint32_t)DIM;
+  uint32_t base, end;
+  if (!tm.tile_ev) {
+    base = t * tile;
+    end = min(base + tile, tm.single_n);
+  } else {
+    const uint32_t e = tm.tile_ev[t];
+    base = tm.off[e] + (t - tm.tb[e]) * tile;
+    end = min(base + tile, tm.off[e + 1]);
+  }
+  uint32_t* dst = planes + m * pstride;
+  for (uint32_t j0 = base + threadIdx.x; j0 < end; j0 += U * kThreads) {
+    uint32_t pt[U];
+    uint4 rr[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t j = j0 + u * stride;
-      if (j >= P) continue;
+      const uint32_t j = j0 + (uint32_t)u * kThreads;
+      pt[u] = j < end ? A[m * astride + j] : 0u;
+    }
 #pragma unroll
-      for (int m = 0; m < DIM; ++m) {
-        const uint32_t rk[4] = {rr[u][m].x, rr[u][m].y, rr[u][m].z, rr[u][m].w};
-        uint32_t* dst = planes + m * pstride;
+    for (int u = 0; u < U; ++u) {
+      const uint32_t j = j0 + (uint32_t)u * kThreads;
+      rr[u] = j < end ? R[pt[u]] : make_uint4(0u, 0u, 0u, 0u);
+    }
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * ps + j] = (k == m) ? (uint32_t)j : rk[k];
-      }
+    for (int u = 0; u < U; ++u) {
+      const uint32_t j = j0 + (uint32_t)u * kThreads;
+      if (j >= end) continue;
+      const uint32_t rk[4] = {rr[u].x, rr[u].y, rr[u].z, rr[u].w};
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) dst[(uint64_t)k * ps + j] = (k == (int)m) ? j : rk[k];
     }
   }
 }
@@ -1001,8 +1014,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && 
   uint16_t* const s_arr_base = reinterpret_cast<uint16_t*>(fin_lds + (uint64_t)DIM * cap);
   auto s_rank = [&](uint32_t k) { return s_rank_base + (uint64_t)k * cap; };
   auto s_arr = [&](uint32_t m, uint32_t c) { return s_arr_base + ((uint64_t)m * 2u + c) * (cap + 1u); };
-  const uint32_t e = a.fm.tile_ev[blockIdx.x];
-  const uint32_t t0 = blockIdx.x - a.fm.tb[e];
+  const uint32_t sub = xcd_item(blockIdx.x, gridDim.x);  // an XCD takes consecutive subtrees (whole events)
+  const uint32_t e = a.fm.tile_ev[sub];
+  const uint32_t t0 = sub - a.fm.tb[e];
   const uint32_t o = a.fm.off[e];
   const uint32_t n = a.fm.off[e + 1] - o;
   const Shape sh = shape_of(n);
@@ -1356,7 +1370,7 @@ __global__ void records_kernel(const float* __restrict__ X, const uint32_t* __re
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void output_kernel(LevelArgs<DIM> a, const float4* __restrict__ rec) {
   constexpr int IT = kLevelTile / kThreads;
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = xcd_item(blockIdx.x, gridDim.x);  // an XCD takes consecutive tiles (whole events)
   const LevelTile lti = level_tile(a.tm, t);
   const uint32_t lt = lti.lt, o = lti.o, n = lti.n;
   const uint32_t v0 = lt * (uint32_t)kLevelTile + threadIdx.x;

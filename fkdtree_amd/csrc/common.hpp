@@ -170,6 +170,16 @@ __device__ inline uint32_t dynamic_block(uint32_t* ctr, uint32_t* lds) {
   return b;
 }
 
+// Workgroups are handed to the 8 XCDs in turn (linear workgroup id modulo
+// 8).  XCD x's share of nb work items is the contiguous range starting at
+// xcd_chunk_start(x, nb); xcd_item maps a linear workgroup id to its item, so
+// that every XCD works on one contiguous range and its L2 holds that range's
+// data (for instance whole events).
+__host__ __device__ inline uint32_t xcd_chunk_start(uint32_t x, uint32_t nb) {
+  return x * (nb >> 3) + (x < (nb & 7u) ? x : (nb & 7u));
+}
+__device__ inline uint32_t xcd_item(uint32_t lid, uint32_t nb) { return xcd_chunk_start(lid & 7u, nb) + (lid >> 3); }
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)

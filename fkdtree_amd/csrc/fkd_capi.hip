@@ -650,8 +650,12 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   uint32_t* PL = t->planes.as<uint32_t>();  // PL[m][c][k][PS]
   const uint64_t PS = fkd::plane_stride(P);
   auto Pbuf = [&](int m, int c) { return PL + (((uint64_t)m * 2 + c) * DIM) * PS; };
-  fkd::planes_kernel<DIM><<<grid, 256, 0, s>>>(Abuf(0, 0), 2 * S, reinterpret_cast<const uint4*>(R), P, Pbuf(0, 0),
-                                               PS, 2 * (uint64_t)DIM * PS);
+  {
+    const fkd::TileMap rtm = t->radix_tiles.map(t->off.as<uint32_t>());
+    fkd::planes_kernel<DIM><<<rtm.T * DIM, fkd::kThreads, 0, s>>>(Abuf(0, 0), 2 * S, reinterpret_cast<const uint4*>(R),
+                                                                  rtm, fkd::kThreads * t->lay_radix_items, Pbuf(0, 0), PS,
+                                                                  2 * (uint64_t)DIM * PS);
+  }
   FKD_TRY(launch_check("planes"));
   // 5. One partition launch per depth.
   int cur[4] = {0, 0, 0, 0};

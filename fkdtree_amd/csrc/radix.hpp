@@ -123,7 +123,12 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   __shared__ uint32_t s_wsum[kThreads / kWave];
   __shared__ uint32_t s_key[kTile];
   __shared__ uint32_t s_val[kTile];
-  const uint32_t t = blockIdx.x, m = blockIdx.y;
+  // (tile, array) from the linear workgroup id, tile-major and XCD-chunked:
+  // an XCD takes consecutive tiles, so the ranks of an event's points (its
+  // 16-byte records, written at random inside the event's range) are
+  // written by one XCD and merge in its L2
+  const uint32_t w = xcd_item(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const uint32_t t = w / gridDim.y, m = w % gridDim.y;
   const int wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
 #pragma unroll

@@ -46,9 +46,6 @@ struct SearchArgs {
 // launch takes position start(b % 8) + b / 8, so that every XCD walks one
 // contiguous range of the spatially ordered queries and its L2 holds that
 // range's part of the tree; `order` (if any) maps positions to blocks.
-__host__ __device__ inline uint32_t xcd_chunk_start(uint32_t x, uint32_t nb) {
-  return x * (nb >> 3) + (x < (nb & 7u) ? x : (nb & 7u));
-}
 __device__ inline uint32_t search_block(uint32_t b, uint32_t nb, const uint32_t* order) {
   const uint32_t j = xcd_chunk_start(b & 7u, nb) + (b >> 3);
   return order ? order[j] : j;
