@@ -31,6 +31,11 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
+def progress(msg: str) -> None:
+    """One line on stderr as each section ends (the JSON result goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +114,7 @@ def main():
         tree.build_async()
     tm.stop(stream)
     build_ms = tm.ms() / args.steps
+    progress(f"build {build_ms:.3f} ms")
 
     # --- search: size the output once ------------------------------------
     d_off = DeviceArray(q + 1, np.uint64)
@@ -152,10 +158,12 @@ def main():
         return step_ms, wall, kms
 
     search_ms, wall_ms, fill_ms = timed(two_pass, True)
+    progress(f"search {search_ms:.3f} ms (fill {fill_ms:.3f} ms)")
     per_box = None
     threads = [int(x) for x in str(args.per_box_threads).split(",") if x.strip() and int(x) > 0]
     if threads:
         per_box = bench_per_box(pts, boxes, d_off.download(stream), d_out, threads)
+        progress("per-box pattern done")
 
     V = int(visited.sum())
     H = total
@@ -218,8 +226,10 @@ def main():
     del d_out
     if args.events:
         res["events"] = bench_events(args.events, args.event_steps)
+        progress("events done")
     if args.serve_events:
         res["serving"] = bench_serving(args.serve_events, [float(r) for r in args.serve_rates.split(",")])
+        progress("serving done")
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(pts, boxes, args.cpu_threads, args.cpu_events if args.events else 0)
     print(json.dumps(res), flush=True)
@@ -291,6 +301,7 @@ def bench_serving(n_events: int, rates) -> list:
         srv.close()
         d = rep.summary()
         d.update({"rate_hz": rate, "points": int(sum(r.n for r in reqs))})
+        progress(f"serving at {rate:.0f} events/s: {d.get('events_per_s', 0):.0f} events/s")
         out.append(d)
     return out
 
@@ -367,6 +378,7 @@ def bench_events(n_events: int, steps: int, batched_only: bool = False) -> dict:
         batched()
     tm.stop(s)
     batched_ms = tm.ms() / steps
+    progress(f"events: {n_events} batched {batched_ms:.2f} ms")
     if batched_only:
         ev.close()
         return {"n_events": n_events, "points": P, "queries": P, "hits": int(total), "batched_ms": batched_ms,
@@ -379,6 +391,7 @@ def bench_events(n_events: int, steps: int, batched_only: bool = False) -> dict:
         g.launch()
     tm.stop(s)
     graph_ms = tm.ms() / steps
+    progress(f"events: graph {graph_ms:.2f} ms")
     tot_g, ok_g = ev.search_status(s)
     assert ok_g and tot_g == total
     g.close()
