@@ -399,6 +399,10 @@ struct fkd_tuning {
   uint32_t fin_log = fkd::kFinLogDefault;  // build: subtrees of < 2^fin_log points are finished in LDS
   uint32_t build_radix_items = fkd::kRadixItems;  // build: keys per thread of the digit passes (8 or 16)
   uint32_t level_pairs = 1;     // build, dim 3: one array moved through two depths per launch
+  uint32_t bucket = 0;          // fill pass, dim <= 3: bottom subtrees tested node by node (bucket_search_kernel):
+                                // 0 walk every node, 1 lanes park at buckets, 2 lanes queue them
+  uint32_t count_bucket = 0;    // count pass: the same choice
+  uint32_t bucket_park = 8;     // bucketed search: eighths of the live lanes parked before the bucket tests run
 };
 
 struct fkd_tree_s {
@@ -784,10 +788,32 @@ int launch_search(fkd_tree_s* t, const fkd::SearchArgs& a, hipStream_t s, const 
   const uint32_t blocks = std::max<uint32_t>(1, (a.nq + fkd::kThreads - 1) / fkd::kThreads);
   const uint64_t bytes = (uint64_t)t->P * fkd::NodeRec<DIM>::kFloats * 4;
   b.node_bytes = (uint32_t)std::min<uint64_t>(bytes, 0xffffffffull);
-  if (bytes < (1ull << 32))
+  b.bucket_park = t->tune.bucket_park;
+  constexpr bool kBucketable = DIM <= 3 && MODE != fkd::kStats;
+  const uint32_t sched = MODE == fkd::kFill ? t->tune.bucket : t->tune.count_bucket;
+  if (kBucketable && sched) {
+    constexpr int BD = DIM <= 3 ? DIM : 3;
+    constexpr int BM = MODE == fkd::kStats ? fkd::kCount : MODE;
+    const bool buf = bytes < (1ull << 32);
+    if (sched == 1) {
+      if (buf) fkd::bucket_search_kernel<BD, BM, true, 0><<<blocks, fkd::kThreads, 0, s>>>(b);
+      else fkd::bucket_search_kernel<BD, BM, false, 0><<<blocks, fkd::kThreads, 0, s>>>(b);
+    } else {
+      constexpr int BQ = BM == fkd::kFill ? fkd::kBucketQueue : fkd::kBucketQueueCount;
+      if (buf) fkd::bucket_search_kernel<BD, BM, true, BQ><<<blocks, fkd::kThreads, 0, s>>>(b);
+      else fkd::bucket_search_kernel<BD, BM, false, BQ><<<blocks, fkd::kThreads, 0, s>>>(b);
+    }
+  } else if (MODE == fkd::kCount && DIM <= 3 && t->tune.bucket && b.work_slot) {
+    // the fill pass is bucketed: its window sort wants the bucketed work
+    if (bytes < (1ull << 32))
+      fkd::search_kernel<DIM, MODE, true, true><<<blocks, fkd::kThreads, 0, s>>>(b);
+    else
+      fkd::search_kernel<DIM, MODE, false, true><<<blocks, fkd::kThreads, 0, s>>>(b);
+  } else if (bytes < (1ull << 32)) {
     fkd::search_kernel<DIM, MODE, true><<<blocks, fkd::kThreads, 0, s>>>(b);
-  else
+  } else {
     fkd::search_kernel<DIM, MODE, false><<<blocks, fkd::kThreads, 0, s>>>(b);
+  }
   return launch_check(what);
 }
 
@@ -1774,6 +1800,12 @@ int fkd_set_tuning(fkd_tree_t t, const char* key, uint32_t value) {
   } else if (k == "level_pairs") {
     if (value > 1) return fail(FKD_ERR_INVALID, "level_pairs must be 0 or 1");
     t->tune.level_pairs = value;
+  } else if (k == "bucket" || k == "count_bucket") {
+    if (value > 2) return fail(FKD_ERR_INVALID, k + " must be 0, 1 or 2");
+    (k == "bucket" ? t->tune.bucket : t->tune.count_bucket) = value;
+  } else if (k == "bucket_park") {
+    if (value > 8) return fail(FKD_ERR_INVALID, "bucket_park must be 0..8");
+    t->tune.bucket_park = value;
   } else if (k == "fill_window" || k == "count_window") {
     if (value != 0 && value != 256 && value != 1024 && value != 4096)
       return fail(FKD_ERR_INVALID, k + " must be 0, 256, 1024 or 4096");
@@ -1795,6 +1827,9 @@ int fkd_get_tuning(fkd_tree_t t, const char* key, uint32_t* value) {
   else if (k == "fin_log") *value = t->tune.fin_log;
   else if (k == "build_radix_items") *value = t->tune.build_radix_items;
   else if (k == "level_pairs") *value = t->tune.level_pairs;
+  else if (k == "bucket") *value = t->tune.bucket;
+  else if (k == "count_bucket") *value = t->tune.count_bucket;
+  else if (k == "bucket_park") *value = t->tune.bucket_park;
   else return fail(FKD_ERR_INVALID, "unknown tuning key " + k);
   return FKD_OK;
 }

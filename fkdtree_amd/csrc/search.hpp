@@ -40,6 +40,7 @@ struct SearchArgs {
   uint64_t cap;           // capacity of out
   uint32_t out_aligned;   // out is 64-byte aligned
   uint32_t node_bytes;    // size of the node records (buffer resource range)
+  uint32_t bucket_park;   // bucketed passes: parked eighths of the live lanes that start the bucket tests
 };
 
 // Workgroups are handed to the 8 XCDs in turn.  Workgroup b of a search
@@ -93,6 +94,8 @@ enum SearchMode { kCount = 0, kFill = 1, kStats = 2 };
 // lane and line.
 constexpr int kRing = 16;
 constexpr int kRingRow = 16;  // words per lane (16-byte aligned rows)
+constexpr int kBucketQueue = 8;       // bucket roots queued per lane in the fill pass (bucketed traversal)
+constexpr int kBucketQueueCount = 4;  // in the count pass (its LDS allows 8 waves per SIMD)
 
 // The loop body is branch free except for writing out staged ids: the next
 // node is selected among (left son, right son, pending right son of the
@@ -101,7 +104,11 @@ constexpr int kRingRow = 16;  // words per lane (16-byte aligned rows)
 // box rotated like the node records of depth d (split dimension first) is
 // the DIM pairs starting at pair d % DIM, found through a 32-entry table:
 // no selection by dimension in the loop.
-template <int DIM, int MODE, bool BUF>
+// WB (count pass): report each box's work as the bucketed fill pass will do
+// it (a step per node down to the bucket depth B, none inside buckets; B as in
+// bucket_search_kernel) instead of nodes visited, so the fill's window sort
+// balances the bucketed walk.
+template <int DIM, int MODE, bool BUF, bool WB = false>
 __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   constexpr bool FILL = MODE == kFill;
   constexpr bool LDS_BOX = DIM >= 2;
@@ -175,8 +182,18 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   // bit 0 is a sentinel, so the deepest pending depth is 31 - clz(pending).
   uint32_t node = 0, depth = 0, pending = 1, vis = 0;
   bool alive = n != 0;
+  uint32_t wb_depth = 32;
+  if (WB) {
+    const uint32_t F = 31u - (uint32_t)__builtin_clz(n + 1u);
+    wb_depth = F > 3u ? F - 3u : (F == 3u ? 0u : 32u);
+  }
   while (alive) {
-    if (!FILL) ++vis;
+    if (!FILL) {
+      if (WB)
+        vis += depth <= wb_depth ? 1u : 0u;
+      else
+        ++vis;
+    }
     float x[DIM], lo[DIM], hi[DIM];
     uint32_t id;
     load_rec<DIM, BUF>(src, base + node, x, id);
@@ -231,6 +248,296 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   if (!FILL) a.counts[q] = c;
   if (MODE == kCount && a.work_slot) a.work_slot[slot] = vis;
   if (MODE == kStats) a.visited[q] = vis;
+}
+
+// Bucketed traversal for the count and fill passes (dim <= 3).
+//
+// The hits of a box are the points inside it, whichever nodes are visited to
+// find them: the push rule above only skips subtrees that cannot hold one
+// (a left subtree has x[d] <= split, a right one x[d] >= split).  Most of a
+// box's visits are on the bottom depths, where its hits are and where pruning
+// saves little.  So the walk stops at depth B = F - 3 (F = number of full
+// depths of the tree) and every subtree it reaches there, a bucket of the 7
+// nodes on depths B..B+2 and up to 8 on the partial depth B+3, is tested node
+// by node with independent loads (about 9 VALU instructions a node against
+// about 30 for a walk step).  The visited-count statistics (kStats) keep the
+// reference walk.
+//
+// Two schedules (BQ):
+//  * BQ = 0, parking: a lane that reaches a bucket waits until `bucket_park`
+//    eighths of the wave's live lanes wait or none can walk; then the waiting
+//    lanes test their buckets together.
+//  * BQ > 0, queueing: a lane puts the bucket roots it reaches into its own
+//    LDS queue of BQ entries and walks on; every iteration issues the load of
+//    its walk step and the loads of its oldest queued bucket together, so one
+//    memory latency covers both (a lane whose queue is full waits).
+template <int DIM, int MODE, bool BUF, int BQ>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE == kCount ? 8 : 4)))
+void bucket_search_kernel(SearchArgs a) {
+  static_assert(DIM >= 1 && DIM <= 3, "one 16-byte record per node");
+  static_assert(MODE == kCount || MODE == kFill, "count and fill passes only");
+  constexpr bool FILL = MODE == kFill;
+  constexpr int BS = 2 * DIM + 1;
+  __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
+  __shared__ float2 s_box[kThreads * BS];
+  __shared__ uint32_t s_rot[32];
+  __shared__ uint32_t s_bq[BQ > 0 ? BQ * kThreads : 1];  // queue entry k of thread t at [k * kThreads + t]
+  if (threadIdx.x < 32u) s_rot[threadIdx.x] = (threadIdx.x % (uint32_t)DIM) * (uint32_t)sizeof(float2);
+  __syncthreads();
+  uint32_t slot = search_block(blockIdx.x, gridDim.x, a.block_order) * kThreads + threadIdx.x;
+  if (slot >= a.nq) return;
+  if (a.slot_order) slot = a.slot_order[slot];
+  const uint32_t q = a.perm ? a.perm[slot] : slot;
+  uint32_t base = 0, n = a.n_single;
+  if (a.off) {
+    const uint32_t e = a.qev ? a.qev[q] : 0u;
+    if (e < a.n_events) {
+      base = a.off[e];
+      n = a.off[e + 1] - base;
+    } else {
+      n = 0;
+    }
+  }
+  float2* const my_box = &s_box[threadIdx.x * BS];
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+    float lo = a.mins[(uint64_t)m * a.nq + slot];
+    float hi = a.maxs[(uint64_t)m * a.nq + slot];
+    if (lo != lo) {  // a NaN lower bound: as in search_kernel
+      lo = -__builtin_huge_valf();
+      hi = __builtin_nanf("");
+    }
+    my_box[m] = my_box[m + DIM] = make_float2(lo, hi);
+  }
+  // F full depths (n >= 2^F - 1); buckets are rooted on depth B.  A tree of
+  // fewer than 7 nodes (F < 3) has no full bucket and is walked (B = 32 is
+  // never reached).
+  const uint32_t F = 31u - (uint32_t)__builtin_clz(n + 1u);
+  const uint32_t B = F > 3u ? F - 3u : (F == 3u ? 0u : 32u);
+  // The bounds compared with slot j of a record on depth d are the pair at
+  // byte s_rot[d] + 8 j of my_box (dimension (d + j) % DIM).  The fill pass
+  // keeps those of depths B .. B + DIM - 1 in registers (bk); the count pass
+  // reads them per use, so they take no registers across its loop.
+  constexpr bool kRegBounds = FILL;
+  float2 bk[kRegBounds ? DIM : 1][DIM];
+  if (kRegBounds) {
+    const uint32_t r0 = B % (uint32_t)DIM;
+#pragma unroll
+    for (int k = 0; k < (kRegBounds ? DIM : 1); ++k) {
+      uint32_t r = r0 + (uint32_t)k;
+      r = r >= (uint32_t)DIM ? r - (uint32_t)DIM : r;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) bk[k][j] = my_box[r + (uint32_t)j];
+    }
+  }
+  // bounds of depth B + k
+  auto bounds = [&](uint32_t k, float2 (&bb)[DIM]) {
+    if (kRegBounds) {
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) bb[j] = bk[kRegBounds ? k % (uint32_t)DIM : 0][j];
+    } else {
+      const float2* bx =
+          reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[(B + k) & 31u]);
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) bb[j] = bx[j];
+    }
+  };
+  NodeSource src;
+  src.flat = a.recs;
+  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.recs, (short)0, (int)a.node_bytes, 0x00020000);
+  uint32_t c = 0;
+  uint32_t* rw = &ring[FILL ? threadIdx.x * kRingRow : 0];
+  const uint64_t w0 = FILL ? a.qoff[q] : 0;
+  const uint64_t wb = w0 & ~(uint64_t)3;
+  const uint32_t i0 = (uint32_t)w0 & 3u;
+  uint32_t wr = i0;  // next ring index
+  uint32_t fq = 0;   // first ring index not written out (multiple of 4)
+  auto write_groups = [&]() {
+    while (fq + 4u <= wr) {
+      const uint4 g = *reinterpret_cast<const uint4*>(rw + (fq & (kRing - 1)));
+      const uint64_t p = wb + fq;
+      if (fq >= i0 && a.out_aligned && p + 4u <= a.cap) {
+        *reinterpret_cast<uint4*>(a.out + p) = g;
+      } else {
+        const uint32_t e[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j)
+          if (fq + j >= i0 && p + j < a.cap) a.out[p + j] = e[j];
+      }
+      fq += 4u;
+    }
+  };
+  // a hit candidate: kept (fill) or counted when `in`
+  auto report = [&](uint32_t id, bool in) {
+    if (FILL) {
+      rw[wr & (kRing - 1)] = id;  // the slot is free; it is kept only for a hit
+      wr += in ? 1u : 0u;
+    } else {
+      c += in ? 1u : 0u;
+    }
+  };
+  // one record: the closed-box test against the bounds of its depth
+  auto inside = [&](const float4 r, const float2 (&bb)[DIM]) -> bool {
+    bool in = (r.x >= bb[0].x) & (r.x <= bb[0].y);
+    if (DIM > 1) in = in & (r.y >= bb[DIM > 1 ? 1 : 0].x) & (r.y <= bb[DIM > 1 ? 1 : 0].y);
+    if (DIM > 2) in = in & (r.z >= bb[DIM > 2 ? 2 : 0].x) & (r.z <= bb[DIM > 2 ? 2 : 0].y);
+    return in;
+  };
+  // a node record; the count pass needs only the coordinates (12 bytes)
+  auto rec16 = [&](uint32_t rec) -> float4 {
+    if (FILL) {
+      if (BUF) return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, rec * 16u, 0, 0));
+      return reinterpret_cast<const float4*>(a.recs)[rec];
+    }
+    if (BUF) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b96(src.rsrc, rec * 16u, 0, 0);
+      return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), 0.f);
+    }
+    const float* f = a.recs + (uint64_t)rec * 4u;
+    return make_float4(f[0], f[1], f[2], 0.f);
+  };
+  // depths B, B+1, B+2 of the bucket rooted at r: nodes r; 2r+1, 2r+2;
+  // 4r+3 .. 4r+6 (all exist)
+  auto load7 = [&](uint32_t r, float4 (&v)[7]) {
+    const uint32_t r1 = base + 2u * r + 1u, r2 = base + 4u * r + 3u;
+    v[0] = rec16(base + r);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) v[1 + i] = rec16(r1 + (uint32_t)i);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[3 + i] = rec16(r2 + (uint32_t)i);
+  };
+  auto test7 = [&](const float4 (&v)[7]) {
+    float2 bb[DIM];
+    bounds(0u, bb);
+    report(__float_as_uint(v[0].w), inside(v[0], bb));
+    bounds(1u, bb);
+    report(__float_as_uint(v[1].w), inside(v[1], bb));
+    report(__float_as_uint(v[2].w), inside(v[2], bb));
+    bounds(2u, bb);
+#pragma unroll
+    for (int i = 3; i < 7; ++i) report(__float_as_uint(v[i].w), inside(v[i], bb));
+  };
+  // depth B + 3 = F under r: nodes 8r+7 .. 8r+14 that exist (a prefix of the depth)
+  auto last_depth = [&](uint32_t r) {
+    const uint32_t l3 = 8u * r + 7u;
+    if (l3 < n) {
+      if (FILL && __ballot(fq + (uint32_t)kRing - wr < 8u) != 0ull) write_groups();
+      const uint32_t last = n - 1u;
+      float4 u[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t k = l3 + (uint32_t)i;
+        u[i] = rec16(base + (k < last ? k : last));
+      }
+      float2 bb[DIM];
+      bounds(3u, bb);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) report(__float_as_uint(u[i].w), (l3 + (uint32_t)i < n) & inside(u[i], bb));
+    }
+  };
+  const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
+  uint32_t node = 0, depth = 0, pending = 1, work = 0;
+  bool alive = n != 0;
+  // one step of the reference walk from `node` (not a bucket root)
+  auto walk_step = [&]() {
+    ++work;
+    float x[DIM];
+    uint32_t id;
+    load_rec<DIM, BUF>(src, base + node, x, id);
+    const float2* bx = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[depth]);
+    float lo[DIM], hi[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+      const float2 t = bx[j];
+      lo[j] = t.x;
+      hi[j] = t.y;
+    }
+    const bool ge = x[0] >= lo[0], le = x[0] <= hi[0], lower = x[0] < lo[0];
+    bool in = ge & le;
+#pragma unroll
+    for (int j = 1; j < DIM; ++j) in = in & (x[j] <= hi[j]) & (x[j] >= lo[j]);
+    report(id, in);
+    const bool go_l = ge && node < tl;
+    const bool go_r = (lower || le) && node < tr;
+    const bool desc = go_l || go_r;
+    pending |= (go_l && go_r) ? (2u << depth) : 0u;
+    const uint32_t b = 31u - (uint32_t)__builtin_clz(pending);
+    const uint32_t up = (node + 1u) >> (depth - b);
+    const uint32_t down = 2u * node + (go_l ? 1u : 2u);
+    alive = desc || pending != 1u;
+    node = desc ? down : up;
+    pending = desc ? pending : __builtin_amdgcn_ubfe(pending, 0u, b);
+    depth = desc ? depth + 1u : b;
+  };
+  // leave the bucket root `node`: on to the deepest pending right son
+  auto pop = [&]() {
+    const uint32_t b = 31u - (uint32_t)__builtin_clz(pending);
+    alive = pending != 1u;
+    node = (node + 1u) >> (depth - b);
+    pending = __builtin_amdgcn_ubfe(pending, 0u, b);
+    depth = b;
+  };
+  if (BQ == 0) {
+    bool parked = alive && B == 0u;
+    const uint32_t park8 = a.bucket_park;
+    for (;;) {
+      if (__ballot(alive) == 0ull) break;
+      // room for one walk step and a bucket's first 7 nodes
+      if (FILL && __ballot(fq + (uint32_t)kRing - wr < 8u) != 0ull) write_groups();
+      if (alive && !parked) {
+        walk_step();
+        parked = alive && depth == B;
+      }
+      const uint64_t pm = __ballot(parked);
+      if (pm == 0ull) continue;
+      const uint64_t wm = __ballot(alive && !parked);
+      if (wm != 0ull && (uint32_t)__popcll(pm) * 8u < park8 * (uint32_t)__popcll(pm | wm)) continue;
+      if (parked) {
+        work += 2u;
+        float4 v[7];
+        load7(node, v);
+        test7(v);
+        last_depth(node);
+        pop();
+        parked = alive && depth == B;
+      }
+    }
+  } else {
+    uint32_t* const my_q = &s_bq[BQ > 0 ? threadIdx.x : 0];
+    uint32_t qh = 0, qt = 0;  // queue head and tail (entries qh .. qt-1 are pending)
+    for (;;) {
+      const bool has_b = qh != qt;
+      if (__ballot(alive || has_b) == 0ull) break;
+      // room for one walk step and a bucket's first 7 nodes
+      if (FILL && __ballot(fq + (uint32_t)kRing - wr < 8u) != 0ull) write_groups();
+      // queue the bucket roots at the head of the walk while there is room
+      while (alive && depth == B && qt - qh < (uint32_t)BQ) {
+        my_q[(qt % (uint32_t)BQ) * kThreads] = node;
+        ++qt;
+        pop();
+      }
+      uint32_t r = 0;
+      float4 v[7];
+      if (has_b) {
+        r = my_q[(qh % (uint32_t)BQ) * kThreads];
+        load7(r, v);
+      }
+      if (alive && depth != B) walk_step();
+      if (has_b) {
+        work += 2u;
+        ++qh;
+        test7(v);
+        last_depth(r);
+      }
+    }
+  }
+  if (FILL) {
+    write_groups();
+    for (uint32_t j = fq > i0 ? fq : i0; j < wr; ++j)
+      if (wb + j < a.cap) a.out[wb + j] = rw[j & (kRing - 1)];
+  }
+  if (!FILL) a.counts[q] = c;
+  if (MODE == kCount && a.work_slot) a.work_slot[slot] = work;
 }
 
 // Low-latency search of a few boxes: one wave per box, breadth first.

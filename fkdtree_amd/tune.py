@@ -8,7 +8,9 @@ On a workload it times, with HIP events:
   box order; 0 keeps the caller's order), `fill_order` (heaviest search
   blocks first in the fill pass), `fill_window` (boxes reordered by work
   inside windows of that many boxes for the fill pass), `count_window` and
-  `key_volume_bits`;
+  `key_volume_bits`, then, on top of the fastest of those, the traversal
+  schedules `bucket` / `count_bucket` / `bucket_park` (bottom subtrees
+  tested node by node);
 * the build, sweeping `fin_log` (subtrees of fewer than 2^fin_log points
   are finished by one workgroup in LDS: 511, 1,023 or 2,047 points; fewer
   global level launches against bigger finisher workgroups) and
@@ -38,6 +40,11 @@ TABLE = os.path.join(HERE, "tuning.json")
 CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill_window": (0, 256, 1024, 4096),
               "count_window": (0, 1024, 4096), "key_volume_bits": (0, 3, 4)}
 BUILD_CANDIDATES = {"fin_log": (9, 10, 11), "build_radix_items": (8, 16), "level_pairs": (1, 0)}
+# Traversal schedules, swept on top of the fastest CANDIDATES setting: the
+# walk, or bucketed fill passes (queued, or parked with a threshold).
+BUCKET_CANDIDATES = ({"bucket": 0, "count_bucket": 0}, {"bucket": 2, "count_bucket": 0},
+                     {"bucket": 1, "bucket_park": 6, "count_bucket": 0}, {"bucket": 1, "bucket_park": 4, "count_bucket": 0},
+                     {"bucket": 2, "count_bucket": 2})
 
 
 _ARCH: Optional[str] = None
@@ -111,8 +118,10 @@ def apply(tree, arch: Optional[str] = None) -> Optional[Dict[str, int]]:
     return applied or None
 
 
-def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=None, d_qev: Optional[int] = None):
-    """Time the two-pass search for every candidate; [(params, ms)] sorted."""
+def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=None, d_qev: Optional[int] = None,
+          settings=None):
+    """Time the two-pass search for every candidate (the product of
+    `candidates`, or the list `settings`); [(params, ms)] sorted."""
     from . import DeviceArray, Stream, Timer
 
     cands = candidates or CANDIDATES
@@ -124,8 +133,9 @@ def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=Non
     tm = Timer()
     out = []
     keys = list(cands)
-    for values in itertools.product(*(cands[k] for k in keys)):
-        params = dict(zip(keys, values))
+    if settings is None:
+        settings = [dict(zip(keys, values)) for values in itertools.product(*(cands[k] for k in keys))]
+    for params in settings:
         tree.set_tuning(**params)
 
         def run():
@@ -196,7 +206,13 @@ def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path
         tree = FKDTree.from_device(pts.dim, pts.n, d_c.ptr, d_i.ptr, s)
         key = _key(pts.dim, pts.n)
     tree.build()
-    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=d_q.ptr if d_q is not None else None)
+    dq = d_q.ptr if d_q is not None else None
+    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq)
+    # the traversal schedules on top of the fastest box order and windows
+    best = res[0][0]
+    sres = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq,
+                 settings=[{**best, **b} for b in BUCKET_CANDIDATES])
+    res = sorted(sres + res[1:], key=lambda x: x[1])
     bres = sweep_build(tree, reps=max(reps, 5))
     tree.close()
     arch = _arch()

@@ -148,7 +148,17 @@ int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t str
  *   "level_pairs"     build, dim 3: 1 (default) moves one point array
  *                     through two depths per launch (half the bytes of
  *                     moving two arrays through one), 0 two arrays through
- *                     one depth.
+ *                     one depth;
+ *   "bucket"          fill pass, dim <= 3: 0 (default) walks every node;
+ *                     1 and 2 stop the walk three depths above the last
+ *                     full depth and test each subtree reached there (7
+ *                     nodes, and up to 8 on the partial depth) node by
+ *                     node: 1 lets a lane that reaches one wait until
+ *                     "bucket_park" eighths of its wave's live lanes wait
+ *                     (0..8, default 8), 2 queues up to 8 of them per lane
+ *                     and tests one beside every walk step; it pays for
+ *                     boxes with many hits;
+ *   "count_bucket"    the same choice for the count pass (default 0).
  * fin_log and build_radix_items cannot change while a graph of the handle
  * exists (FKD_ERR_STATE); the next build lays the tilings out for new values.
  * They replace the reference's fixed launch geometry

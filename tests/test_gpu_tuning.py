@@ -69,7 +69,8 @@ def test_tuning_rejects_bad_values(require_gpu):
     p = adversarial_points("mixed", 100)
     t = FKDTree(coords=p.coords, ids=p.ids)
     for k, v in (("pool_batch", 32), ("query_key_bits", 11), ("fill_order", 2), ("fill_window", 512),
-                 ("count_window", 100), ("key_volume_bits", 7), ("no_such_key", 1)):
+                 ("count_window", 100), ("key_volume_bits", 7), ("bucket", 3), ("count_bucket", 3),
+                 ("bucket_park", 9), ("no_such_key", 1)):
         with pytest.raises(RuntimeError):
             t.set_tuning(**{k: v})
     t.close()

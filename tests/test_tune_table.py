@@ -48,6 +48,10 @@ def test_candidates_are_valid():
     assert set(tune.BUILD_CANDIDATES["fin_log"]) <= {9, 10, 11}
     assert set(tune.BUILD_CANDIDATES["build_radix_items"]) <= {8, 16}
     assert set(tune.BUILD_CANDIDATES["level_pairs"]) <= {0, 1}
+    # traversal schedules: walk, parked (threshold in eighths) or queued buckets
+    for b in tune.BUCKET_CANDIDATES:
+        assert b["bucket"] in (0, 1, 2) and b.get("count_bucket", 0) in (0, 1, 2)
+        assert 0 <= b.get("bucket_park", 8) <= 8
 
 
 class _FakeTree:
