@@ -177,7 +177,8 @@ def main():
     D = int(np.floor(np.log2(max(n, 1))))
     build_bytes = 36 * n * D
     achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
-    traffic = measured_traffic(args.workload, "fkd::search_kernel<3, 1, true>")
+    fill_kernel = fill_kernel_name(tree, dim)
+    traffic = measured_traffic(args.workload, fill_kernel)
     # HBM bytes the counters saw per fill launch (calibrated FETCH_SIZE +
     # WRITE_SIZE of the committed profile) over the launch time measured here
     counter_gbs = traffic / (fill_ms * 1e-3) / 1e9 if traffic else None
@@ -208,7 +209,7 @@ def main():
         "visited_nodes": V,
         "search_algorithmic_GBs": search_bytes / (search_ms * 1e-3) / 1e9,
         "build_algorithmic_GBs": build_bytes / (build_ms * 1e-3) / 1e9,
-        "roofline": {"bound": "hbm", "kernel": "search_kernel<3,kFill> (fill traversal)", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": f"{fill_kernel} (fill traversal)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic, "algorithmic_bytes": fill_bytes,
                      # the same launch by its counted HBM bytes: node re-reads hit L2 / MALL
@@ -304,6 +305,17 @@ def bench_serving(n_events: int, rates) -> list:
         progress(f"serving at {rate:.0f} events/s: {d.get('events_per_s', 0):.0f} events/s")
         out.append(d)
     return out
+
+
+def fill_kernel_name(tree, dim: int) -> str:
+    """Name of the fill-pass kernel the tree's traversal schedule launches
+    (node records in a buffer resource: fewer than 2^28 nodes at dim <= 3)."""
+    sched = tree.get_tuning("bucket") if dim <= 3 else 0
+    if sched == 1:
+        return f"fkd::bucket_search_kernel<{dim}, 1, true, 0>"
+    if sched == 2:
+        return f"fkd::bucket_search_kernel<{dim}, 1, true, 8>"
+    return f"fkd::search_kernel<{dim}, 1, true, false>"
 
 
 def measured_traffic(workload: str, kernel: str):

@@ -367,6 +367,14 @@ void bucket_search_kernel(SearchArgs a) {
       fq += 4u;
     }
   };
+  // Keeps room for 8 more ids in every lane's ring: when some lane has more
+  // than 8 pending, the lanes holding at least 8 write their complete groups
+  // (32 bytes or more each; lanes with fewer keep theirs for a longer run).
+  auto make_room = [&]() {
+    if (FILL && __ballot(wr - fq > 8u) != 0ull) {
+      if (wr - fq >= 8u) write_groups();
+    }
+  };
   // a hit candidate: kept (fill) or counted when `in`
   auto report = [&](uint32_t id, bool in) {
     if (FILL) {
@@ -421,7 +429,7 @@ void bucket_search_kernel(SearchArgs a) {
   auto last_depth = [&](uint32_t r) {
     const uint32_t l3 = 8u * r + 7u;
     if (l3 < n) {
-      if (FILL && __ballot(fq + (uint32_t)kRing - wr < 8u) != 0ull) write_groups();
+      make_room();
       const uint32_t last = n - 1u;
       float4 u[8];
 #pragma unroll
@@ -483,7 +491,7 @@ void bucket_search_kernel(SearchArgs a) {
     for (;;) {
       if (__ballot(alive) == 0ull) break;
       // room for one walk step and a bucket's first 7 nodes
-      if (FILL && __ballot(fq + (uint32_t)kRing - wr < 8u) != 0ull) write_groups();
+      make_room();
       if (alive && !parked) {
         walk_step();
         parked = alive && depth == B;
@@ -509,7 +517,7 @@ void bucket_search_kernel(SearchArgs a) {
       const bool has_b = qh != qt;
       if (__ballot(alive || has_b) == 0ull) break;
       // room for one walk step and a bucket's first 7 nodes
-      if (FILL && __ballot(fq + (uint32_t)kRing - wr < 8u) != 0ull) write_groups();
+      make_room();
       // queue the bucket roots at the head of the walk while there is room
       while (alive && depth == B && qt - qh < (uint32_t)BQ) {
         my_q[(qt % (uint32_t)BQ) * kThreads] = node;

@@ -42,9 +42,9 @@ CANDIDATES = {"query_key_bits": (0, 5, 6, 7, 8, 10), "fill_order": (0, 1), "fill
 BUILD_CANDIDATES = {"fin_log": (9, 10, 11), "build_radix_items": (8, 16), "level_pairs": (1, 0)}
 # Traversal schedules, swept on top of the fastest CANDIDATES setting: the
 # walk, or bucketed fill passes (queued, or parked with a threshold).
-BUCKET_CANDIDATES = ({"bucket": 0, "count_bucket": 0}, {"bucket": 2, "count_bucket": 0},
+BUCKET_CANDIDATES = ({"bucket": 0, "count_bucket": 0, "bucket_park": 8}, {"bucket": 2, "count_bucket": 0, "bucket_park": 8},
                      {"bucket": 1, "bucket_park": 6, "count_bucket": 0}, {"bucket": 1, "bucket_park": 4, "count_bucket": 0},
-                     {"bucket": 2, "count_bucket": 2})
+                     {"bucket": 2, "count_bucket": 2, "bucket_park": 8})
 
 
 _ARCH: Optional[str] = None
@@ -175,6 +175,8 @@ def sweep_build(tree, reps: int = 5, candidates=None):
 
 
 KEEP_SWEEP = 16  # sweep rows kept per table entry
+FINAL_TOP = 8    # leading settings timed again ...
+FINAL_REPS = 8   # ... with this many times the repetitions
 
 
 def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path: str = TABLE) -> Dict:
@@ -207,12 +209,18 @@ def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path
         key = _key(pts.dim, pts.n)
     tree.build()
     dq = d_q.ptr if d_q is not None else None
-    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq)
+    # every setting names every key it depends on (set_tuning leaves the others as they were)
+    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq,
+                candidates={**CANDIDATES, **{k: (v,) for k, v in BUCKET_CANDIDATES[0].items()}})
     # the traversal schedules on top of the fastest box order and windows
     best = res[0][0]
     sres = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq,
                  settings=[{**best, **b} for b in BUCKET_CANDIDATES])
     res = sorted(sres + res[1:], key=lambda x: x[1])
+    # single sweeps differ by noise near the top: the leaders are timed again, longer
+    top = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=FINAL_REPS * reps, d_qev=dq,
+                settings=[p for p, _ in res[:FINAL_TOP]])
+    res = top + res[FINAL_TOP:]
     bres = sweep_build(tree, reps=max(reps, 5))
     tree.close()
     arch = _arch()
