@@ -152,8 +152,9 @@ def sweep(tree, nq: int, d_mins: int, d_maxs: int, reps: int = 3, candidates=Non
     return out
 
 
-def sweep_build(tree, reps: int = 5, candidates=None):
-    """Time the build for every candidate geometry; [(params, ms)] sorted."""
+def sweep_build(tree, reps: int = 5, candidates=None, settings=None):
+    """Time the build for every candidate geometry (the product of
+    `candidates`, or the list `settings`); [(params, ms)] sorted."""
     from . import Stream, Timer
 
     cands = candidates or BUILD_CANDIDATES
@@ -161,8 +162,9 @@ def sweep_build(tree, reps: int = 5, candidates=None):
     tm = Timer()
     out = []
     keys = list(cands)
-    for values in itertools.product(*(cands[k] for k in keys)):
-        params = dict(zip(keys, values))
+    if settings is None:
+        settings = [dict(zip(keys, values)) for values in itertools.product(*(cands[k] for k in keys))]
+    for params in settings:
         tree.set_tuning(**params)
         tree.build()             # lays the tilings out for the new geometry
         tm.start(s)
@@ -222,6 +224,8 @@ def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path
                 settings=[p for p, _ in res[:FINAL_TOP]])
     res = top + res[FINAL_TOP:]
     bres = sweep_build(tree, reps=max(reps, 5))
+    btop = sweep_build(tree, reps=FINAL_REPS * max(reps, 5), settings=[p for p, _ in bres[:FINAL_TOP]])
+    bres = btop + bres[FINAL_TOP:]
     tree.close()
     arch = _arch()
     # the table keeps the fastest settings of the sweeps; the whole sweeps are returned
