@@ -531,42 +531,57 @@ template <int ITEMS>
 int radix_pass(const fkd::TileMap& tm, uint32_t P, uint32_t M, uint32_t* counts, uint32_t* offsets,
                uint64_t* desc, uint32_t* ctr, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
                uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift, bool write_keys,
-               bool write_ranks, hipStream_t s, uint32_t* bbox = nullptr) {
+               bool write_ranks, hipStream_t s, uint32_t* bbox = nullptr, const fkd::XSource* xs = nullptr) {
   const uint64_t nc = (uint64_t)M * 256u * tm.T;
   dim3 grid(tm.T, M);
-  fkd::radix_count_kernel<ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts);
+  if (xs) {
+    // first pass of a build in tie order: keys from the coordinates (M = dim)
+    if (M == 1) fkd::radix_count_kernel<ITEMS, 1><<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts, *xs);
+    else if (M == 2) fkd::radix_count_kernel<ITEMS, 2><<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts, *xs);
+    else if (M == 3) fkd::radix_count_kernel<ITEMS, 3><<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts, *xs);
+    else fkd::radix_count_kernel<ITEMS, 4><<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts, *xs);
+  } else {
+    fkd::radix_count_kernel<ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, stride, shift, tm, counts, fkd::XSource{});
+  }
   FKD_TRY(launch_check("radix_count"));
   const uint64_t st = fkd::scan_tiles(nc);
   fkd::scan_exclusive_kernel<uint32_t><<<(uint32_t)st, fkd::kThreads, 0, s>>>(counts, offsets, nc, desc, ctr);
   FKD_TRY(launch_check("radix_scan"));
-  if (write_keys && !write_ranks)
+  if (xs && write_keys && !write_ranks)
+    fkd::radix_scatter_kernel<true, false, ITEMS, true><<<grid, fkd::kThreads, 0, s>>>(
+        kin, kout, vin, vout, stride, ranks, P, shift, tm, offsets, bbox, xs->X);
+  else if (xs)
+    return fail(FKD_ERR_STATE, "internal: a pass from coordinates writes keys only");
+  else if (write_keys && !write_ranks)
     fkd::radix_scatter_kernel<true, false, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
-                                                                                 P, shift, tm, offsets, bbox);
+                                                                                 P, shift, tm, offsets, bbox, nullptr);
   else if (!write_keys && write_ranks)
     fkd::radix_scatter_kernel<false, true, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
-                                                                                 P, shift, tm, offsets, bbox);
+                                                                                 P, shift, tm, offsets, bbox, nullptr);
   else if (write_keys && write_ranks)
     fkd::radix_scatter_kernel<true, true, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride, ranks,
-                                                                                P, shift, tm, offsets, bbox);
+                                                                                P, shift, tm, offsets, bbox, nullptr);
   else
     fkd::radix_scatter_kernel<false, false, ITEMS><<<grid, fkd::kThreads, 0, s>>>(kin, kout, vin, vout, stride,
-                                                                                  ranks, P, shift, tm, offsets, bbox);
+                                                                                  ranks, P, shift, tm, offsets, bbox,
+                                                                                  nullptr);
   return launch_check("radix_scatter");
 }
 
 template <int DIM>
 int run_radix_pass(fkd_tree_s* t, ZeroArena& za, uint32_t M, const uint32_t* kin, uint32_t* kout,
                    const uint32_t* vin, uint32_t* vout, uint64_t stride, uint32_t* ranks, uint32_t shift,
-                   bool write_keys, bool write_ranks, hipStream_t s, uint32_t* bbox = nullptr) {
+                   bool write_keys, bool write_ranks, hipStream_t s, uint32_t* bbox = nullptr,
+                   const fkd::XSource* xs = nullptr) {
   const fkd::TileMap tm = t->radix_tiles.map(t->off.as<uint32_t>());
   uint64_t* desc = za.take(radix_pass_words(M, tm.T));
   if (!desc) return arena_exhausted();
   uint32_t* ctr = reinterpret_cast<uint32_t*>(desc + fkd::scan_tiles((uint64_t)M * 256u * tm.T));
   if (t->lay_radix_items == 8)
     return radix_pass<8>(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, kin, kout, vin,
-                         vout, stride, ranks, shift, write_keys, write_ranks, s, bbox);
+                         vout, stride, ranks, shift, write_keys, write_ranks, s, bbox, xs);
   return radix_pass<16>(tm, t->P, M, t->counts.as<uint32_t>(), t->offsets.as<uint32_t>(), desc, ctr, kin, kout, vin,
-                        vout, stride, ranks, shift, write_keys, write_ranks, s, bbox);
+                        vout, stride, ranks, shift, write_keys, write_ranks, s, bbox, xs);
 }
 
 // Level tiles of the handle; one tree needs no tile table (its kernels take
@@ -633,9 +648,15 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   // point records (search nodes' payload) with the keys when point order is
   // the tie order; otherwise records_kernel below
   float4* fused_rec = (DIM <= 3 && !order) ? t->Xr.as<float4>() : nullptr;
-  fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>(),
-                                             t->bbox.as<uint32_t>(), t->ids.as<uint32_t>(), fused_rec);
-  FKD_TRY(launch_check("keys"));
+  // In point order the first digit pass reads the coordinates itself (keys,
+  // positions, point records and the bounding box's starting values).
+  const fkd::XSource xsrc{X, P, t->ids.as<uint32_t>(), fused_rec, t->bbox.as<uint32_t>()};
+  const fkd::XSource* first_pass_x = order ? nullptr : &xsrc;
+  if (order) {
+    fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>(),
+                                               t->bbox.as<uint32_t>(), t->ids.as<uint32_t>(), nullptr);
+    FKD_TRY(launch_check("keys"));
+  }
   // 3. Stable LSD sort of every coordinate inside every event; the last pass
   //    stores each point's rank R[p][m] (the values end in buffer 0) and
   //    narrows the key-space bounding box (used to order search queries)
@@ -646,7 +667,7 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
       const bool lastp = pass == 3;
       FKD_TRY(run_radix_pass<DIM>(t, za, DIM, Kbuf(0, c), Kbuf(0, 1 - c), Abuf(0, c), Abuf(0, 1 - c), 2 * S,
                                   lastp ? R : nullptr, 8u * pass, !lastp, lastp, s,
-                                  lastp ? t->bbox.as<uint32_t>() : nullptr));
+                                  lastp ? t->bbox.as<uint32_t>() : nullptr, pass == 0 ? first_pass_x : nullptr));
       c = 1 - c;
     }
   }

@@ -48,16 +48,31 @@ __device__ inline TileInfo tile_info(const TileMap& tm, uint32_t t) {
   return ti;
 }
 
+// The first digit pass of a build whose points are in tie order reads the
+// coordinates themselves (XDIM = dimension): the key of position i in array m
+// is fkey(X[m][i]) and its value is i.  Its count kernel also writes the point
+// records (dim <= 3: coordinates and id in 16 bytes, array 0's blocks) and the
+// starting values of the key-space bounding box that the last pass narrows.
+struct XSource {
+  const float* X;        // [DIM][P] coordinates
+  uint32_t P;
+  const uint32_t* ids;   // point ids (records)
+  float4* rec;           // point records, or null
+  uint32_t* bbox;        // [2 DIM]: min keys, then max keys
+};
+
 // counts[m][256*tb[e] + digit*nt + lt] = number of keys of tile t in array m
 // whose digit (key >> shift) & 255 equals digit.
-template <int ITEMS>
+template <int ITEMS, int XDIM = 0>
 __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* __restrict__ keys,
                                                                uint64_t kstride, uint32_t shift,
-                                                               TileMap tm, uint32_t* counts) {
+                                                               TileMap tm, uint32_t* counts, XSource xs) {
   constexpr int kItems = ITEMS;
   constexpr int kTile = kThreads * ITEMS;
   __shared__ uint32_t hist[4][256];
   const uint32_t t = blockIdx.x, m = blockIdx.y;
+  if (XDIM && t == 0 && m == 0 && threadIdx.x < 2u * XDIM)
+    xs.bbox[threadIdx.x] = threadIdx.x < (uint32_t)XDIM ? 0xffffffffu : 0u;
   const int wave = threadIdx.x / kWave;
 #pragma unroll
   for (int w = 0; w < 4; ++w) hist[w][threadIdx.x] = 0;
@@ -66,10 +81,30 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
   const uint32_t* K = keys + m * kstride;
   // all loads of the thread first, so they are in flight together
   uint32_t key[kItems];
+  if (XDIM) {
+    const float* Xm = xs.X + (uint64_t)m * xs.P;
+    float x[kItems];
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    const uint32_t i = ti.base + threadIdx.x + (uint32_t)r * kThreads;
-    key[r] = i < ti.end ? K[i] : 0u;
+    for (int r = 0; r < kItems; ++r) {
+      const uint32_t i = ti.base + threadIdx.x + (uint32_t)r * kThreads;
+      x[r] = i < ti.end ? Xm[i] : 0.f;
+      key[r] = fkey(x[r]);
+    }
+    if (XDIM <= 3 && m == 0 && xs.rec) {
+#pragma unroll
+      for (int r = 0; r < kItems; ++r) {
+        const uint32_t i = ti.base + threadIdx.x + (uint32_t)r * kThreads;
+        if (i < ti.end)
+          xs.rec[i] = make_float4(x[r], XDIM > 1 ? xs.X[(uint64_t)xs.P + i] : 0.f,
+                                  XDIM > 2 ? xs.X[2ull * xs.P + i] : 0.f, __uint_as_float(xs.ids[i]));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+      const uint32_t i = ti.base + threadIdx.x + (uint32_t)r * kThreads;
+      key[r] = i < ti.end ? K[i] : 0u;
+    }
   }
   // Lanes that share a digit would serialise on one LDS address (a wave of
   // coordinates that differ only in their upper bits has a handful of
@@ -108,12 +143,14 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
 // item, lane by lane), then written out in that order, so consecutive threads
 // write consecutive slots of one digit's run instead of up to 64 buckets per
 // store.
-template <bool WRITE_KEYS, bool WRITE_RANKS, int ITEMS>
+// FROM_X: the first pass of XSource: keys from the coordinates xs, values the
+// positions (keys_in and vals_in unused).
+template <bool WRITE_KEYS, bool WRITE_RANKS, int ITEMS, bool FROM_X = false>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ keys_in, uint32_t* __restrict__ keys_out,
     const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out, uint64_t stride,
     uint32_t* __restrict__ ranks, uint32_t P, uint32_t shift, TileMap tm,
-    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ bbox) {
+    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ bbox, const float* __restrict__ xs) {
   constexpr int kItems = ITEMS;
   constexpr int kTile = kThreads * ITEMS;
   constexpr int kPerWave = kTile / (kThreads / kWave);
@@ -148,8 +185,13 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   for (int r = 0; r < kItems; ++r) {
     const uint32_t pos = wbase + (uint32_t)r * kWave;
     const bool valid = pos < ti.end;
-    key[r] = valid ? Ki[pos] : 0u;
-    val[r] = valid ? Vi[pos] : 0u;
+    if (FROM_X) {
+      key[r] = valid ? fkey(xs[(uint64_t)m * P + pos]) : 0u;
+      val[r] = pos;
+    } else {
+      key[r] = valid ? Ki[pos] : 0u;
+      val[r] = valid ? Vi[pos] : 0u;
+    }
   }
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
