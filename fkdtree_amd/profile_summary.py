@@ -6,7 +6,8 @@
         --calib-fetch D --calib-write D --calib-bytes B --out F.json
     python -m fkdtree_amd.profile_summary gaps DIR --split KERNEL  # idle time per batch
 
-`traffic` scales FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of every kernel
+`traffic` scales FETCH_SIZE / WRITE_SIZE (KiB per dispatch, the median over
+a kernel's dispatches) of every kernel
 by the ratio true bytes / counted bytes of a plain copy kernel of known size
 (copy_u32_kernel, `bench.py --calibrate`), and writes a JSON table that
 bench.py reads for the `roofline.traffic` figure.
@@ -41,11 +42,15 @@ def kernel_stats(path: str):
     return [(short(n), int(c), a, t / 1e3, p) for n, c, t, a, p in rows]
 
 
-def counters(path: str):
-    """{(kernel, counter): average value per dispatch}."""
+def counters(path: str, stat: str = "mean"):
+    """{(kernel, counter): value per dispatch}: the mean over the kernel's
+    dispatches, or their median (stat="median": a run's sizing calls, which
+    write no output, do not pull it down)."""
     agg = collections.defaultdict(list)
     for k, c, v in _db(path).execute("select kernel_name, counter_name, value from counters_collection"):
         agg[(short(k), c)].append(v)
+    if stat == "median":
+        return {key: sorted(v)[len(v) // 2] for key, v in agg.items()}
     return {key: sum(v) / len(v) for key, v in agg.items()}
 
 
@@ -116,7 +121,7 @@ def main(argv=None):
     else:
         cf = a.calib_bytes / 1024.0 / counters(a.calib_fetch)[("fkd::copy_u32_kernel", "FETCH_SIZE")]
         cw = a.calib_bytes / 1024.0 / counters(a.calib_write)[("fkd::copy_u32_kernel", "WRITE_SIZE")]
-        fetch, write = counters(a.fetch), counters(a.write)
+        fetch, write = counters(a.fetch, "median"), counters(a.write, "median")
         table = {"fetch_scale": cf, "write_scale": cw, "kernels": {}}
         for (k, cn), v in fetch.items():
             if cn == "FETCH_SIZE":
