@@ -988,13 +988,30 @@ struct FinArgs {
   uint32_t cap;               // most points of one subtree (fin_subtree_max over the events)
   uint32_t fin_log;           // the subtrees start at depth fin_depth(D, fin_log)
   uint32_t pairs;             // the depths above were built by level_pair_kernel
+  // With out.S0, the tree arrays and search records of the subtree's nodes are
+  // written here (out's S0, X, ids, P, Tc, Tid, recs; prec the point records)
+  // instead of F, and workgroup t of an event also writes the event's top
+  // node t (above the subtrees' depth) from the F the level launches wrote.
+  LevelArgs<DIM> out;
+  const float4* prec;
 };
+
+// Tree arrays and search record of heap node v of the event at o, whose point
+// has dimension-0 rank `rank` (S0: rank -> point).
+template <int DIM>
+__device__ inline void out_node(const LevelArgs<DIM>& a, const float4* prec, uint32_t o, uint32_t v, uint32_t rank) {
+  float c[DIM];
+  uint32_t id;
+  node_point<DIM>(a, a.S0[rank], prec, c, id);
+  write_node<DIM>(a, o, v, floor_log2(v + 1u), c, id);
+}
 
 // The levels fin_depth(D) .. D-1 of one subtree in LDS: the same medians and
 // stable partitions as level_kernel (FKDTree_cpu.h:229-268), with the
 // subtree's points named by their index in array 0 (sorted by dimension 0)
 // and their ranks kept in a table, so only 16-bit names move.  Writes F
-// (heap index -> rank) for the subtree's nodes, leaves included.
+// (heap index -> rank) for the subtree's nodes, leaves included, or the nodes'
+// tree arrays and search records themselves (FinArgs::out).
 // ITEMS consecutive positions per thread: subtrees of up to NT * ITEMS
 // points.
 // NT threads per workgroup (one to four waves).
@@ -1057,7 +1074,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && 
     }
   }
   if (sh.D == 0) {  // a single point
-    if (threadIdx.x == 0 && n == 1) a.F[o] = pl[0][o];
+    if (threadIdx.x == 0 && n == 1) {
+      if (a.out.S0)
+        out_node<DIM>(a.out, a.prec, o, 0u, pl[0][o]);
+      else
+        a.F[o] = pl[0][o];
+    }
     return;
   }
   __syncthreads();
@@ -1274,13 +1296,32 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DIM <= 3 && 
   // it held then: array m moves on every depth of [L0, L) except those with
   // L % DIM == m, and starts in buffer 0.
   const auto same = [](uint32_t x, uint32_t m) { return (x + (uint32_t)DIM - 1u - m) / (uint32_t)DIM; };
+  uint32_t rk[kFinItems];
 #pragma unroll
   for (int i = 0; i < kFinItems; ++i) {
     const uint32_t j = first + i;
     if (j >= size0) continue;
     const uint32_t hd = floor_log2(fh[i] + 1u), L = hd < sh.D ? hd : sh.D - 1u, m = L % (uint32_t)DIM;
     const uint32_t moves = (L - L0) - (same(L, m) - same(L0, m));
-    a.F[o + fh[i]] = s_rank(0)[s_arr(m, moves & 1u)[j]];
+    rk[i] = s_rank(0)[s_arr(m, moves & 1u)[j]];
+    if (!a.out.S0) a.F[o + fh[i]] = rk[i];
+  }
+  if (a.out.S0) {
+    // point, then its record, for all items before any is used; the writes
+    // land in this subtree's heap ranges (one per depth) and merge in L2
+    uint32_t pt[kFinItems];
+#pragma unroll
+    for (int i = 0; i < kFinItems; ++i) pt[i] = first + i < size0 ? a.out.S0[rk[i]] : 0u;
+    float c[kFinItems][DIM];
+    uint32_t id[kFinItems];
+#pragma unroll
+    for (int i = 0; i < kFinItems; ++i)
+      if (first + i < size0) node_point<DIM>(a.out, pt[i], a.prec, c[i], id[i]);
+#pragma unroll
+    for (int i = 0; i < kFinItems; ++i)
+      if (first + i < size0) write_node<DIM>(a.out, o, fh[i], floor_log2(fh[i] + 1u), c[i], id[i]);
+    // one of the event's 2^L0 - 1 nodes above the subtrees per workgroup
+    if (threadIdx.x == 0 && t0 + 1u < (1u << L0)) out_node<DIM>(a.out, a.prec, o, t0, a.F[o + t0]);
   }
 }
 

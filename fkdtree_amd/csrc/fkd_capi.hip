@@ -519,7 +519,7 @@ struct ZeroArena {
 
 int arena_exhausted() { return fail(FKD_ERR_STATE, "internal: zeroed scratch arena too small for this launch"); }
 
-int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za);
+int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za, bool clear = true);
 
 // Words one radix pass takes from the arena: scan descriptors + counter.
 uint64_t radix_pass_words(uint32_t M, uint32_t T) { return fkd::scan_tiles((uint64_t)M * 256u * T) + fkd::kCtrWords; }
@@ -616,10 +616,12 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   const bool pairs = DIM == 3 && t->tune.level_pairs && levels > 0;
   const uint64_t level_words =
       pairs ? (uint64_t)(levels + 1) * (2ull * tm.T + fkd::kCtrWords) : (uint64_t)levels * ((uint64_t)M * tm.T + fkd::kCtrWords);
+  // In point order the first digit pass's count kernel clears the arena (no
+  // kernel before it uses zeroed words); otherwise a fill launch does.
   ZeroArena za;
   FKD_TRY(zero_arena(t,
                      (t->ids_in_order ? 0 : 4 * radix_pass_words(1, RT)) + 4 * radix_pass_words(DIM, RT) + level_words,
-                     s, za));
+                     s, za, !t->ids_in_order));
   const uint64_t S = P;  // stride of one array
   uint32_t* A = t->A.as<uint32_t>();  // A[m][2][P]
   uint32_t* K = t->K.as<uint32_t>();  // K[m][2][P]
@@ -650,12 +652,16 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
   float4* fused_rec = (DIM <= 3 && !order) ? t->Xr.as<float4>() : nullptr;
   // In point order the first digit pass reads the coordinates itself (keys,
   // positions, point records and the bounding box's starting values).
-  const fkd::XSource xsrc{X, P, t->ids.as<uint32_t>(), fused_rec, t->bbox.as<uint32_t>()};
+  const fkd::XSource xsrc{X, P, t->ids.as<uint32_t>(), fused_rec, t->bbox.as<uint32_t>(), za.base, za.cap};
   const fkd::XSource* first_pass_x = order ? nullptr : &xsrc;
   if (order) {
     fkd::keys_kernel<DIM><<<grid, 256, 0, s>>>(X, order, P, Kbuf(0, 0), Abuf(0, 0), 2 * S, t->err.as<uint32_t>(),
                                                t->bbox.as<uint32_t>(), t->ids.as<uint32_t>(), nullptr);
     FKD_TRY(launch_check("keys"));
+    if (DIM <= 3) {  // the point records the finisher gathers
+      fkd::records_kernel<DIM><<<grid, 256, 0, s>>>(X, t->ids.as<uint32_t>(), P, t->Xr.as<float4>());
+      FKD_TRY(launch_check("records"));
+    }
   }
   // 3. Stable LSD sort of every coordinate inside every event; the last pass
   //    stores each point's rank R[p][m] (the values end in buffer 0) and
@@ -766,6 +772,15 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
     f.cap = t->fin_cap;
     f.fin_log = t->lay_fin_log;
     f.pairs = pairs ? 1u : 0u;
+    // the finisher writes the tree arrays and search records (step 7)
+    f.out.S0 = Abuf(0, 0);
+    f.out.X = X;
+    f.out.ids = t->ids.as<uint32_t>();
+    f.out.P = P;
+    f.out.Tc = t->Tc.as<float>();
+    f.out.Tid = t->Tid.as<uint32_t>();
+    f.out.recs = t->recs.as<float>();
+    f.prec = t->Xr.as<float4>();
     const uint32_t lds = fkd::fin_lds_bytes(DIM, f.cap);
     const uint32_t items = fkd::fin_items(f.cap);
     if (items == 1)
@@ -778,12 +793,8 @@ int build_impl(fkd_tree_s* t, hipStream_t s) {
       fkd::finish_kernel<DIM, 8, fkd::kThreads><<<t->fin_tiles.T, fkd::kThreads, lds, s>>>(f);
     FKD_TRY(launch_check("finish"));
   }
-  // 7. Tree arrays and search records from F.
-  {
-    if (DIM <= 3 && !fused_rec) {
-      fkd::records_kernel<DIM><<<grid, 256, 0, s>>>(X, t->ids.as<uint32_t>(), P, t->Xr.as<float4>());
-      FKD_TRY(launch_check("records"));
-    }
+  // 7. Tree arrays and search records from F, where no finisher wrote them.
+  if (!t->fin_tiles.T) {
     fkd::LevelArgs<DIM> a{};
     a.tm = tm;
     a.S0 = Abuf(0, 0);
@@ -1099,12 +1110,14 @@ int validate_offsets(uint32_t n_events, const uint64_t* event_offsets, uint64_t*
 
 // Allocates everything, given the input already in t->X / t->ids; validates.
 // Scratch of the build (released by fkd_trim, re-created on the next build).
-int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za) {
+// clear = false: the caller's first kernel clears the arena (it must run
+// before any kernel that relies on zeroed words).
+int zero_arena(fkd_tree_s* t, uint64_t words, hipStream_t s, ZeroArena& za, bool clear) {
   FKD_TRY(t->zarena.ensure(std::max<uint64_t>(words, 1) * 8));
   za.base = t->zarena.as<uint64_t>();
   za.used = 0;
   za.cap = words;
-  return fill_u64(za.base, 0, words, s);
+  return clear ? fill_u64(za.base, 0, words, s) : FKD_OK;
 }
 
 int ensure_build_scratch(fkd_tree_s* t) {

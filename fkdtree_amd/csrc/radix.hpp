@@ -52,13 +52,16 @@ __device__ inline TileInfo tile_info(const TileMap& tm, uint32_t t) {
 // coordinates themselves (XDIM = dimension): the key of position i in array m
 // is fkey(X[m][i]) and its value is i.  Its count kernel also writes the point
 // records (dim <= 3: coordinates and id in 16 bytes, array 0's blocks) and the
-// starting values of the key-space bounding box that the last pass narrows.
+// starting values of the key-space bounding box that the last pass narrows,
+// and clears the build's scratch arena (zero words that later kernels need).
 struct XSource {
   const float* X;        // [DIM][P] coordinates
   uint32_t P;
   const uint32_t* ids;   // point ids (records)
   float4* rec;           // point records, or null
   uint32_t* bbox;        // [2 DIM]: min keys, then max keys
+  uint64_t* zero;        // words to clear
+  uint64_t zero_words;
 };
 
 // counts[m][256*tb[e] + digit*nt + lt] = number of keys of tile t in array m
@@ -73,6 +76,11 @@ __global__ __launch_bounds__(kThreads) void radix_count_kernel(const uint32_t* _
   const uint32_t t = blockIdx.x, m = blockIdx.y;
   if (XDIM && t == 0 && m == 0 && threadIdx.x < 2u * XDIM)
     xs.bbox[threadIdx.x] = threadIdx.x < (uint32_t)XDIM ? 0xffffffffu : 0u;
+  if (XDIM) {
+    const uint64_t nthreads = (uint64_t)gridDim.x * gridDim.y * kThreads;
+    for (uint64_t i = ((uint64_t)m * gridDim.x + t) * kThreads + threadIdx.x; i < xs.zero_words; i += nthreads)
+      xs.zero[i] = 0ull;
+  }
   const int wave = threadIdx.x / kWave;
 #pragma unroll
   for (int w = 0; w < 4; ++w) hist[w][threadIdx.x] = 0;
