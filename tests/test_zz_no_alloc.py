@@ -50,8 +50,11 @@ def test_noalloc_matches_search_and_oracle(require_gpu):
     idx = (gen.u64(31, 40, 150_000) % np.uint64(p.n)).astype(np.int64)
     big = gen.boxes_around(p.coords[:, idx], 0.02)
     small = gen.boxes_around(p.coords[:, :30_000] + np.float32(0.001), 0.01)
-    for name, boxes in (("150000 boxes", big), ("30000 boxes", small)):
-        print(f"[noalloc] one tree, {name}", flush=True)
+    # every traversal schedule of the fill pass (walk, parked or queued buckets)
+    for bucket, (name, boxes) in ((0, ("150000 boxes", big)), (0, ("30000 boxes", small)),
+                                  (1, ("150000 boxes", big)), (2, ("30000 boxes", small))):
+        print(f"[noalloc] one tree, {name}, bucket {bucket}", flush=True)
+        t.set_tuning(bucket=bucket)
         o_off, o_ids = oracle.search(oc, oi, boxes.mins, boxes.maxs)
         total = int(o_off[-1])
         r_off, r_ids, _ = _search(L.fkd_search, t, boxes, s, cap=total)
