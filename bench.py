@@ -177,6 +177,10 @@ def main():
     D = int(np.floor(np.log2(max(n, 1))))
     build_bytes = 36 * n * D
     achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
+    # compulsory bytes: points read once, boxes read and offsets written once,
+    # every hit id written once (search); the records read and the tree written (build)
+    search_compulsory = 16 * n + 28 * q + 4 * H
+    build_compulsory = 32 * n
     fill_kernel = fill_kernel_name(tree, dim)
     traffic = measured_traffic(args.workload, fill_kernel)
     # HBM bytes the counters saw per fill launch (calibrated FETCH_SIZE +
@@ -209,6 +213,10 @@ def main():
         "visited_nodes": V,
         "search_algorithmic_GBs": search_bytes / (search_ms * 1e-3) / 1e9,
         "build_algorithmic_GBs": build_bytes / (build_ms * 1e-3) / 1e9,
+        "search_compulsory_bytes": search_compulsory,
+        "search_compulsory_GBs": search_compulsory / (search_ms * 1e-3) / 1e9,
+        "build_compulsory_bytes": build_compulsory,
+        "build_compulsory_GBs": build_compulsory / (build_ms * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": f"{fill_kernel} (fill traversal)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic, "algorithmic_bytes": fill_bytes,
@@ -407,6 +415,22 @@ def bench_events(n_events: int, steps: int, batched_only: bool = False) -> dict:
     tot_g, ok_g = ev.search_status(s)
     assert ok_g and tot_g == total
     g.close()
+    # algorithmic bytes of the batch: the per-event sums of the search model
+    # (28 + 12 V + 8 H per box, V from the reference's push rule) and of the
+    # build model (36 n D per event), and the compulsory bytes (16 N + 28 Q +
+    # 4 H for the search, 32 N for the build)
+    d_cnt = DeviceArray(P, np.uint32)
+    d_vis = DeviceArray(P, np.uint32)
+    check(lib().fkd_search_stats(ev._h, P, d_mi.ptr, d_ma.ptr, d_qev.ptr, d_cnt.ptr, d_vis.ptr, s.handle))
+    V = int(d_vis.download(s).astype(np.int64).sum())
+    H = int(d_cnt.download(s).astype(np.int64).sum())
+    assert H == total
+    del d_cnt, d_vis
+    sizes = np.diff(batch.offsets).astype(np.int64)
+    depths = np.floor(np.log2(np.maximum(sizes, 1))).astype(np.int64)
+    search_bytes = 28 * P + 12 * V + 8 * H
+    build_bytes = int((36 * sizes * depths).sum())
+    compulsory = (16 * P + 28 * P + 4 * H) + 32 * P
     ev.close()
     del d_out
 
@@ -448,8 +472,13 @@ def bench_events(n_events: int, steps: int, batched_only: bool = False) -> dict:
     assert plain_hits == total
     for t in trees:
         t.close()
-    return {"n_events": n_events, "points": P, "queries": P, "hits": total,
+    return {"n_events": n_events, "points": P, "queries": P, "hits": total, "visited_nodes": V,
             "batched_ms": batched_ms, "graph_ms": graph_ms, "plain_ms": plain_ms,
+            "algorithmic_bytes": {"search": search_bytes, "build": build_bytes},
+            "algorithmic_GBs": (search_bytes + build_bytes) / (batched_ms * 1e-3) / 1e9,
+            "algorithmic_frac": (search_bytes + build_bytes) / (batched_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "compulsory_bytes": compulsory,
+            "compulsory_GBs": compulsory / (batched_ms * 1e-3) / 1e9,
             "batched_events_per_s": n_events / (batched_ms * 1e-3),
             "graph_events_per_s": n_events / (graph_ms * 1e-3),
             "plain_events_per_s": n_events / (plain_ms * 1e-3),
