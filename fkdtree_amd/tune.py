@@ -176,6 +176,20 @@ def sweep_build(tree, reps: int = 5, candidates=None, settings=None):
     return out
 
 
+def search_stage1():
+    """The first search sweep: the product of CANDIDATES, each setting with the
+    walk schedule named explicitly (set_tuning leaves unnamed keys as they
+    were, so a setting must name every key the sweep ever changes)."""
+    keys = list(CANDIDATES)
+    base = dict(BUCKET_CANDIDATES[0])
+    return [{**dict(zip(keys, values)), **base} for values in itertools.product(*(CANDIDATES[k] for k in keys))]
+
+
+def search_stage2(best: Dict[str, int]):
+    """The traversal schedules on top of the fastest first-stage setting."""
+    return [{**best, **b} for b in BUCKET_CANDIDATES]
+
+
 KEEP_SWEEP = 16  # sweep rows kept per table entry
 FINAL_TOP = 8    # leading settings timed again ...
 FINAL_REPS = 8   # ... with this many times the repetitions
@@ -211,13 +225,9 @@ def autotune(workload: str = "mixed_1m", reps: int = 3, write: bool = True, path
         key = _key(pts.dim, pts.n)
     tree.build()
     dq = d_q.ptr if d_q is not None else None
-    # every setting names every key it depends on (set_tuning leaves the others as they were)
-    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq,
-                candidates={**CANDIDATES, **{k: (v,) for k, v in BUCKET_CANDIDATES[0].items()}})
+    res = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq, settings=search_stage1())
     # the traversal schedules on top of the fastest box order and windows
-    best = res[0][0]
-    sres = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq,
-                 settings=[{**best, **b} for b in BUCKET_CANDIDATES])
+    sres = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=reps, d_qev=dq, settings=search_stage2(res[0][0]))
     res = sorted(sres + res[1:], key=lambda x: x[1])
     # single sweeps differ by noise near the top: the leaders are timed again, longer
     top = sweep(tree, boxes.q, d_mi.ptr, d_ma.ptr, reps=FINAL_REPS * reps, d_qev=dq,

@@ -98,3 +98,18 @@ def test_library_accepts_every_candidate_key():
     hdr = open(os.path.join(os.path.dirname(tune.HERE), "include", "fkdtree", "fkd_capi.h")).read()
     for k in list(tune.CANDIDATES) + list(tune.BUILD_CANDIDATES):
         assert f'"{k}"' in hdr, k
+
+
+def test_every_search_setting_names_every_swept_key():
+    """set_tuning leaves keys a setting does not name as they were, so each
+    setting of both search stages must name every key either stage sweeps:
+    otherwise a setting is timed with its predecessor's traversal schedule
+    and stored without it."""
+    stage1 = tune.search_stage1()
+    keys = set(tune.CANDIDATES) | {k for b in tune.BUCKET_CANDIDATES for k in b}
+    assert len(stage1) == int(__import__("math").prod(len(v) for v in tune.CANDIDATES.values()))
+    assert all(set(s) == keys for s in stage1)
+    stage2 = tune.search_stage2(stage1[0])
+    assert len(stage2) == len(tune.BUCKET_CANDIDATES)
+    assert all(set(s) == keys for s in stage2)
+    assert {s["bucket"] for s in stage2} == {b["bucket"] for b in tune.BUCKET_CANDIDATES}
