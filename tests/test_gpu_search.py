@@ -175,3 +175,30 @@ def test_visited_counts_match_oracle(require_gpu, kind):
             assert np.array_equal(cnt.download(), ocnt), f"{kind} n={n} {label}: counts differ"
             assert np.array_equal(vis.download(), ovis), f"{kind} n={n} {label}: visited differ"
             t.close()
+
+
+def test_event_visited_counts_match_oracle(require_gpu):
+    """The statistics pass over a batch of events (the visited-node totals the
+    bench's per-event byte model uses): every box's hit and visited-node count
+    equals the oracle's walk of its own event's tree (FKDTree_cpu.h:150-157),
+    boxes in mixed event order, events of every size including empty ones."""
+    from fkdtree_amd import DeviceArray, FKDEvents, gen
+    from fkdtree_amd._lib import check, lib
+    sizes = [5000, 0, 1, 2, 17, 1023, 1024, 30_000, 3, 0, 70_001]
+    b = gen.concat_events([gen.mixed_event(s, 61, e) for e, s in enumerate(sizes)])
+    boxes = gen.event_self_boxes(b, 0.015)
+    qev = np.repeat(np.arange(b.n_events, dtype=np.uint32), np.diff(b.offsets))
+    perm = np.argsort(gen.u64(61, 9, boxes.q), kind="stable")
+    mins, maxs, qev = boxes.mins[:, perm].copy(), boxes.maxs[:, perm].copy(), qev[perm].copy()
+    print(f"[visited] events n={b.points.n} q={boxes.q}", flush=True)
+    tc, ti = oracle.build_events(b.points.coords, b.points.ids, b.offsets)
+    ooff, _, ovis = oracle.search_events(tc, ti, b.offsets, qev, mins, maxs, with_visited=True)
+    ev = FKDEvents(b.points.coords, b.points.ids, b.offsets)
+    ev.build()
+    dmi, dma, dq = DeviceArray.from_host(mins), DeviceArray.from_host(maxs), DeviceArray.from_host(qev)
+    cnt = DeviceArray(boxes.q, np.uint32)
+    vis = DeviceArray(boxes.q, np.uint32)
+    check(lib().fkd_search_stats(ev._h, boxes.q, dmi.ptr, dma.ptr, dq.ptr, cnt.ptr, vis.ptr, None))
+    assert np.array_equal(cnt.download().astype(np.int64), np.diff(ooff).astype(np.int64))
+    assert np.array_equal(vis.download(), ovis)
+    ev.close()
