@@ -8,7 +8,7 @@
 // Instead of the level-synchronous frontier (FQueue, FKDTree_cpu.h:126-167)
 // every lane walks its query's visited set depth first without a stack: a
 // 32-bit mask remembers on which depths a right son is still pending, and the
-// heap index of that son follows from the current index by a shift.
+// heap index of that son follows from the current index by a shift (Walk).
 //
 // Queries are processed in spatial (Morton) order of their box centres so
 // that the lanes of a wave walk nearly the same nodes: loads coalesce, the
@@ -86,6 +86,45 @@ __device__ inline void load_rec(const NodeSource& src, uint32_t rec, float (&c)[
 
 enum SearchMode { kCount = 0, kFill = 1, kStats = 2 };
 
+// Depth-first walk state.  k is the current node's heap index counted from 1
+// (record k - 1; sons 2k and 2k + 1; depth 31 - clz(k)).  For j below the
+// current depth, bit j of p is set when the right sibling of the current
+// node's ancestor j depths up (j = 0: the node itself) is still to be
+// visited; that ancestor is then a left son.  Bit `depth` of p is always set
+// (a sentinel standing for the root, which has no sibling).  Descending
+// shifts k and p left by one; returning to the deepest pending right son
+// shifts both right by the lowest set bit t of p: that son is (k >> t) + 1
+// and its own bit is bit 0 of p >> t.  When only the sentinel is left, the
+// same shift leads back to the root (k = 1): the walk is over.
+struct Walk {
+  uint32_t k = 1, p = 1;
+  // After testing node k: on to its left son if go_l (its right son pending
+  // too if go_r), else to its right son if go_r, else to the deepest pending
+  // right son.  False when none is left.
+  __device__ inline bool next(bool go_l, bool go_r) {
+    const bool desc = go_l || go_r;
+    const uint32_t t = (uint32_t)__builtin_ctz(p);  // p != 0 (sentinel)
+    uint32_t co;
+    const uint32_t dk = __builtin_addc(k, k, go_l ? 0u : 1u, &co);  // 2k or 2k + 1
+    const uint32_t dp = __builtin_addc(p, p, go_l && go_r ? 1u : 0u, &co);
+    const uint32_t uk = (k >> t) | 1u;
+    const uint32_t up = (p >> t) ^ 1u;
+    k = desc ? dk : uk;
+    p = desc ? dp : up;
+    return desc || uk != 1u;
+  }
+  // Leaves node k without visiting its sons.
+  __device__ inline bool pop() { return next(false, false); }
+};
+
+// Byte offset, in a lane's twice-stored box, of the bounds rotated like the
+// records of depth 31 - c (split dimension first): the rotation tables below
+// are indexed by c = clz(k), so the walk needs no depth variable.
+template <int DIM>
+__device__ inline uint8_t rot_of_clz(uint32_t c) {
+  return (uint8_t)(((31u - c) % (uint32_t)DIM) * (uint32_t)sizeof(float2));
+}
+
 // Fill staging: hits go to a ring of kRing ids per lane in LDS, ring index
 // i holding the id for output slot wb + i (wb = the query's first output slot
 // rounded down to 4 ids) at ring slot i % kRing.  Whenever some lane's ring
@@ -115,9 +154,9 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
   constexpr int BS = 2 * DIM + 1;  // pairs per lane (odd: the lanes' reads avoid bank conflicts)
   __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
   __shared__ float2 s_box[LDS_BOX ? kThreads * BS : 1];
-  __shared__ uint32_t s_rot[LDS_BOX ? 32 : 1];  // depth -> byte offset of its rotation
+  __shared__ uint8_t s_rot[LDS_BOX ? 32 : 1];  // clz(k) -> byte offset of its rotation
   if (LDS_BOX) {
-    if (threadIdx.x < 32u) s_rot[threadIdx.x] = (threadIdx.x % (uint32_t)DIM) * (uint32_t)sizeof(float2);
+    if (threadIdx.x < 32u) s_rot[threadIdx.x] = rot_of_clz<DIM>(threadIdx.x);
     __syncthreads();
   }
   uint32_t slot = search_block(blockIdx.x, gridDim.x, a.block_order) * kThreads + threadIdx.x;
@@ -141,7 +180,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     mn[m] = a.mins[(uint64_t)m * a.nq + slot];
     mx[m] = a.maxs[(uint64_t)m * a.nq + slot];
     // A NaN lower bound matches nothing and sends the walk left only; with
-    // (-inf, NaN) the three compares below give exactly that.
+    // (-inf, NaN) the two compares below give exactly that.
     if (mn[m] != mn[m]) {
       mn[m] = -__builtin_huge_valf();
       mx[m] = __builtin_nanf("");
@@ -176,29 +215,31 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
       fq += 4u;
     }
   };
-  // node has a left (right) son iff node < tl (tr), i.e. 2 node + 1 (2) < n
-  const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
-  // pending: bit d set = the right son at depth d is still to be visited;
-  // bit 0 is a sentinel, so the deepest pending depth is 31 - clz(pending).
-  uint32_t node = 0, depth = 0, pending = 1, vis = 0;
+  // node k has a left (right) son iff 2k (2k + 1) <= n, i.e. k < tl (tr)
+  const uint32_t tl = (n >> 1) + 1u, tr = ((n - 1u) >> 1) + 1u;
+  const uint32_t rb = base - 1u;  // record of node k: rb + k
+  Walk w;
+  uint32_t vis = 0;
   bool alive = n != 0;
-  uint32_t wb_depth = 32;
+  // WB: nodes on depths 0 .. B count, i.e. clz(k) >= 31 - B
+  uint32_t wb_clz = 0;
   if (WB) {
     const uint32_t F = 31u - (uint32_t)__builtin_clz(n + 1u);
-    wb_depth = F > 3u ? F - 3u : (F == 3u ? 0u : 32u);
+    wb_clz = F > 3u ? 34u - F : (F == 3u ? 31u : 0u);
   }
   while (alive) {
+    const uint32_t cz = (uint32_t)__builtin_clz(w.k);
     if (!FILL) {
       if (WB)
-        vis += depth <= wb_depth ? 1u : 0u;
+        vis += cz >= wb_clz ? 1u : 0u;
       else
         ++vis;
     }
     float x[DIM], lo[DIM], hi[DIM];
     uint32_t id;
-    load_rec<DIM, BUF>(src, base + node, x, id);
+    load_rec<DIM, BUF>(src, rb + w.k, x, id);
     if (LDS_BOX) {
-      const float2* bx = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[depth]);
+      const float2* bx = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[cz]);
 #pragma unroll
       for (int j = 0; j < DIM; ++j) {
         const float2 t = bx[j];
@@ -211,10 +252,11 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     }
     // x[0] is the split value and lo[0], hi[0] the bounds on the split
     // dimension.  The reference's rule (left son unless the split value is
-    // below the box, right son if it is below or inside) from three compares:
-    // with lo[0] not NaN, !(x < lo) is x >= lo, and (x < lo) || inside is
-    // (x < lo) || (x <= hi).
-    const bool ge = x[0] >= lo[0], le = x[0] <= hi[0], lower = x[0] < lo[0];
+    // below the box, right son if it is below or inside) from two compares:
+    // neither x (points hold no NaN) nor lo[0] (a NaN lower bound became
+    // -inf) is NaN, so "below" (x < lo) is !(x >= lo), and below || inside
+    // is !(x >= lo) || (x <= hi).
+    const bool ge = x[0] >= lo[0], le = x[0] <= hi[0];
     bool in = ge & le;
 #pragma unroll
     for (int j = 1; j < DIM; ++j) in = in & (x[j] <= hi[j]) & (x[j] >= lo[j]);
@@ -228,17 +270,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     } else {
       c += in ? 1u : 0u;
     }
-    const bool go_l = ge && node < tl;
-    const bool go_r = (lower || le) && node < tr;
-    const bool desc = go_l || go_r;
-    pending |= (go_l && go_r) ? (2u << depth) : 0u;
-    const uint32_t b = 31u - (uint32_t)__builtin_clz(pending);  // deepest pending depth
-    const uint32_t up = (node + 1u) >> (depth - b);              // its right son
-    const uint32_t down = 2u * node + (go_l ? 1u : 2u);
-    alive = desc || pending != 1u;
-    node = desc ? down : up;
-    pending = desc ? pending : __builtin_amdgcn_ubfe(pending, 0u, b);  // clears bit b
-    depth = desc ? depth + 1u : b;
+    alive = w.next(ge && w.k < tl, (!ge || le) && w.k < tr);
   }
   if (FILL) {
     write_groups();
@@ -282,9 +314,9 @@ void bucket_search_kernel(SearchArgs a) {
   constexpr int BS = 2 * DIM + 1;
   __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
   __shared__ float2 s_box[kThreads * BS];
-  __shared__ uint32_t s_rot[32];
+  __shared__ uint8_t s_rot[32];  // clz(k) -> byte offset of its rotation
   __shared__ uint32_t s_bq[BQ > 0 ? BQ * kThreads : 1];  // queue entry k of thread t at [k * kThreads + t]
-  if (threadIdx.x < 32u) s_rot[threadIdx.x] = (threadIdx.x % (uint32_t)DIM) * (uint32_t)sizeof(float2);
+  if (threadIdx.x < 32u) s_rot[threadIdx.x] = rot_of_clz<DIM>(threadIdx.x);
   __syncthreads();
   uint32_t slot = search_block(blockIdx.x, gridDim.x, a.block_order) * kThreads + threadIdx.x;
   if (slot >= a.nq) return;
@@ -316,8 +348,9 @@ void bucket_search_kernel(SearchArgs a) {
   // never reached).
   const uint32_t F = 31u - (uint32_t)__builtin_clz(n + 1u);
   const uint32_t B = F > 3u ? F - 3u : (F == 3u ? 0u : 32u);
+  const uint32_t cB = 31u - B;  // clz(k) of the nodes on depth B
   // The bounds compared with slot j of a record on depth d are the pair at
-  // byte s_rot[d] + 8 j of my_box (dimension (d + j) % DIM).  The fill pass
+  // byte s_rot[31 - d] + 8 j of my_box (dimension (d + j) % DIM).  The fill pass
   // keeps those of depths B .. B + DIM - 1 in registers (bk); the count pass
   // reads them per use, so they take no registers across its loop.
   constexpr bool kRegBounds = FILL;
@@ -339,7 +372,7 @@ void bucket_search_kernel(SearchArgs a) {
       for (int j = 0; j < DIM; ++j) bb[j] = bk[kRegBounds ? k % (uint32_t)DIM : 0][j];
     } else {
       const float2* bx =
-          reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[(B + k) & 31u]);
+          reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[(cB - k) & 31u]);
 #pragma unroll
       for (int j = 0; j < DIM; ++j) bb[j] = bx[j];
     }
@@ -406,11 +439,12 @@ void bucket_search_kernel(SearchArgs a) {
     const float* f = a.recs + (uint64_t)rec * 4u;
     return make_float4(f[0], f[1], f[2], 0.f);
   };
-  // depths B, B+1, B+2 of the bucket rooted at r: nodes r; 2r+1, 2r+2;
-  // 4r+3 .. 4r+6 (all exist)
-  auto load7 = [&](uint32_t r, float4 (&v)[7]) {
-    const uint32_t r1 = base + 2u * r + 1u, r2 = base + 4u * r + 3u;
-    v[0] = rec16(base + r);
+  const uint32_t rb = base - 1u;  // record of node k (heap index from 1): rb + k
+  // depths B, B+1, B+2 of the bucket rooted at node k: k; 2k, 2k+1;
+  // 4k .. 4k+3 (all exist)
+  auto load7 = [&](uint32_t k, float4 (&v)[7]) {
+    const uint32_t r1 = rb + 2u * k, r2 = rb + 4u * k;
+    v[0] = rec16(rb + k);
 #pragma unroll
     for (int i = 0; i < 2; ++i) v[1 + i] = rec16(r1 + (uint32_t)i);
 #pragma unroll
@@ -427,34 +461,37 @@ void bucket_search_kernel(SearchArgs a) {
 #pragma unroll
     for (int i = 3; i < 7; ++i) report(__float_as_uint(v[i].w), inside(v[i], bb));
   };
-  // depth B + 3 = F under r: nodes 8r+7 .. 8r+14 that exist (a prefix of the depth)
-  auto last_depth = [&](uint32_t r) {
-    const uint32_t l3 = 8u * r + 7u;
-    if (l3 < n) {
+  // depth B + 3 = F under node k: nodes 8k .. 8k+7 that exist (a prefix of the depth)
+  auto last_depth = [&](uint32_t k) {
+    const uint32_t l3 = 8u * k;
+    if (l3 <= n) {
       make_room();
-      const uint32_t last = n - 1u;
       float4 u[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const uint32_t k = l3 + (uint32_t)i;
-        u[i] = rec16(base + (k < last ? k : last));
+        const uint32_t j = l3 + (uint32_t)i;
+        u[i] = rec16(rb + (j < n ? j : n));
       }
       float2 bb[DIM];
       bounds(3u, bb);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) report(__float_as_uint(u[i].w), (l3 + (uint32_t)i < n) & inside(u[i], bb));
+      for (int i = 0; i < 8; ++i) report(__float_as_uint(u[i].w), (l3 + (uint32_t)i <= n) & inside(u[i], bb));
     }
   };
-  const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
-  uint32_t node = 0, depth = 0, pending = 1, work = 0;
+  // node k has a left (right) son iff 2k (2k + 1) <= n, i.e. k < tl (tr)
+  const uint32_t tl = (n >> 1) + 1u, tr = ((n - 1u) >> 1) + 1u;
+  Walk w;
+  uint32_t work = 0;
   bool alive = n != 0;
-  // one step of the reference walk from `node` (not a bucket root)
+  auto at_bucket = [&]() { return (uint32_t)__builtin_clz(w.k) == cB; };
+  // one step of the reference walk from node w.k (not a bucket root)
   auto walk_step = [&]() {
     ++work;
     float x[DIM];
     uint32_t id;
-    load_rec<DIM, BUF>(src, base + node, x, id);
-    const float2* bx = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[depth]);
+    load_rec<DIM, BUF>(src, rb + w.k, x, id);
+    const float2* bx =
+        reinterpret_cast<const float2*>(reinterpret_cast<const char*>(my_box) + s_rot[__builtin_clz(w.k)]);
     float lo[DIM], hi[DIM];
 #pragma unroll
     for (int j = 0; j < DIM; ++j) {
@@ -462,31 +499,16 @@ void bucket_search_kernel(SearchArgs a) {
       lo[j] = t.x;
       hi[j] = t.y;
     }
-    const bool ge = x[0] >= lo[0], le = x[0] <= hi[0], lower = x[0] < lo[0];
+    // the walk kernel's rule (neither x nor lo[0] is NaN)
+    const bool ge = x[0] >= lo[0], le = x[0] <= hi[0];
     bool in = ge & le;
 #pragma unroll
     for (int j = 1; j < DIM; ++j) in = in & (x[j] <= hi[j]) & (x[j] >= lo[j]);
     report(id, in);
-    const bool go_l = ge && node < tl;
-    const bool go_r = (lower || le) && node < tr;
-    const bool desc = go_l || go_r;
-    pending |= (go_l && go_r) ? (2u << depth) : 0u;
-    const uint32_t b = 31u - (uint32_t)__builtin_clz(pending);
-    const uint32_t up = (node + 1u) >> (depth - b);
-    const uint32_t down = 2u * node + (go_l ? 1u : 2u);
-    alive = desc || pending != 1u;
-    node = desc ? down : up;
-    pending = desc ? pending : __builtin_amdgcn_ubfe(pending, 0u, b);
-    depth = desc ? depth + 1u : b;
+    alive = w.next(ge && w.k < tl, (!ge || le) && w.k < tr);
   };
-  // leave the bucket root `node`: on to the deepest pending right son
-  auto pop = [&]() {
-    const uint32_t b = 31u - (uint32_t)__builtin_clz(pending);
-    alive = pending != 1u;
-    node = (node + 1u) >> (depth - b);
-    pending = __builtin_amdgcn_ubfe(pending, 0u, b);
-    depth = b;
-  };
+  // leave the bucket root w.k: on to the deepest pending right son
+  auto pop = [&]() { alive = w.pop(); };
   if (BQ == 0) {
     bool parked = alive && B == 0u;
     const uint32_t park8 = a.bucket_park;
@@ -496,7 +518,7 @@ void bucket_search_kernel(SearchArgs a) {
       make_room();
       if (alive && !parked) {
         walk_step();
-        parked = alive && depth == B;
+        parked = alive && at_bucket();
       }
       const uint64_t pm = __ballot(parked);
       if (pm == 0ull) continue;
@@ -505,11 +527,11 @@ void bucket_search_kernel(SearchArgs a) {
       if (parked) {
         work += 2u;
         float4 v[7];
-        load7(node, v);
+        load7(w.k, v);
         test7(v);
-        last_depth(node);
+        last_depth(w.k);
         pop();
-        parked = alive && depth == B;
+        parked = alive && at_bucket();
       }
     }
   } else {
@@ -521,8 +543,8 @@ void bucket_search_kernel(SearchArgs a) {
       // room for one walk step and a bucket's first 7 nodes
       make_room();
       // queue the bucket roots at the head of the walk while there is room
-      while (alive && depth == B && qt - qh < (uint32_t)BQ) {
-        my_q[(qt % (uint32_t)BQ) * kThreads] = node;
+      while (alive && at_bucket() && qt - qh < (uint32_t)BQ) {
+        my_q[(qt % (uint32_t)BQ) * kThreads] = w.k;
         ++qt;
         pop();
       }
@@ -532,7 +554,7 @@ void bucket_search_kernel(SearchArgs a) {
         r = my_q[(qh % (uint32_t)BQ) * kThreads];
         load7(r, v);
       }
-      if (alive && depth != B) walk_step();
+      if (alive && !at_bucket()) walk_step();
       if (has_b) {
         work += 2u;
         ++qh;
