@@ -100,21 +100,34 @@ struct Walk {
   uint32_t k = 1, p = 1;
   // After testing node k: on to its left son if go_l (its right son pending
   // too if go_r), else to its right son if go_r, else to the deepest pending
-  // right son.  False when none is left.
-  __device__ inline bool next(bool go_l, bool go_r) {
-    const bool desc = go_l || go_r;
+  // right son.  False when none is left.  go_l and go_r come as the wave's
+  // lane masks (ballots), so that they combine on the scalar unit, and each
+  // of the son's k and p is one add with carry in (2k + carry), which the
+  // compiler would otherwise form from a select and a shift.
+  __device__ inline bool next(uint64_t go_l, uint64_t go_r) {
+    const bool desc = __builtin_amdgcn_inverse_ballot_w64(go_l | go_r);
     const uint32_t t = (uint32_t)__builtin_ctz(p);  // p != 0 (sentinel)
-    uint32_t co;
-    const uint32_t dk = __builtin_addc(k, k, go_l ? 0u : 1u, &co);  // 2k or 2k + 1
-    const uint32_t dp = __builtin_addc(p, p, go_l && go_r ? 1u : 0u, &co);
+    uint32_t dk, dp;
+    uint64_t co;
+    asm("v_addc_co_u32 %0, %1, %2, %2, %3" : "=v"(dk), "=s"(co) : "v"(k), "s"(~go_l));
+    asm("v_addc_co_u32 %0, %1, %2, %2, %3" : "=v"(dp), "=s"(co) : "v"(p), "s"(go_l & go_r));
     const uint32_t uk = (k >> t) | 1u;
     const uint32_t up = (p >> t) ^ 1u;
     k = desc ? dk : uk;
     p = desc ? dp : up;
     return desc || uk != 1u;
   }
+  // After testing node k, whose split value x gave ge = (x >= lo) and
+  // le = (x <= hi) against the bounds of its dimension: the reference's rule
+  // (left son unless x < lo, right son if x < lo or x <= hi, a son only if it
+  // exists: k < tl, k < tr).  Neither x nor lo is NaN, so x < lo is !ge.
+  __device__ inline bool next(bool ge, bool le, uint32_t tl, uint32_t tr) {
+    const uint64_t g = __builtin_amdgcn_ballot_w64(ge), l = __builtin_amdgcn_ballot_w64(le);
+    const uint64_t el = __builtin_amdgcn_ballot_w64(k < tl), er = __builtin_amdgcn_ballot_w64(k < tr);
+    return next(g & el, (~g | l) & er);
+  }
   // Leaves node k without visiting its sons.
-  __device__ inline bool pop() { return next(false, false); }
+  __device__ inline bool pop() { return next(0ull, 0ull); }
 };
 
 // Byte offset, in a lane's twice-stored box, of the bounds rotated like the
@@ -270,7 +283,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
     } else {
       c += in ? 1u : 0u;
     }
-    alive = w.next(ge && w.k < tl, (!ge || le) && w.k < tr);
+    alive = w.next(ge, le, tl, tr);
   }
   if (FILL) {
     write_groups();
@@ -505,7 +518,7 @@ void bucket_search_kernel(SearchArgs a) {
 #pragma unroll
     for (int j = 1; j < DIM; ++j) in = in & (x[j] <= hi[j]) & (x[j] >= lo[j]);
     report(id, in);
-    alive = w.next(ge && w.k < tl, (!ge || le) && w.k < tr);
+    alive = w.next(ge, le, tl, tr);
   };
   // leave the bucket root w.k: on to the deepest pending right son
   auto pop = [&]() { alive = w.pop(); };
