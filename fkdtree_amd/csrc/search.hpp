@@ -316,10 +316,12 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
 //    LDS queue of BQ entries and walks on; every iteration issues the load of
 //    its walk step and the loads of its oldest queued bucket together, so one
 //    memory latency covers both (a lane whose queue is full waits).
-// The parked count pass fits 8 waves per SIMD without spilling; the others
-// ask for 4 (the fill's LDS ring allows 4-5).
+// The parked count pass through a buffer resource fits 8 waves per SIMD
+// without spilling; the others ask for 4 (the fill's LDS ring allows 4-5, and
+// the flat-pointer count pass would spill at 8).
 template <int DIM, int MODE, bool BUF, int BQ>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE == kCount && BQ == 0 ? 8 : 4)))
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(MODE == kCount && BQ == 0 && BUF ? 8 : 4)))
 void bucket_search_kernel(SearchArgs a) {
   static_assert(DIM >= 1 && DIM <= 3, "one 16-byte record per node");
   static_assert(MODE == kCount || MODE == kFill, "count and fill passes only");
