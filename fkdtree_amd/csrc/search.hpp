@@ -130,9 +130,10 @@ struct Walk {
   __device__ inline bool pop() { return next(0ull, 0ull); }
 };
 
-// Byte offset, in a lane's twice-stored box, of the bounds rotated like the
-// records of depth 31 - c (split dimension first): the rotation tables below
-// are indexed by c = clz(k), so the walk needs no depth variable.
+// Byte offset, in a lane's box (the pairs of dimensions 0 .. DIM-1, then
+// 0 .. DIM-2), of the bounds rotated like the records of depth 31 - c (split
+// dimension first): the rotation tables below are indexed by c = clz(k), so
+// the walk needs no depth variable.
 template <int DIM>
 __device__ inline uint8_t rot_of_clz(uint32_t c) {
   return (uint8_t)(((31u - c) % (uint32_t)DIM) * (uint32_t)sizeof(float2));
@@ -152,19 +153,24 @@ constexpr int kBucketQueueCount = 4;  // in the count pass (its LDS allows 8 wav
 // The loop body is branch free except for writing out staged ids: the next
 // node is selected among (left son, right son, pending right son of the
 // deepest pending depth) with conditional moves.  Every lane keeps its box
-// twice in LDS (the (min, max) pairs of all dimensions, then again), so the
-// box rotated like the node records of depth d (split dimension first) is
-// the DIM pairs starting at pair d % DIM, found through a 32-entry table:
-// no selection by dimension in the loop.
+// in LDS as the (min, max) pairs of dimensions 0 .. DIM-1, 0 .. DIM-2 (2 DIM - 1
+// pairs, an odd stride, so the lanes' reads avoid bank conflicts), so the box
+// rotated like the node records of depth d (split dimension first) is the DIM
+// pairs starting at pair d % DIM, found through a 32-entry table: no
+// selection by dimension in the loop.
 // WB (count pass): report each box's work as the bucketed fill pass will do
 // it (a step per node down to the bucket depth B, none inside buckets; B as in
 // bucket_search_kernel) instead of nodes visited, so the fill's window sort
 // balances the bucketed walk.
+// The fill pass asks for the waves per SIMD its LDS allows: 6 blocks a CU up
+// to dim 3, 5 above.
 template <int DIM, int MODE, bool BUF, bool WB = false>
-__global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(MODE == kFill ? (DIM <= 3 ? 6 : 5) : 1)))
+void search_kernel(SearchArgs a) {
   constexpr bool FILL = MODE == kFill;
   constexpr bool LDS_BOX = DIM >= 2;
-  constexpr int BS = 2 * DIM + 1;  // pairs per lane (odd: the lanes' reads avoid bank conflicts)
+  constexpr int BS = 2 * DIM - 1;  // pairs per lane
   __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
   __shared__ float2 s_box[LDS_BOX ? kThreads * BS : 1];
   __shared__ uint8_t s_rot[LDS_BOX ? 32 : 1];  // clz(k) -> byte offset of its rotation
@@ -198,7 +204,10 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
       mn[m] = -__builtin_huge_valf();
       mx[m] = __builtin_nanf("");
     }
-    if (LDS_BOX) my_box[m] = my_box[m + DIM] = make_float2(mn[m], mx[m]);
+    if (LDS_BOX) {
+      my_box[m] = make_float2(mn[m], mx[m]);
+      if (m + 1 < DIM) my_box[m + DIM] = make_float2(mn[m], mx[m]);
+    }
   }
   NodeSource src;
   src.flat = a.recs;
@@ -316,17 +325,18 @@ __global__ __launch_bounds__(kThreads) void search_kernel(SearchArgs a) {
 //    LDS queue of BQ entries and walks on; every iteration issues the load of
 //    its walk step and the loads of its oldest queued bucket together, so one
 //    memory latency covers both (a lane whose queue is full waits).
-// The parked count pass through a buffer resource fits 8 waves per SIMD
-// without spilling; the others ask for 4 (the fill's LDS ring allows 4-5, and
-// the flat-pointer count pass would spill at 8).
+// Occupancy: the parked count pass through a buffer resource fits 8 waves
+// per SIMD without spilling (the flat-pointer one would spill); the parked
+// fill pass 6, which its LDS (ring and box: 26.6 KB a block) also allows; the
+// queued passes 4.
 template <int DIM, int MODE, bool BUF, int BQ>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(MODE == kCount && BQ == 0 && BUF ? 8 : 4)))
+__attribute__((amdgpu_waves_per_eu(BQ > 0 ? 4 : (MODE == kCount ? (BUF ? 8 : 4) : 6))))
 void bucket_search_kernel(SearchArgs a) {
   static_assert(DIM >= 1 && DIM <= 3, "one 16-byte record per node");
   static_assert(MODE == kCount || MODE == kFill, "count and fill passes only");
   constexpr bool FILL = MODE == kFill;
-  constexpr int BS = 2 * DIM + 1;
+  constexpr int BS = 2 * DIM - 1;  // box pairs per lane, as in search_kernel
   __shared__ __align__(16) uint32_t ring[FILL ? kThreads * kRingRow : 1];
   __shared__ float2 s_box[kThreads * BS];
   __shared__ uint8_t s_rot[32];  // clz(k) -> byte offset of its rotation
@@ -356,7 +366,8 @@ void bucket_search_kernel(SearchArgs a) {
       lo = -__builtin_huge_valf();
       hi = __builtin_nanf("");
     }
-    my_box[m] = my_box[m + DIM] = make_float2(lo, hi);
+    my_box[m] = make_float2(lo, hi);
+    if (m + 1 < DIM) my_box[m + DIM] = make_float2(lo, hi);
   }
   // F full depths (n >= 2^F - 1); buckets are rooted on depth B.  A tree of
   // fewer than 7 nodes (F < 3) has no full bucket and is walked (B = 32 is
