@@ -177,6 +177,37 @@ def test_visited_counts_match_oracle(require_gpu, kind):
             t.close()
 
 
+@pytest.mark.parametrize("dim", [1, 2, 4])
+def test_visited_counts_other_dims(require_gpu, dim):
+    """The same for dims 1, 2 and 4 (box bounds in registers for dim 1, 3 and
+    7 bound pairs per lane in LDS for dims 2 and 4, rotated by depth % dim):
+    hit and visited-node counts of every box equal the oracle's walk
+    (FKDTree_cpu.h:150-157), the id sets of the searches its level-BFS search
+    (FKDTree_cpu.h:123-169), special boxes included."""
+    from fkdtree_amd import DeviceArray
+    from fkdtree_amd._lib import check, lib
+    for n in (1, 2, 7, 100, 4097, 20_000):
+        p = adversarial_points("mixed", n, dim=dim)
+        for label, boxes in (("hw0.04", self_boxes(p, 0.04)), ("special", special_boxes(p))):
+            print(f"[visited] dim {dim} n={n} {label} q={boxes.q}", flush=True)
+            t = FKDTree(coords=p.coords, ids=p.ids)
+            t.build()
+            dmi = DeviceArray.from_host(boxes.mins)
+            dma = DeviceArray.from_host(boxes.maxs)
+            cnt = DeviceArray(boxes.q, np.uint32)
+            vis = DeviceArray(boxes.q, np.uint32)
+            check(lib().fkd_search_stats(t._h, boxes.q, dmi.ptr, dma.ptr, None, cnt.ptr, vis.ptr, None))
+            tc, ti = oracle.build(p.coords, p.ids)
+            ocnt, ovis = oracle.search_counts(tc, ti, boxes.mins, boxes.maxs)
+            assert np.array_equal(cnt.download(), ocnt), f"dim {dim} n={n} {label}: counts differ"
+            assert np.array_equal(vis.download(), ovis), f"dim {dim} n={n} {label}: visited differ"
+            goff, gids = t.search_in_the_box_multiple_csr(boxes.mins, boxes.maxs)
+            ooff, oids = oracle.search(tc, ti, boxes.mins, boxes.maxs)
+            assert np.array_equal(goff, ooff), f"dim {dim} n={n} {label}: search counts differ"
+            assert np.array_equal(sorted_sets(goff, gids), sorted_sets(ooff, oids)), f"dim {dim} n={n} {label}"
+            t.close()
+
+
 def test_event_visited_counts_match_oracle(require_gpu):
     """The statistics pass over a batch of events (the visited-node totals the
     bench's per-event byte model uses): every box's hit and visited-node count
