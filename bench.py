@@ -323,7 +323,7 @@ def fill_kernel_name(tree, dim: int) -> str:
         return f"fkd::bucket_search_kernel<{dim}, 1, true, 0>"
     if sched == 2:
         return f"fkd::bucket_search_kernel<{dim}, 1, true, 8>"
-    return f"fkd::search_kernel<{dim}, 1, true, false>"
+    return f"fkd::search_kernel<{dim}, 1, true, 0>"
 
 
 def measured_traffic(workload: str, kernel: str):

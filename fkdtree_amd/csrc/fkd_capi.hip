@@ -428,7 +428,8 @@ struct fkd_tree_s {
   DevBuf q_counts, q_off, q_hits, q_boxes, q_ev, q_aos;  // search scratch
   DevBuf q_keys, q_perm, q_sbox, q_rcounts, q_roffsets;
   DevBuf q_bwork, q_border;    // fill pass: work per search block and dispatch order
-  DevBuf q_wslot;              // count pass: visits in processing order
+  DevBuf q_wslot;              // count pass: work per box in processing order ...
+  bool q_work_valid = false;   // ... recorded by the last count pass (else the fill sorts nothing)
   DevBuf q_sorder;             // fill pass: dispatch slot -> processing slot
   DevBuf zarena;               // look-back descriptors and block counters, zeroed per build / search
   uint64_t last_capacity = 0;
@@ -835,16 +836,25 @@ int launch_search(fkd_tree_s* t, const fkd::SearchArgs& a, hipStream_t s, const 
       if (buf) fkd::bucket_search_kernel<BD, BM, true, BQ><<<blocks, fkd::kThreads, 0, s>>>(b);
       else fkd::bucket_search_kernel<BD, BM, false, BQ><<<blocks, fkd::kThreads, 0, s>>>(b);
     }
-  } else if (MODE == fkd::kCount && DIM <= 3 && t->tune.bucket && b.work_slot) {
-    // the fill pass is bucketed: its window sort wants the bucketed work
-    if (bytes < (1ull << 32))
-      fkd::search_kernel<DIM, MODE, true, true><<<blocks, fkd::kThreads, 0, s>>>(b);
-    else
-      fkd::search_kernel<DIM, MODE, false, true><<<blocks, fkd::kThreads, 0, s>>>(b);
+  } else if (MODE == fkd::kCount) {
+    // the work figure the fill will sort by: none, nodes visited, or the
+    // bucketed fill's work when the fill pass is bucketed
+    const int wk = !b.work_slot ? 0 : (DIM <= 3 && t->tune.bucket ? 2 : 1);
+    const bool buf = bytes < (1ull << 32);
+    if (wk == 0) {
+      if (buf) fkd::search_kernel<DIM, MODE, true, 0><<<blocks, fkd::kThreads, 0, s>>>(b);
+      else fkd::search_kernel<DIM, MODE, false, 0><<<blocks, fkd::kThreads, 0, s>>>(b);
+    } else if (wk == 1) {
+      if (buf) fkd::search_kernel<DIM, MODE, true, 1><<<blocks, fkd::kThreads, 0, s>>>(b);
+      else fkd::search_kernel<DIM, MODE, false, 1><<<blocks, fkd::kThreads, 0, s>>>(b);
+    } else {
+      if (buf) fkd::search_kernel<DIM, MODE, true, 2><<<blocks, fkd::kThreads, 0, s>>>(b);
+      else fkd::search_kernel<DIM, MODE, false, 2><<<blocks, fkd::kThreads, 0, s>>>(b);
+    }
   } else if (bytes < (1ull << 32)) {
-    fkd::search_kernel<DIM, MODE, true><<<blocks, fkd::kThreads, 0, s>>>(b);
+    fkd::search_kernel<DIM, MODE, true, 0><<<blocks, fkd::kThreads, 0, s>>>(b);
   } else {
-    fkd::search_kernel<DIM, MODE, false><<<blocks, fkd::kThreads, 0, s>>>(b);
+    fkd::search_kernel<DIM, MODE, false, 0><<<blocks, fkd::kThreads, 0, s>>>(b);
   }
   return launch_check(what);
 }
@@ -961,16 +971,21 @@ int search_count_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float
     a.maxs = a.mins + (uint64_t)DIM * n;
     a.counts = t->q_counts.as<uint32_t>();
     FKD_TRY(t->q_wslot.ensure((size_t)std::max<uint32_t>(nq, 1) * 4));
-    a.work_slot = t->q_wslot.as<uint32_t>();
+    // The count pass records each box's work only for a fill that orders
+    // boxes or blocks by it (fill_window, fill_order as set now).
+    const uint32_t nb = (uint32_t)((nq + fkd::kThreads - 1) / fkd::kThreads);
+    t->q_work_valid = (t->tune.fill_window != 0 && nq > t->tune.fill_window) || (t->tune.fill_order && nb >= 64);
+    a.work_slot = t->q_work_valid ? t->q_wslot.as<uint32_t>() : nullptr;
     const uint32_t cw = t->tune.count_window;
     if (cw != 0 && nq > cw) {
       // Work is unknown before counting; the box volume (expected hits for
       // the handle's mean point density) stands in for it.  The count pass
-      // overwrites the estimate with the visited counts.
+      // overwrites the estimate with its own work figure.
       const float density = (float)t->P / (float)std::max<uint32_t>(t->n_events, 1u);
-      fkd::volume_work_kernel<DIM><<<grid_for(nq), 256, 0, s>>>(a.mins, a.maxs, nq, density, a.work_slot);
+      uint32_t* est = t->q_wslot.as<uint32_t>();
+      fkd::volume_work_kernel<DIM><<<grid_for(nq), 256, 0, s>>>(a.mins, a.maxs, nq, density, est);
       FKD_TRY(launch_check("volume_work"));
-      FKD_TRY(window_sort(t, cw, a.work_slot, nq, s));
+      FKD_TRY(window_sort(t, cw, est, nq, s));
       a.slot_order = t->q_sorder.as<uint32_t>();
     }
     FKD_TRY((launch_search<DIM, fkd::kCount>(t, a, s, "search_count")));
@@ -1017,8 +1032,10 @@ int search_fill_impl(fkd_tree_s* t, uint32_t nq, const float* mins, const float*
   a.out_aligned = (reinterpret_cast<uintptr_t>(out) & 63u) == 0;
   const uint32_t nb = (uint32_t)((nq + fkd::kThreads - 1) / fkd::kThreads);
   const uint32_t win = t->tune.fill_window;
-  const bool sorted = win != 0 && nq > win;
-  const bool ordered = t->tune.fill_order && nb >= 64;
+  // by the work the count pass recorded (none if the tuning asked for no
+  // ordering when it ran: any order gives the same results)
+  const bool sorted = win != 0 && nq > win && t->q_work_valid;
+  const bool ordered = t->tune.fill_order && nb >= 64 && t->q_work_valid;
   if (sorted || ordered) FKD_TRY(t->q_bwork.ensure((size_t)nb * 4));
   if (sorted) {
     FKD_TRY(window_sort(t, win, t->q_wslot.as<uint32_t>(), nq, s));

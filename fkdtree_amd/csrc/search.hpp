@@ -158,13 +158,14 @@ constexpr int kBucketQueueCount = 4;  // in the count pass (its LDS allows 8 wav
 // rotated like the node records of depth d (split dimension first) is the DIM
 // pairs starting at pair d % DIM, found through a 32-entry table: no
 // selection by dimension in the loop.
-// WB (count pass): report each box's work as the bucketed fill pass will do
-// it (a step per node down to the bucket depth B, none inside buckets; B as in
-// bucket_search_kernel) instead of nodes visited, so the fill's window sort
-// balances the bucketed walk.
+// WK (count pass): the work figure each box reports in work_slot for the
+// fill's window sort and block order: 0 none (the fill sorts nothing, and the
+// loop keeps no counter), 1 nodes visited, 2 the work of the bucketed fill (a
+// step per node down to the bucket depth B, none inside buckets; B as in
+// bucket_search_kernel), so that the window sort balances the bucketed walk.
 // The fill pass asks for the waves per SIMD its LDS allows: 6 blocks a CU up
 // to dim 3, 5 above.
-template <int DIM, int MODE, bool BUF, bool WB = false>
+template <int DIM, int MODE, bool BUF, int WK>
 __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(MODE == kFill ? (DIM <= 3 ? 6 : 5) : 1)))
 void search_kernel(SearchArgs a) {
@@ -243,20 +244,18 @@ void search_kernel(SearchArgs a) {
   Walk w;
   uint32_t vis = 0;
   bool alive = n != 0;
-  // WB: nodes on depths 0 .. B count, i.e. clz(k) >= 31 - B
+  // WK 2: nodes on depths 0 .. B count, i.e. clz(k) >= 31 - B
   uint32_t wb_clz = 0;
-  if (WB) {
+  if (MODE == kCount && WK == 2) {
     const uint32_t F = 31u - (uint32_t)__builtin_clz(n + 1u);
     wb_clz = F > 3u ? 34u - F : (F == 3u ? 31u : 0u);
   }
   while (alive) {
     const uint32_t cz = (uint32_t)__builtin_clz(w.k);
-    if (!FILL) {
-      if (WB)
-        vis += cz >= wb_clz ? 1u : 0u;
-      else
-        ++vis;
-    }
+    if (MODE == kStats || (MODE == kCount && WK == 1))
+      ++vis;
+    else if (MODE == kCount && WK == 2)
+      vis += cz >= wb_clz ? 1u : 0u;
     float x[DIM], lo[DIM], hi[DIM];
     uint32_t id;
     load_rec<DIM, BUF>(src, rb + w.k, x, id);
@@ -300,7 +299,7 @@ void search_kernel(SearchArgs a) {
       if (wb + j < a.cap) a.out[wb + j] = rw[j & (kRing - 1)];
   }
   if (!FILL) a.counts[q] = c;
-  if (MODE == kCount && a.work_slot) a.work_slot[slot] = vis;
+  if (MODE == kCount && WK != 0) a.work_slot[slot] = vis;
   if (MODE == kStats) a.visited[q] = vis;
 }
 

@@ -65,6 +65,34 @@ def test_tuning_keeps_results(require_gpu):
     t.close()
 
 
+def test_tuning_changed_between_passes(require_gpu):
+    """The count pass records each box's work only when the fill will order
+    boxes or blocks by it; a fill whose tuning asks for ordering after a count
+    that recorded none runs unordered, and the reverse ignores the recorded
+    work.  Either way the id sets are the oracle's (FKDTree_cpu.h:123-169)."""
+    p = adversarial_points("mixed", 30_000)
+    boxes = self_boxes(p, 0.04)
+    tc, ti = oracle.build(p.coords, p.ids)
+    ooff, oids = oracle.search(tc, ti, boxes.mins, boxes.maxs)
+    want = sorted_sets(ooff, oids)
+    t = FKDTree(coords=p.coords, ids=p.ids)
+    t.build()
+    dmi = DeviceArray.from_host(boxes.mins)
+    dma = DeviceArray.from_host(boxes.maxs)
+    q = boxes.q
+    for before, after in (((0, 0), (1, 1024)), ((1, 4096), (0, 0)), ((0, 0), (0, 256)), ((1, 0), (0, 4096))):
+        off = DeviceArray(q + 1, np.uint64)
+        t.set_tuning(fill_order=before[0], fill_window=before[1])
+        t.search_device(q, dmi.ptr, dma.ptr, off.ptr, None, 0, fill=False)
+        o = off.download()
+        assert np.array_equal(o, ooff), f"{before} -> {after}: counts differ"
+        out = DeviceArray(int(o[-1]), np.uint32)
+        t.set_tuning(fill_order=after[0], fill_window=after[1])
+        t.search_device(q, dmi.ptr, dma.ptr, off.ptr, out.ptr, int(o[-1]), count=False)
+        assert np.array_equal(sorted_sets(o, out.download()), want), f"{before} -> {after}: ids differ"
+    t.close()
+
+
 def test_tuning_rejects_bad_values(require_gpu):
     p = adversarial_points("mixed", 100)
     t = FKDTree(coords=p.coords, ids=p.ids)
