@@ -159,6 +159,10 @@ int fkd_deserialize(const void* buf, uint64_t size, int verify, fkd_stream_t str
  *                     and tests one beside every walk step; it pays for
  *                     boxes with many hits;
  *   "count_bucket"    the same choice for the count pass (default 0).
+ * The count pass records each box's work only when fill_order or
+ * fill_window, as set when it runs, will order the fill by it; a fill
+ * whose settings change after its count pass orders only by work that
+ * count recorded.
  * fin_log and build_radix_items cannot change while a graph of the handle
  * exists (FKD_ERR_STATE); the next build lays the tilings out for new values.
  * They replace the reference's fixed launch geometry
