@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--serve-events", type=int, default=512,
                     help="events of the serving stream (0 skips)")
     ap.add_argument("--serve-rates", default="1000,5000,20000", help="Poisson arrival rates (events/s)")
-    ap.add_argument("--per-box-threads", default="16,256",
+    ap.add_argument("--per-box-threads", default="1,16,256",
                     help="host thread counts of the per-box search pattern (empty or 0 skips)")
     ap.add_argument("--events-only", action="store_true",
                     help="time only the batched events (build + search per batch) and exit; for profiling")
@@ -248,9 +248,9 @@ def main():
 def bench_per_box(pts, boxes, off, d_out, thread_counts) -> list:
     """The reference driver's per-box pattern (main.cpp:351-359, :372): host
     threads call FKDTree_HIP::search_in_the_box once per box of the workload
-    (tests/cpp/fkdtree_perbox.cpp; concurrent calls are coalesced into shared
-    launches).  Every result's size is checked against the batched search's
-    offsets and the first 1,000 id sets against its ids."""
+    (tests/cpp/fkdtree_perbox.cpp; the calls go to the tree's resident search
+    kernel, fkd_box_server_*).  Every result's size is checked against the
+    batched search's offsets and the first 1,000 id sets against its ids."""
     import subprocess
     import tempfile
 

@@ -269,13 +269,22 @@ class _TreeBase:
         if os.path.exists(tune.TABLE):
             tune.apply(self)
 
+    _server = None
+
+    def _drop_server(self) -> None:
+        if self._server is not None:
+            self._server.close()
+            self._server = None
+
     def build(self) -> None:
+        self._drop_server()
         check(lib().fkd_build(self._h, _sh(self.stream)))
         check(lib().fkd_stream_sync(_sh(self.stream)))
         self._mirror = None
 
     def build_async(self) -> None:
         """Enqueue the build on the tree's stream and return."""
+        self._drop_server()
         check(lib().fkd_build(self._h, _sh(self.stream)))
         self._mirror = None
 
@@ -383,6 +392,7 @@ class _TreeBase:
         """Free the handle.  With a stream, its memory returns to the
         allocator in stream order (the handle's work must be on that stream
         or complete); without one, after a device-wide synchronisation."""
+        self._drop_server()
         if self._h:
             if self.stream is None:
                 lib().fkd_destroy(self._h)
@@ -423,7 +433,21 @@ class FKDTree(_TreeBase):
     def getIdVector(self) -> np.ndarray:
         return self.tree_arrays()[1]
 
+    def box_server(self, slots: int = 64, slot_ids: int = 4096, idle_us: int = 20000) -> "BoxServer":
+        """A resident per-box search kernel on this built tree
+        (fkd_box_server_create); building again or closing the tree stops it."""
+        return BoxServer(self, slots, slot_ids, idle_us)
+
     def search_in_the_box(self, min_point: Sequence[float], max_point: Sequence[float]) -> np.ndarray:
+        """Ids of the points in the closed box (FKDTree_cpu.h:123-169), through
+        the tree's resident per-box search kernel (started at the first call
+        after a build); a walk too wide for it or a result larger than its
+        slot goes through the batched search."""
+        if self._server is None:
+            self._server = self.box_server()
+        ids = self._server.search(min_point, max_point)
+        if ids is not None:
+            return ids
         lo = np.asarray(min_point, dtype=np.float32).reshape(self.dim, 1)
         hi = np.asarray(max_point, dtype=np.float32).reshape(self.dim, 1)
         _, ids = self._search_host(lo, hi)
@@ -441,6 +465,62 @@ class FKDTree(_TreeBase):
 
     def search_in_the_box_multiple_csr(self, mins, maxs) -> Tuple[np.ndarray, np.ndarray]:
         return self._search_host(mins, maxs)
+
+
+class BoxServer:
+    """Resident per-box search kernel of one built tree (fkd_box_server_*):
+    search() may be called from several Python threads at once (ctypes
+    releases the GIL), each call taking one request slot."""
+
+    OVERFLOW = 0xFFFFFFFF
+
+    def __init__(self, tree: "FKDTree", slots: int = 64, slot_ids: int = 4096, idle_us: int = 20000):
+        h = C.c_void_p()
+        check(lib().fkd_box_server_create(tree._h, slots, slot_ids, idle_us, C.byref(h)))
+        self._h = h.value
+        self.dim = tree.dim
+        self.slot_ids = slot_ids
+        self._tree = tree  # the tree outlives its server
+
+    def search_raw(self, min_point, max_point, cap: Optional[int] = None) -> Tuple[int, np.ndarray]:
+        """(count, ids): count is the box's hits or OVERFLOW; ids holds the
+        first min(count, cap, slot_ids) of them."""
+        box = np.concatenate([np.asarray(min_point, np.float32).ravel(),
+                              np.asarray(max_point, np.float32).ravel()])
+        assert box.size == 2 * self.dim
+        cap = self.slot_ids if cap is None else cap
+        out = np.empty(max(cap, 1), np.uint32)
+        cnt = C.c_uint32()
+        check(lib().fkd_box_server_search(self._h, box.ctypes.data, out.ctypes.data, cap, C.byref(cnt)))
+        c = cnt.value
+        return c, out[:min(c, cap, self.slot_ids)] if c != self.OVERFLOW else out[:0]
+
+    def search(self, min_point, max_point) -> Optional[np.ndarray]:
+        """The box's ids, or None when the walk was too wide for one wave or
+        the result larger than a slot."""
+        c, ids = self.search_raw(min_point, max_point)
+        if c == self.OVERFLOW or c > self.slot_ids:
+            return None
+        return ids
+
+    def stats(self) -> dict:
+        la, rq, run = C.c_uint64(), C.c_uint64(), C.c_int()
+        check(lib().fkd_box_server_stats(self._h, C.byref(la), C.byref(rq), C.byref(run)))
+        rt, wk, rp = C.c_double(), C.c_double(), C.c_uint64()
+        check(lib().fkd_box_server_timing(self._h, C.byref(rt), C.byref(wk), C.byref(rp)))
+        return {"launches": la.value, "requests": rq.value, "running": bool(run.value),
+                "round_trip_us": rt.value, "walk_us": wk.value, "reposts": rp.value}
+
+    def close(self) -> None:
+        if self._h:
+            lib().fkd_box_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class FKDEvents(_TreeBase):

@@ -41,6 +41,11 @@ _SIGS = {
     "fkd_search": (I32, [P, U32, P, P, P, P, P, U64, P]),
     "fkd_search_noalloc": (I32, [P, U32, P, P, P, P, P, U64, P]),
     "fkd_search_small": (I32, [P, U32, P, U32, P, P, P]),
+    "fkd_box_server_create": (I32, [P, U32, U32, U32, C.POINTER(P)]),
+    "fkd_box_server_search": (I32, [P, P, P, U32, C.POINTER(U32)]),
+    "fkd_box_server_stats": (I32, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(I32)]),
+    "fkd_box_server_timing": (I32, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(U64)]),
+    "fkd_box_server_destroy": (None, [P]),
     "fkd_search_status": (I32, [P, P, C.POINTER(U64), C.POINTER(I32)]),
     "fkd_trace_events": (I32, [P, P, P]),
     "fkd_graph_create": (I32, [P, U32, P, P, P, P, P, U64, P, C.POINTER(P)]),

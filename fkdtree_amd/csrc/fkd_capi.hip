@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -10,6 +12,7 @@
 #include <string>
 #include <unordered_map>
 #include <type_traits>
+#include <thread>
 #include <vector>
 
 #include "build.hpp"
@@ -446,6 +449,8 @@ struct fkd_tree_s {
   PinnedBuf small_in, small_out;         // staging of fkd_search_small
   uint32_t graph_count = 0;              // live graphs captured on this handle
   std::vector<fkd_graph_s*> graph_list;  // ... detached when the handle goes first
+  std::vector<fkd_box_server_s*> server_list;  // live box servers (they read recs): stopped and
+                                               // detached when the handle goes first
   ~fkd_tree_s();
 
   // Every device buffer of the handle.
@@ -489,12 +494,60 @@ struct fkd_graph_s {
   }
 };
 
+// A resident per-box search kernel and its request / reply slots
+// (box_server_kernel).  Host memory the kernel polls and writes is
+// page-locked, coherent and mapped; the sequence numbers in it are the only
+// synchronisation between the callers and the kernel.
+struct fkd_box_server_s {
+  fkd_tree_s* tree = nullptr;  // null once the tree is destroyed
+  int dim = 3;
+  uint32_t nslots = 0, slot_ids = 0;
+  hipStream_t stream = nullptr;
+  PinnedBuf req, rep, ids, word;  // BoxReq[nslots], BoxRep[nslots], ids[nslots][slot_ids], exit word
+  DevBuf ctl;                     // BoxCtl, then done[nslots] (last sequence number served per slot)
+  unsigned long long idle_ticks = 0;
+  uint32_t max_idle_polls = 0;
+  std::mutex launch_mu;
+  std::atomic<uint32_t> gen{0};         // generation of the last launch (0: none yet)
+  std::atomic<uint64_t> free_mask{0};   // slots no caller holds
+  std::vector<uint32_t> seq;            // last sequence number posted per slot (by its holder)
+  std::atomic<uint64_t> launches{0}, requests{0};
+  std::atomic<uint64_t> reposts{0};        // requests posted again after a torn read (echo differed)
+  std::atomic<uint64_t> round_trip_ns{0};  // summed over answered requests: post to reply, host clock
+  std::atomic<uint64_t> walk_ticks{0};     // ... and the walks, device wall-clock ticks
+  int rate_khz = 0;                        // device wall-clock rate
+  bool stopped = false;
+
+  fkd::BoxReq* h_req() const { return static_cast<fkd::BoxReq*>(req.h); }
+  fkd::BoxRep* h_rep() const { return static_cast<fkd::BoxRep*>(rep.h); }
+  uint32_t* h_exit() const { return static_cast<uint32_t*>(word.h); }
+  bool running() const {
+    return gen.load(std::memory_order_acquire) != __atomic_load_n(h_exit(), __ATOMIC_ACQUIRE);
+  }
+  // Every slot's next request is the stop value: the kernel's waves leave at
+  // their next poll; then the stream drains.
+  void stop() {
+    if (stopped) return;
+    stopped = true;
+    for (uint32_t s = 0; s < nslots; ++s) __atomic_store_n(&h_req()[s].seq, fkd::kServerStop, __ATOMIC_RELEASE);
+    if (stream) (void)hipStreamSynchronize(stream);
+  }
+  ~fkd_box_server_s() {
+    stop();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 fkd_tree_s::fkd_tree_s() {
   for (DevBuf* b : graph_held()) b->graphs = &graph_count;
 }
 
 fkd_tree_s::~fkd_tree_s() {
   for (fkd_graph_s* g : graph_list) g->tree = nullptr;
+  for (fkd_box_server_s* v : server_list) {
+    v->stop();
+    v->tree = nullptr;
+  }
 }
 
 namespace {
@@ -598,6 +651,8 @@ fkd::TileMap level_map(fkd_tree_s* t) {
 
 template <int DIM>
 int build_impl(fkd_tree_s* t, hipStream_t s) {
+  if (!t->server_list.empty())
+    return fail(FKD_ERR_STATE, "a box server reads this tree: destroy it before building again");
   const uint32_t P = t->P;
   if (P == 0) {
     t->built = true;
@@ -1729,6 +1784,201 @@ int fkd_search_small(fkd_tree_t t, uint32_t nb, const float* host_boxes, uint32_
                                                    (size_t)std::min(c, slot) * 4);
   }
   return FKD_OK;
+}
+
+namespace {
+
+// One launch of box_server_kernel as generation gen + 1 (caller holds
+// launch_mu and has seen no live launch).
+int server_launch(fkd_box_server_s* v) {
+  fkd_tree_s* t = v->tree;
+  const uint32_t g = v->gen.load(std::memory_order_relaxed) + 1u;
+  fkd::BoxCtl* ctl = v->ctl.as<fkd::BoxCtl>();
+  uint32_t* done = reinterpret_cast<uint32_t*>(ctl + 1);
+  FKD_HIP(hipMemsetAsync(&ctl->exited, 0, sizeof(uint32_t), v->stream));
+  FKD_TRY(dispatch_dim(v->dim, [&](auto D) {
+    constexpr int DIM = decltype(D)::value;
+    const uint64_t bytes = (uint64_t)t->P * fkd::NodeRec<DIM>::kFloats * 4;
+    const fkd::BoxReq* rq = static_cast<const fkd::BoxReq*>(v->req.d);
+    fkd::BoxRep* rp = static_cast<fkd::BoxRep*>(v->rep.d);
+    uint32_t* ids = static_cast<uint32_t*>(v->ids.d);
+    uint32_t* ew = static_cast<uint32_t*>(v->word.d);
+    if (bytes < (1ull << 32))
+      fkd::box_server_kernel<DIM, true><<<v->nslots, fkd::kWave, 0, v->stream>>>(
+          t->recs.as<float>(), (uint32_t)bytes, t->P, rq, rp, ids, v->slot_ids, done, ctl, ew, g, v->idle_ticks,
+          v->max_idle_polls);
+    else
+      fkd::box_server_kernel<DIM, false><<<v->nslots, fkd::kWave, 0, v->stream>>>(
+          t->recs.as<float>(), 0u, t->P, rq, rp, ids, v->slot_ids, done, ctl, ew, g, v->idle_ticks,
+          v->max_idle_polls);
+    return launch_check("box_server");
+  }));
+  v->gen.store(g, std::memory_order_release);
+  v->launches.fetch_add(1, std::memory_order_relaxed);
+  return FKD_OK;
+}
+
+// Starts a launch unless one is live.  A launch that is closing still counts
+// as live until its last wave has left; a request it did not take is served
+// by the next one, which the waiting caller starts.
+int server_ensure(fkd_box_server_s* v) {
+  if (v->running()) return FKD_OK;
+  std::lock_guard<std::mutex> g(v->launch_mu);
+  if (v->running()) return FKD_OK;
+  return server_launch(v);
+}
+
+}  // namespace
+
+int fkd_box_server_create(fkd_tree_t t, uint32_t slots, uint32_t slot_ids, uint32_t idle_us,
+                          fkd_box_server_t* out) {
+  if (!t || !out) return fail(FKD_ERR_INVALID, "tree / out is NULL");
+  *out = nullptr;
+  if (!t->built) return fail(FKD_ERR_STATE, "tree not built");
+  if (t->n_events != 1) return fail(FKD_ERR_INVALID, "a box server searches a single tree");
+  if (slots == 0 || slots > FKD_SERVER_MAX_SLOTS)
+    return fail(FKD_ERR_INVALID, "slots must be 1.." + std::to_string(FKD_SERVER_MAX_SLOTS));
+  if (slot_ids == 0 || slot_ids > FKD_SMALL_MAX_SLOT) return fail(FKD_ERR_INVALID, "slot_ids must be 1..2^20");
+  if (idle_us == 0 || idle_us > 10u * 1000u * 1000u) return fail(FKD_ERR_INVALID, "idle_us must be 1..10^7");
+  int dev = 0, rate_khz = 0;
+  FKD_HIP(hipGetDevice(&dev));
+  FKD_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+  if (rate_khz <= 0) return fail(FKD_ERR_HIP, "the device reports no wall-clock rate");
+  auto* v = new fkd_box_server_s();
+  auto bail = [&](int rc) {
+    delete v;
+    return rc;
+  };
+  v->tree = t;
+  v->dim = t->dim;
+  v->nslots = slots;
+  v->slot_ids = slot_ids;
+  v->rate_khz = rate_khz;
+  v->idle_ticks = (unsigned long long)rate_khz * idle_us / 1000ull;
+  // Checks of an idle launch come at least 16 PCIe round trips apart (about
+  // 16 us): this bound only matters if the clock were to stall.
+  v->max_idle_polls = std::max<uint32_t>(idle_us, 1000u);
+  v->seq.assign(slots, 0);
+  int rc = v->req.ensure((size_t)slots * sizeof(fkd::BoxReq));
+  if (rc == FKD_OK) rc = v->rep.ensure((size_t)slots * sizeof(fkd::BoxRep));
+  if (rc == FKD_OK) rc = v->ids.ensure((size_t)slots * slot_ids * 4);
+  if (rc == FKD_OK) rc = v->word.ensure(64);
+  if (rc == FKD_OK) rc = v->ctl.ensure(sizeof(fkd::BoxCtl) + (size_t)slots * 4);
+  if (rc != FKD_OK) return bail(rc);
+  std::memset(v->req.h, 0, v->req.bytes);
+  std::memset(v->rep.h, 0, v->rep.bytes);
+  std::memset(v->word.h, 0, v->word.bytes);
+  hipError_t e = hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemsetAsync(v->ctl.p, 0, v->ctl.bytes, v->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(v->stream);
+  if (e != hipSuccess) return bail(fail(FKD_ERR_HIP, std::string("box server set-up: ") + hipGetErrorString(e)));
+  v->free_mask.store(slots == 64 ? ~0ull : ((1ull << slots) - 1ull));
+  t->server_list.push_back(v);
+  *out = v;
+  return FKD_OK;
+}
+
+int fkd_box_server_search(fkd_box_server_t v, const float* box, uint32_t* ids_out, uint32_t cap,
+                          uint32_t* count) {
+  if (!v || !box || !count) return fail(FKD_ERR_INVALID, "server / box / count is NULL");
+  if (cap && !ids_out) return fail(FKD_ERR_INVALID, "ids_out is NULL");
+  if (!v->tree || v->stopped) return fail(FKD_ERR_STATE, "the box server's tree was destroyed");
+  // take a free slot (yielding: its holder may need this core)
+  uint32_t s = 0;
+  uint64_t m = v->free_mask.load(std::memory_order_acquire);
+  while (true) {
+    if (!m) {
+      std::this_thread::yield();
+      m = v->free_mask.load(std::memory_order_acquire);
+      continue;
+    }
+    const uint64_t bit = m & (~m + 1ull);
+    if (v->free_mask.compare_exchange_weak(m, m & ~bit, std::memory_order_acquire)) {
+      s = (uint32_t)__builtin_ctzll(bit);
+      break;
+    }
+  }
+  auto release = [&]() { v->free_mask.fetch_or(1ull << s, std::memory_order_release); };
+  const int nf = 2 * v->dim;
+  fkd::BoxReq* rq = v->h_req() + s;
+  const fkd::BoxRep* rp = v->h_rep() + s;
+  for (int j = 0; j < nf; ++j) rq->box[j] = box[j];
+  const auto t0 = std::chrono::steady_clock::now();
+  while (true) {
+    uint32_t q = v->seq[s] + 1u;
+    if (q == fkd::kServerStop) q = 0;
+    v->seq[s] = q;
+    __atomic_store_n(&rq->seq, q, __ATOMIC_RELEASE);
+    int rc = server_ensure(v);
+    if (rc != FKD_OK) {
+      release();  // a later launch may still answer it; the next holder waits for its own number
+      return rc;
+    }
+    // Spin (a sleeping caller takes tens of microseconds to wake, longer than
+    // an answer); yield once the wait is long, in case callers outnumber cores.
+    uint32_t spins = 0;
+    while (__atomic_load_n(&rp->seq, __ATOMIC_ACQUIRE) != q) {
+      ++spins;
+      if (spins < 2048u) {
+        __builtin_ia32_pause();
+        continue;
+      }
+      std::this_thread::yield();
+      if ((spins & 255u) == 0u) {
+        rc = server_ensure(v);
+        if (rc != FKD_OK) {
+          release();
+          return rc;
+        }
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(10))
+          return fail(FKD_ERR_HIP, "box server: no reply within 10 s (the slot stays taken)");
+      }
+    }
+    // The wave read the slot's line with one load, which need not be one
+    // snapshot of it: a box other than the one posted is searched again.
+    if (std::memcmp(rp->echo, rq->box, (size_t)nf * 4) == 0) break;
+    v->reposts.fetch_add(1, std::memory_order_relaxed);
+  }
+  const uint32_t c = rp->count;
+  *count = c;
+  if (c != fkd::kSmallOverflow && cap) {
+    const uint32_t k = std::min(c, std::min(cap, v->slot_ids));
+    std::memcpy(ids_out, static_cast<const uint32_t*>(v->ids.h) + (size_t)s * v->slot_ids, (size_t)k * 4);
+  }
+  v->round_trip_ns.fetch_add(
+      (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count(),
+      std::memory_order_relaxed);
+  v->walk_ticks.fetch_add(rp->walk_ticks, std::memory_order_relaxed);
+  v->requests.fetch_add(1, std::memory_order_relaxed);
+  release();
+  return FKD_OK;
+}
+
+int fkd_box_server_stats(fkd_box_server_t v, uint64_t* launches, uint64_t* requests, int* running) {
+  if (!v) return fail(FKD_ERR_INVALID, "server is NULL");
+  if (launches) *launches = v->launches.load();
+  if (requests) *requests = v->requests.load();
+  if (running) *running = v->running() ? 1 : 0;
+  return FKD_OK;
+}
+
+int fkd_box_server_timing(fkd_box_server_t v, double* round_trip_us, double* walk_us, uint64_t* reposts) {
+  if (!v) return fail(FKD_ERR_INVALID, "server is NULL");
+  const uint64_t n = v->requests.load();
+  if (round_trip_us) *round_trip_us = n ? (double)v->round_trip_ns.load() / n / 1000.0 : 0.0;
+  if (walk_us) *walk_us = n ? (double)v->walk_ticks.load() * 1000.0 / v->rate_khz / n : 0.0;
+  if (reposts) *reposts = v->reposts.load();
+  return FKD_OK;
+}
+
+void fkd_box_server_destroy(fkd_box_server_t v) {
+  if (!v) return;
+  v->stop();
+  if (v->tree) {
+    auto& l = v->tree->server_list;
+    l.erase(std::remove(l.begin(), l.end(), v), l.end());
+  }
+  delete v;
 }
 
 int fkd_search_status(fkd_tree_t t, fkd_stream_t stream, uint64_t* total, int* complete) {

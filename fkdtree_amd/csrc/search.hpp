@@ -606,79 +606,96 @@ void bucket_search_kernel(SearchArgs a) {
 // nodes of one depth of the box's frontier together, as the reference's
 // level-synchronous FQueue walk does (FKDTree_cpu.h:126-167), so the chain
 // is one load per depth.  The visited set and the reported ids are the
-// reference's (push rule FKDTree_cpu.h:150-163, closed box :145-148).  Hits
-// go to ids[b * slot + i] for i < slot; counts[b] is the true number of hits
-// (so counts[b] > slot means truncated), or kSmallOverflow when a depth's
-// frontier exceeds kWaveFrontier nodes.  Boxes are read as [nb][2 * DIM]
-// (mins, then maxs) and the outputs written straight to page-locked host
-// memory: no copies before or after the launch.
-// The records of the top kTopLevels depths (the first 2^kTopLevels - 1
-// nodes) are first copied to LDS with independent, coalesced loads, so the
-// walk's dependent global loads start below them.
+// reference's (push rule FKDTree_cpu.h:150-163, closed box :145-148).
 constexpr uint32_t kWaveFrontier = 2048;
 constexpr uint32_t kSmallOverflow = 0xffffffffu;
 constexpr uint32_t kTopLevels = 10;
 constexpr uint32_t kTopNodes = (1u << kTopLevels) - 1u;
 
-template <int DIM, bool BUF>
-__global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint32_t node_bytes, uint32_t n,
-                                                         const float* boxes, uint32_t nb, uint32_t slot,
-                                                         uint32_t* counts, uint32_t* ids) {
+// Copies the records of the top min(n, 2^TOP - 1) nodes to LDS with
+// independent, coalesced loads (one wave).
+template <int DIM, uint32_t TOP>
+__device__ inline void wave_load_top(const float* recs, uint32_t n, float* top) {
   constexpr int RF = NodeRec<DIM>::kFloats;
-  __shared__ uint32_t fr[2][kWaveFrontier];
-  __shared__ __align__(16) float top[kTopNodes * RF];
-  const uint32_t b = blockIdx.x;
-  if (b >= nb) return;
+  constexpr uint32_t kNodes = (1u << TOP) - 1u;
+  const uint32_t ntop = n < kNodes ? n : kNodes;
+  for (uint32_t i = lane_id(); i < ntop * (uint32_t)(RF / 4); i += kWave)
+    reinterpret_cast<float4*>(top)[i] = reinterpret_cast<const float4*>(recs)[i];
+}
+
+// One box walked by the calling wave (a workgroup of exactly one wave: the
+// rounds synchronise with __syncthreads).  The records of depths < TOP come
+// from `top` (wave_load_top), the rest from `src`.  Hits go to out[i] for
+// i < slot; returns the true number of hits, or kSmallOverflow when a
+// stored frontier exceeds kWaveFrontier nodes.  A NaN lower bound is read as
+// (-inf, NaN), as in search_kernel.
+//
+// Breadth first, several depths per round while the frontier is small: with
+// f <= 16 frontier nodes each takes 2^K lanes (K = 6 - ceil(log2 f)) for the
+// 2^K - 1 nodes of its subtree's top K depths, all loaded at once.  A lane's
+// node is visited when every ancestor inside the window pushed towards it
+// (the reference's push rule, tested on the ancestors' lanes through the
+// wave's ballots of left and right pushes), so the visited set and the hits
+// are exactly the depth-by-depth walk's; only the window's bottom depth
+// writes the next frontier.  Wider frontiers take one depth per round, one
+// node per lane.  A box around one point (a frontier of 1-4 nodes down to
+// the last depths) needs about a third of the dependent rounds.
+template <int DIM, bool BUF, uint32_t TOP>
+__device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top, uint32_t n, float (&mn)[DIM],
+                                         float (&mx)[DIM], uint32_t (*fr)[kWaveFrontier], uint32_t* out,
+                                         uint32_t slot) {
+  constexpr int RF = NodeRec<DIM>::kFloats;
   const uint32_t lane = lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
-  const uint32_t ntop = n < kTopNodes ? n : kTopNodes;
-  for (uint32_t i = lane; i < ntop * (uint32_t)(RF / 4); i += kWave)
-    reinterpret_cast<float4*>(top)[i] = reinterpret_cast<const float4*>(recs)[i];
-  float mn[DIM], mx[DIM];
 #pragma unroll
-  for (int m = 0; m < DIM; ++m) {
-    mn[m] = boxes[(uint64_t)b * 2 * DIM + m];
-    mx[m] = boxes[(uint64_t)b * 2 * DIM + DIM + m];
-    if (mn[m] != mn[m]) {  // a NaN lower bound: as in search_kernel
+  for (int m = 0; m < DIM; ++m)
+    if (mn[m] != mn[m]) {
       mn[m] = -__builtin_huge_valf();
       mx[m] = __builtin_nanf("");
     }
-  }
-  NodeSource src;
-  src.flat = recs;
-  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)node_bytes, 0x00020000);
   const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
   uint32_t nf = n ? 1u : 0u, cur = 0, hits = 0, depth = 0;
   bool overflow = false;
   if (lane == 0) fr[0][0] = 0;
   __syncthreads();
   while (nf) {
-    // the box rotated like this depth's records: slot j is dimension (depth + j) % DIM
-    const uint32_t dd = depth % (uint32_t)DIM;
+    // lw: log2 of the lanes per frontier node; K: depths this round
+    const uint32_t lw = nf > 16u ? 0u : 6u - (nf > 1u ? 32u - (uint32_t)__builtin_clz(nf - 1u) : 0u);
+    const uint32_t K = lw ? lw : 1u;
+    const uint32_t per = kWave >> lw;  // frontier nodes per pass
+    const uint32_t e = lane >> lw;
+    const uint32_t j = lw ? (lane & ((1u << lw) - 1u)) : 1u;  // 1-based heap index inside the window
+    const uint32_t lvl = j ? 31u - (uint32_t)__builtin_clz(j) : 0u;
+    const uint32_t ndepth = depth + lvl;
+    // the box rotated like this lane's records: slot i is dimension (ndepth + i) % DIM
+    const uint32_t dd = ndepth % (uint32_t)DIM;
     float lo[DIM], hi[DIM];
 #pragma unroll
-    for (int j = 0; j < DIM; ++j) {
-      uint32_t m = dd + (uint32_t)j;
+    for (int i = 0; i < DIM; ++i) {
+      uint32_t m = dd + (uint32_t)i;
       m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
-      lo[j] = mn[0];
-      hi[j] = mx[0];
+      lo[i] = mn[0];
+      hi[i] = mx[0];
 #pragma unroll
       for (int k = 1; k < DIM; ++k)
-        if (m == (uint32_t)k) lo[j] = mn[k], hi[j] = mx[k];
+        if (m == (uint32_t)k) lo[i] = mn[k], hi[i] = mx[k];
     }
     uint32_t nn = 0;
-    for (uint32_t base = 0; base < nf; base += kWave) {
-      const uint32_t i = base + lane;
-      const bool valid = i < nf;
-      const uint32_t node = valid ? fr[cur][i] : 0u;
+    for (uint32_t base = 0; base < nf; base += per) {
+      const bool slot_ok = j != 0u && base + e < nf;
+      const uint32_t root = slot_ok ? fr[cur][base + e] : 0u;
+      // 64-bit: a window below a deep root can pass 2^32 (such nodes are past n)
+      const uint64_t node64 = (((uint64_t)root + 1ull) << lvl) + (uint64_t)(j - (1u << lvl)) - 1ull;
+      const bool valid = slot_ok && node64 < (uint64_t)n;
+      const uint32_t node = valid ? (uint32_t)node64 : 0u;
       float x[DIM];
       uint32_t id = 0;
 #pragma unroll
-      for (int j = 0; j < DIM; ++j) x[j] = 0.f;
-      if (valid && depth < kTopLevels) {
+      for (int i = 0; i < DIM; ++i) x[i] = 0.f;
+      if (valid && ndepth < TOP) {
         const float* r = top + (uint64_t)node * RF;
 #pragma unroll
-        for (int j = 0; j < DIM; ++j) x[j] = r[j];
+        for (int i = 0; i < DIM; ++i) x[i] = r[i];
         id = __float_as_uint(r[NodeRec<DIM>::kId]);
       } else if (valid) {
         load_rec<DIM, BUF>(src, node, x, id);
@@ -686,21 +703,36 @@ __global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint
       const bool ge = x[0] >= lo[0], le = x[0] <= hi[0], lower = x[0] < lo[0];
       bool in = valid & ge & le;
 #pragma unroll
-      for (int j = 1; j < DIM; ++j) in = in & (x[j] <= hi[j]) & (x[j] >= lo[j]);
-      const uint64_t hm = __ballot(in);
-      if (in) {
-        const uint32_t pos = hits + (uint32_t)__popcll(hm & below);
-        if (pos < slot) ids[(uint64_t)b * slot + pos] = id;
-      }
-      hits += (uint32_t)__popcll(hm);
+      for (int i = 1; i < DIM; ++i) in = in & (x[i] <= hi[i]) & (x[i] >= lo[i]);
       const bool go_l = valid && ge && node < tl;
       const bool go_r = valid && (lower || le) && node < tr;
-      const uint64_t lm = __ballot(go_l), rm = __ballot(go_r);
+      const uint64_t gl = __ballot(go_l), gr = __ballot(go_r);
+      // visited: every window ancestor pushed towards this lane's node
+      bool reach = valid;
+#pragma unroll
+      for (uint32_t a = 1; a < 6u; ++a) {
+        if (a <= lvl) {
+          const uint32_t al = (e << lw) | (j >> a);
+          const uint64_t side = ((j >> (a - 1u)) & 1u) ? gr : gl;
+          reach = reach && ((side >> al) & 1ull);
+        }
+      }
+      const bool hit = reach && in;
+      const uint64_t hm = __ballot(hit);
+      if (hit) {
+        const uint32_t pos = hits + (uint32_t)__popcll(hm & below);
+        if (pos < slot) out[pos] = id;
+      }
+      hits += (uint32_t)__popcll(hm);
+      // the window's bottom depth writes the next frontier
+      const bool bottom = reach && lvl + 1u == K;
+      const bool pl = bottom && go_l, pr = bottom && go_r;
+      const uint64_t lm = __ballot(pl), rm = __ballot(pr);
       const uint32_t at = nn + (uint32_t)__popcll(lm & below) + (uint32_t)__popcll(rm & below);
       const uint32_t nxt = cur ^ 1u;
-      if (go_l && at < kWaveFrontier) fr[nxt][at] = 2u * node + 1u;
-      const uint32_t atr = at + (go_l ? 1u : 0u);
-      if (go_r && atr < kWaveFrontier) fr[nxt][atr] = 2u * node + 2u;
+      if (pl && at < kWaveFrontier) fr[nxt][at] = 2u * node + 1u;
+      const uint32_t atr = at + (pl ? 1u : 0u);
+      if (pr && atr < kWaveFrontier) fr[nxt][atr] = 2u * node + 2u;
       nn += (uint32_t)(__popcll(lm) + __popcll(rm));
     }
     __syncthreads();
@@ -710,9 +742,164 @@ __global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint
     }
     cur ^= 1u;
     nf = nn;
-    ++depth;
+    depth += K;
   }
-  if (lane == 0) counts[b] = overflow ? kSmallOverflow : hits;
+  return overflow ? kSmallOverflow : hits;
+}
+
+// fkd_search_small: box b walked by workgroup b.  Boxes are read as
+// [nb][2 * DIM] (mins, then maxs); hits go to ids[b * slot + i] for i < slot
+// and counts[b] is wave_box_walk's result.  Boxes and outputs live in
+// page-locked host memory: no copies before or after the launch.
+template <int DIM, bool BUF>
+__global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint32_t node_bytes, uint32_t n,
+                                                         const float* boxes, uint32_t nb, uint32_t slot,
+                                                         uint32_t* counts, uint32_t* ids) {
+  constexpr int RF = NodeRec<DIM>::kFloats;
+  __shared__ uint32_t fr[2][kWaveFrontier];
+  __shared__ __align__(16) float top[kTopNodes * RF];
+  const uint32_t b = blockIdx.x;
+  if (b >= nb) return;
+  wave_load_top<DIM, kTopLevels>(recs, n, top);
+  float mn[DIM], mx[DIM];
+#pragma unroll
+  for (int m = 0; m < DIM; ++m) {
+    mn[m] = boxes[(uint64_t)b * 2 * DIM + m];
+    mx[m] = boxes[(uint64_t)b * 2 * DIM + DIM + m];
+  }
+  NodeSource src;
+  src.flat = recs;
+  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)node_bytes, 0x00020000);
+  const uint32_t c = wave_box_walk<DIM, BUF, kTopLevels>(src, top, n, mn, mx, fr, ids + (uint64_t)b * slot, slot);
+  if (lane_id() == 0) counts[b] = c;
+}
+
+// Resident per-box search (fkd_box_server_*).  Each workgroup (one wave)
+// owns one request slot in page-locked host memory and polls it: the host
+// writes a box and then a new sequence number into the slot's 64-byte line;
+// the wave reads the whole line with one load, walks the box
+// (wave_box_walk), writes the ids, then the count and the box it searched
+// (the host re-posts a request whose echo differs: the line's read need not
+// be one snapshot), and publishes the sequence number after a system-scope
+// release fence.  A call therefore costs no launch and no stream
+// synchronisation, only PCIe round trips and the walk.  The top
+// ServerTop::kLevels depths stay in LDS for the kernel's lifetime.
+//
+// Every wave ends: on a stop sequence number in its slot, when any wave of
+// the launch has closed it (ctl->closing == gen), or when the whole launch
+// has seen no request for idle_ticks of the device wall clock
+// (ctl->last_act, raised by every request) or for max_idle_polls checks in
+// a row with last_act unchanged, whichever comes first; an idle wave then
+// closes the launch.  A wave whose slot has been idle for a while polls
+// less often (s_sleep between polls), so idle slots do not crowd the PCIe
+// reads of busy ones.  The last wave to leave writes gen to *exit_word
+// (host memory), which tells the host that requests posted since must be
+// served by a new launch.
+struct alignas(64) BoxReq {
+  uint32_t seq;
+  uint32_t pad0;
+  float box[8];  // mins then maxs, 2 * dim floats
+  uint32_t pad1[6];
+};
+struct alignas(64) BoxRep {
+  uint32_t seq;
+  uint32_t count;  // true hits, or kSmallOverflow
+  float echo[8];   // the box the wave searched
+  uint32_t walk_ticks;  // wall-clock ticks from reading the request to the end of the walk
+  uint32_t pad[5];
+};
+struct BoxCtl {
+  unsigned long long last_act;  // wall clock of the last request (or of a wave's start)
+  uint32_t closing;             // generation of a launch that is closing
+  uint32_t exited;              // waves of the current launch that have left
+};
+constexpr uint32_t kServerStop = 0xffffffffu;
+template <int DIM>
+struct ServerTop {
+  static constexpr uint32_t kLevels = DIM <= 3 ? 13u : 12u;  // 128 KB of records in LDS
+};
+
+__device__ inline uint32_t sys_load(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <int DIM, bool BUF>
+__global__ __launch_bounds__(kWave) void box_server_kernel(const float* recs, uint32_t node_bytes, uint32_t n,
+                                                           const BoxReq* req, BoxRep* rep, uint32_t* ids,
+                                                           uint32_t slot_ids, uint32_t* done, BoxCtl* ctl,
+                                                           uint32_t* exit_word, uint32_t gen,
+                                                           unsigned long long idle_ticks,
+                                                           uint32_t max_idle_polls) {
+  constexpr int RF = NodeRec<DIM>::kFloats;
+  constexpr uint32_t TOP = ServerTop<DIM>::kLevels;
+  __shared__ uint32_t fr[2][kWaveFrontier];
+  __shared__ __align__(16) float top[((1u << TOP) - 1u) * RF];
+  const uint32_t s = blockIdx.x;
+  const uint32_t lane = lane_id();
+  wave_load_top<DIM, TOP>(recs, n, top);
+  NodeSource src;
+  src.flat = recs;
+  if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)node_bytes, 0x00020000);
+  uint32_t handled = done[s];
+  if (lane == 0) atomicMax(&ctl->last_act, (unsigned long long)wall_clock64());
+  const uint32_t* line = reinterpret_cast<const uint32_t*>(req + s);
+  uint32_t idle = 0, checks = 0;
+  unsigned long long seen_act = 0;
+  while (true) {
+    // the slot's line: word 0 the sequence number, words 2.. the box
+    const uint32_t w = sys_load(line + (lane & 15u));
+    const uint32_t q = __builtin_amdgcn_readfirstlane(w);
+    if (q == kServerStop) break;
+    if (q != handled) {
+      const unsigned long long t0 = wall_clock64();
+      float mn[DIM], mx[DIM];
+#pragma unroll
+      for (int m = 0; m < DIM; ++m) {
+        mn[m] = __uint_as_float((uint32_t)__shfl((int)w, 2 + m));
+        mx[m] = __uint_as_float((uint32_t)__shfl((int)w, 2 + DIM + m));
+      }
+      const uint32_t c =
+          wave_box_walk<DIM, BUF, TOP>(src, top, n, mn, mx, fr, ids + (uint64_t)s * slot_ids, slot_ids);
+      const unsigned long long t1 = wall_clock64();
+      BoxRep* r = rep + s;
+      if (lane == 0) {
+        r->count = c;
+        r->walk_ticks = (uint32_t)(t1 - t0);
+      }
+      if (lane >= 2u && lane < 2u + 2u * (uint32_t)DIM) r->echo[lane - 2u] = __uint_as_float(w);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+      if (lane == 0) {
+        __hip_atomic_store(&r->seq, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        done[s] = q;
+        atomicMax(&ctl->last_act, (unsigned long long)t1);
+      }
+      handled = q;
+      idle = 0;
+      continue;
+    }
+    ++idle;
+    if ((idle & 15u) == 0u) {
+      const uint32_t cl = __hip_atomic_load(&ctl->closing, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__builtin_amdgcn_readfirstlane(cl) == gen) break;
+      const unsigned long long la = __hip_atomic_load(&ctl->last_act, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long now = wall_clock64();
+      // the poll-count bound counts checks since the launch last saw a request
+      checks = la != seen_act ? 0u : checks + 1u;
+      seen_act = la;
+      if ((now > la && now - la > idle_ticks) || checks > max_idle_polls) {
+        if (lane == 0) __hip_atomic_store(&ctl->closing, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    // a slot idle for a while polls about every 2 us instead of back to back
+    if (idle > 256u)
+      __builtin_amdgcn_s_sleep(127);
+    else
+      __builtin_amdgcn_s_sleep(2);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+  if (lane == 0 && atomicAdd(&ctl->exited, 1u) == gridDim.x - 1u)
+    __hip_atomic_store(exit_word, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Work estimate of every search block for the fill pass: the most nodes any

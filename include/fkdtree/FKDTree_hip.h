@@ -12,13 +12,14 @@
 //    search_in_the_box_multiple_csr returns every box's ids;
 //  * the order of ids within one result is unspecified (the set is exact);
 //  * calls on one object are thread safe.  Concurrent search_in_the_box calls
-//    from many host threads (main.cpp:351-359) are coalesced: one caller
-//    leads and searches every box pending at that moment in one
-//    fkd_search_small launch (one wave per box, breadth first), the others
-//    wait for their result, spinning; boxes whose result does not fit the per-box
-//    output slot (which then grows) or whose walk is too wide go through
-//    fkd_search.  Other calls are serialised by a mutex; the host copies
-//    behind getDimensionVector / getIdVector are made once per build.
+//    from many host threads (main.cpp:351-359) go to a resident search
+//    kernel (fkd_box_server_*, started at the first call after a build): each
+//    call writes its box into a free request slot in page-locked host memory
+//    and spins until the kernel's wave has written the ids back, so a call
+//    costs no launch and no stream synchronisation.  Boxes whose walk is too
+//    wide for one wave or whose result exceeds a slot go through fkd_search.
+//    Other calls are serialised by a mutex; the host copies behind
+//    getDimensionVector / getIdVector are made once per build.
 #ifndef FKDTREE_AMD_FKDTREE_HIP_H_
 #define FKDTREE_AMD_FKDTREE_HIP_H_
 
@@ -38,10 +39,10 @@
 #include "FKDTree.h"
 #include "fkd_capi.h"
 
-// Longest wait, in microseconds, of a leading search_in_the_box call for the
-// other callers to queue their boxes before it launches.
-#ifndef FKDTREE_HIP_GATHER_US
-#define FKDTREE_HIP_GATHER_US 8
+// Microseconds without a search_in_the_box call after which the resident
+// search kernel leaves the GPU (the next call starts it again).
+#ifndef FKDTREE_HIP_SERVER_IDLE_US
+#define FKDTREE_HIP_SERVER_IDLE_US 20000
 #endif
 
 template <class TYPE, unsigned int numberOfDimensions>
@@ -60,6 +61,7 @@ class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
   }
 
   ~FKDTree_HIP() override {
+    fkd_box_server_destroy(server_.exchange(nullptr));
     if (tree_) fkd_destroy(tree_);
   }
 
@@ -68,6 +70,7 @@ class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
 
   void build() override {
     std::lock_guard<std::mutex> g(mu_);
+    fkd_box_server_destroy(server_.exchange(nullptr));  // it reads the tree being rebuilt
     check(fkd_build(tree_, stream_));
     check(fkd_stream_sync(stream_));
     built_ = true;
@@ -75,54 +78,23 @@ class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
   }
 
   std::vector<unsigned int> search_in_the_box(const Point& minPoint, const Point& maxPoint) override {
-    Pending r;
+    float box[2 * numberOfDimensions];
     for (unsigned d = 0; d < numberOfDimensions; ++d) {
-      r.box[d] = minPoint[d];
-      r.box[numberOfDimensions + d] = maxPoint[d];
+      box[d] = minPoint[d];
+      box[numberOfDimensions + d] = maxPoint[d];
     }
-    {
-      std::lock_guard<std::mutex> g(q_mu_);
-      queue_.push_back(&r);
-    }
-    // Wait by spinning (a sleeping waiter takes tens of microseconds to wake,
-    // longer than a launch), leading a launch whenever none is running.
-    while (!r.done.load(std::memory_order_acquire)) {
-      if (leading_.exchange(true, std::memory_order_acquire)) {
-        std::this_thread::yield();
-        continue;
-      }
-      // Callers released by the last launch come back within microseconds:
-      // wait (at most FKDTREE_HIP_GATHER_US) until as many boxes are pending as that
-      // launch held, so launches stay full instead of alternating with
-      // nearly empty ones.
-      const size_t want = expect_;
-      const auto t0 = std::chrono::steady_clock::now();
-      while (true) {
-        {
-          std::lock_guard<std::mutex> g(q_mu_);
-          if (queue_.size() >= want) break;
-        }
-        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(FKDTREE_HIP_GATHER_US)) break;
-        std::this_thread::yield();
-      }
-      std::vector<Pending*> batch;
-      {
-        std::lock_guard<std::mutex> g(q_mu_);
-        const size_t k = std::min<size_t>(queue_.size(), FKD_SMALL_MAX_BOXES);
-        batch.assign(queue_.begin(), queue_.begin() + k);
-        queue_.erase(queue_.begin(), queue_.begin() + k);
-      }
-      expect_ = std::max<size_t>(batch.size(), 1);
-      if (!batch.empty()) {
-        run_batch(batch);
-        launches_.fetch_add(1);
-        launched_boxes_.fetch_add(batch.size());
-      }
-      for (Pending* p : batch) p->done.store(true, std::memory_order_release);
-      leading_.store(false, std::memory_order_release);
-    }
-    if (r.error) std::rethrow_exception(r.error);
-    return std::move(r.ids);
+    fkd_box_server_t srv = server();
+    thread_local std::vector<unsigned int> buf;
+    if (buf.size() < kSlotIds) buf.resize(kSlotIds);
+    uint32_t c = 0;
+    check(fkd_box_server_search(srv, box, buf.data(), kSlotIds, &c));
+    if (c != FKD_SMALL_OVERFLOW && c <= kSlotIds) return std::vector<unsigned int>(buf.begin(), buf.begin() + c);
+    // a walk too wide for one wave, or more ids than a slot holds
+    std::vector<Point> lo(1, minPoint), hi(1, maxPoint);
+    std::vector<uint64_t> off;
+    std::vector<unsigned int> ids;
+    search_csr(lo, hi, off, ids);
+    return ids;
   }
 
   std::vector<unsigned int> search_in_the_box_multiple(const std::vector<Point>& minPoints,
@@ -193,66 +165,44 @@ class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
 
   fkd_tree_t handle() const { return tree_; }
 
-  // Launches made for search_in_the_box calls, and the boxes they carried.
-  uint64_t coalesced_launches() const { return launches_.load(); }
-  uint64_t coalesced_boxes() const { return launched_boxes_.load(); }
+  // Launches of the resident search kernel and the search_in_the_box calls
+  // it answered, since the last build.
+  uint64_t server_launches() const { return server_stat(0); }
+  uint64_t server_requests() const { return server_stat(1); }
+  // Mean search_in_the_box round trip and the walk inside it (microseconds),
+  // and the requests posted again after a torn read of a request slot.
+  void server_timing(double& round_trip_us, double& walk_us, uint64_t& reposts) const {
+    round_trip_us = walk_us = 0.0;
+    reposts = 0;
+    fkd_box_server_t s = server_.load(std::memory_order_acquire);
+    if (s) check(fkd_box_server_timing(s, &round_trip_us, &walk_us, &reposts));
+  }
 
  private:
   static void check(int rc) {
     if (rc != FKD_OK) throw std::runtime_error(std::string("fkdtree: ") + fkd_last_error());
   }
 
-  struct Pending {
-    float box[2 * numberOfDimensions];
-    std::vector<unsigned int> ids;
-    std::exception_ptr error;
-    std::atomic<bool> done{false};
-  };
-
-  // One fkd_search_small launch for the pending boxes; a box whose result
-  // exceeded the slot grows the slot for later batches and, like a box whose
-  // walk was too wide, is searched again through fkd_search.
-  void run_batch(const std::vector<Pending*>& batch) {
-    const size_t nb = batch.size();
-    std::vector<float> boxes(nb * 2 * numberOfDimensions);
-    for (size_t b = 0; b < nb; ++b)
-      std::copy(batch[b]->box, batch[b]->box + 2 * numberOfDimensions, boxes.begin() + b * 2 * numberOfDimensions);
-    std::vector<uint32_t> counts(nb);
-    uint32_t slot = 0;
-    try {
-      {
-        std::lock_guard<std::mutex> g(mu_);
-        if (!built_) throw std::logic_error("fkdtree: search before build()");
-        slot = std::min<uint32_t>(slot_, static_cast<uint32_t>((1u << 28) / std::max<size_t>(nb, 1)));
-        small_ids_.resize(nb * (size_t)slot);
-        check(fkd_search_small(tree_, static_cast<uint32_t>(nb), boxes.data(), slot, counts.data(),
-                               small_ids_.data(), stream_));
-        for (size_t b = 0; b < nb; ++b) {
-          const uint32_t c = counts[b];
-          if (c == FKD_SMALL_OVERFLOW || c > slot) continue;
-          batch[b]->ids.assign(small_ids_.begin() + b * slot, small_ids_.begin() + b * slot + c);
-        }
-      }
-      for (size_t b = 0; b < nb; ++b) {
-        const uint32_t c = counts[b];
-        if (c != FKD_SMALL_OVERFLOW && c <= slot) continue;
-        if (c != FKD_SMALL_OVERFLOW) {
-          std::lock_guard<std::mutex> g(mu_);
-          uint32_t grow = slot_;
-          while (grow < c && grow < kMaxSlot) grow *= 2;
-          slot_ = std::max(slot_, grow);
-        }
-        std::vector<Point> lo(1), hi(1);
-        for (unsigned d = 0; d < numberOfDimensions; ++d) {
-          lo[0][d] = batch[b]->box[d];
-          hi[0][d] = batch[b]->box[numberOfDimensions + d];
-        }
-        std::vector<uint64_t> off;
-        search_csr(lo, hi, off, batch[b]->ids);
-      }
-    } catch (...) {
-      for (Pending* p : batch) p->error = std::current_exception();
+  // The resident search kernel's handle, created at the first
+  // search_in_the_box after a build.
+  fkd_box_server_t server() {
+    fkd_box_server_t s = server_.load(std::memory_order_acquire);
+    if (s) return s;
+    std::lock_guard<std::mutex> g(mu_);
+    if (!built_) throw std::logic_error("fkdtree: search before build()");
+    s = server_.load(std::memory_order_acquire);
+    if (!s) {
+      check(fkd_box_server_create(tree_, kSlots, kSlotIds, FKDTREE_HIP_SERVER_IDLE_US, &s));
+      server_.store(s, std::memory_order_release);
     }
+    return s;
+  }
+
+  uint64_t server_stat(int which) const {
+    fkd_box_server_t s = server_.load(std::memory_order_acquire);
+    uint64_t launches = 0, requests = 0;
+    if (s) check(fkd_box_server_stats(s, &launches, &requests, nullptr));
+    return which == 0 ? launches : requests;
   }
 
   void search_csr(const std::vector<Point>& minPoints, const std::vector<Point>& maxPoints,
@@ -285,17 +235,11 @@ class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
     mirror_ = true;
   }
 
-  static constexpr uint32_t kMaxSlot = 1u << 16;  // ids per box in one small launch
+  static constexpr uint32_t kSlots = FKD_SERVER_MAX_SLOTS;  // concurrent search_in_the_box calls served
+  static constexpr uint32_t kSlotIds = 4096;               // ids per call answered by the resident kernel
   fkd_tree_t tree_ = nullptr;
   fkd_stream_t stream_ = nullptr;
-  // coalescing of concurrent search_in_the_box calls
-  std::mutex q_mu_;
-  std::vector<Pending*> queue_;
-  std::atomic<bool> leading_{false};
-  size_t expect_ = 1;  // boxes of the last launch (read and written by the leader only)
-  std::atomic<uint64_t> launches_{0}, launched_boxes_{0};
-  uint32_t slot_ = 256;
-  std::vector<uint32_t> small_ids_;
+  std::atomic<fkd_box_server_t> server_{nullptr};
   size_t n_ = 0;
   bool built_ = false;
   bool mirror_ = false;

@@ -41,6 +41,7 @@ extern "C" {
 
 typedef struct fkd_tree_s* fkd_tree_t;
 typedef struct fkd_graph_s* fkd_graph_t;
+typedef struct fkd_box_server_s* fkd_box_server_t;
 typedef void* fkd_stream_t; /* hipStream_t */
 
 const char* fkd_last_error(void);
@@ -237,6 +238,51 @@ int fkd_search_noalloc(fkd_tree_t tree, uint32_t nq, const float* dev_mins, cons
 #define FKD_SMALL_OVERFLOW 0xffffffffu
 int fkd_search_small(fkd_tree_t tree, uint32_t nb, const float* host_boxes, uint32_t slot,
                      uint32_t* host_counts, uint32_t* host_ids, fkd_stream_t stream);
+
+/* Resident per-box search: the reference's search_in_the_box
+ * (FKDTree_cpu.h:123-169) as the reference driver calls it, one box per call
+ * from many host threads that each wait for their result (main.cpp:351-359,
+ * :372).  A kernel stays resident on its own stream and polls `slots`
+ * request slots in page-locked host memory; fkd_box_server_search writes its
+ * box into a free slot and spins until the reply is there, so a call costs
+ * no launch and no stream synchronisation, only PCIe round trips and a
+ * breadth-first walk of one box by one wave (the top 12 depths, 11 at dim
+ * 4, read from LDS).  The kernel leaves after `idle_us` microseconds without
+ * a request (1..10^7); the next call starts it again.
+ *   fkd_box_server_create   single built trees only; 1..FKD_SERVER_MAX_SLOTS
+ *                           slots of `slot_ids` ids (1..2^20) each.  While a
+ *                           server exists fkd_build on its tree fails
+ *                           (FKD_ERR_STATE); destroying the tree stops the
+ *                           server, whose searches then fail.
+ *   fkd_box_server_search   box = 2 * dim floats, mins then maxs.  *count is
+ *                           the box's number of hits, or FKD_SMALL_OVERFLOW
+ *                           when one depth of its walk holds more than 2,048
+ *                           nodes (search it with fkd_search); the first
+ *                           min(count, cap, slot_ids) ids, in unspecified
+ *                           order, go to ids_out: a count above min(cap,
+ *                           slot_ids) means the ids were cut off.  Thread
+ *                           safe: calls from several threads run together,
+ *                           one slot each (more callers than slots wait for
+ *                           one).  Must not overlap fkd_box_server_destroy.
+ *   fkd_box_server_stats    launches made, requests answered, and whether a
+ *                           launch is live.
+ *   fkd_box_server_timing   mean time of an answered call from posting its
+ *                           box to reading the reply (host clock), the mean
+ *                           walk inside it (device clock), and the requests
+ *                           posted again because the wave read a box other
+ *                           than the one posted (its one load of the request
+ *                           line need not be a snapshot; the reply echoes
+ *                           the box it searched).
+ *   fkd_box_server_destroy  stops the kernel (waits for it to leave) and
+ *                           frees the slots. */
+#define FKD_SERVER_MAX_SLOTS 64u
+int fkd_box_server_create(fkd_tree_t tree, uint32_t slots, uint32_t slot_ids, uint32_t idle_us,
+                          fkd_box_server_t* out);
+int fkd_box_server_search(fkd_box_server_t server, const float* box, uint32_t* ids_out, uint32_t cap,
+                          uint32_t* count);
+int fkd_box_server_stats(fkd_box_server_t server, uint64_t* launches, uint64_t* requests, int* running);
+int fkd_box_server_timing(fkd_box_server_t server, double* round_trip_us, double* walk_us, uint64_t* reposts);
+void fkd_box_server_destroy(fkd_box_server_t server);
 
 /* Boxes [c - half_width, c + half_width] around every point of the handle,
  * in the order the points were given (SoA [dim][n_points], one float

@@ -61,10 +61,10 @@ int main(int argc, char** argv) {
   FKDTree_HIP<float, 3> tree(points);
   tree.build();
   FKDTree<float, 3>& base = tree;
-  // warm-up: the first calls size the per-box output slot and staging
+  // warm-up: the first call starts the resident search kernel
   for (uint32_t i = 0; i < std::min<uint32_t>(q, 64); ++i) base.search_in_the_box(mins[i], maxs[i]);
 
-  const uint64_t l0 = tree.coalesced_launches(), b0 = tree.coalesced_boxes();
+  const uint64_t l0 = tree.server_launches(), r0 = tree.server_requests();
   std::vector<uint32_t> got(q, 0);
   std::vector<std::vector<unsigned int>> sample(k);
   std::atomic<uint32_t> next(0);
@@ -80,7 +80,10 @@ int main(int argc, char** argv) {
     });
   for (auto& th : pool) th.join();
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  const uint64_t launches = tree.coalesced_launches() - l0, launched = tree.coalesced_boxes() - b0;
+  const uint64_t launches = tree.server_launches() - l0, answered = tree.server_requests() - r0;
+  double round_trip_us = 0, walk_us = 0;
+  uint64_t reposts = 0;
+  tree.server_timing(round_trip_us, walk_us, reposts);
 
   uint64_t count_bad = 0, set_bad = 0, hits = 0;
   for (uint32_t i = 0; i < q; ++i) {
@@ -97,8 +100,10 @@ int main(int argc, char** argv) {
   std::printf(
       "{\"threads\": %d, \"boxes\": %u, \"points\": %u, \"seconds\": %.6f, \"box_queries_per_s\": %.1f, "
       "\"hits\": %llu, \"count_mismatches\": %llu, \"checked_sets\": %u, \"set_mismatches\": %llu, "
-      "\"launches\": %llu, \"mean_boxes_per_launch\": %.2f}\n",
+      "\"server_launches\": %llu, \"server_answered\": %llu, \"mean_round_trip_us\": %.2f, "
+      "\"mean_walk_us\": %.2f, \"reposts\": %llu}\n",
       threads, q, n, secs, q / secs, (unsigned long long)hits, (unsigned long long)count_bad, k,
-      (unsigned long long)set_bad, (unsigned long long)launches, launches ? (double)launched / launches : 0.0);
+      (unsigned long long)set_bad, (unsigned long long)launches, (unsigned long long)answered, round_trip_us, walk_us,
+      (unsigned long long)reposts);
   return (count_bad || set_bad) ? 1 : 0;
 }

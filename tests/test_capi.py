@@ -46,6 +46,13 @@ def test_error_paths_without_gpu():
     assert b"2^30" in L.fkd_last_error()
     t, st, sz = C.c_uint32(), C.c_uint32(), C.c_uint32()
     assert L.fkd_shape_locate(10, 5, 0, C.byref(t)) == -1
+    srv = C.c_void_p()
+    assert L.fkd_box_server_create(None, 64, 4096, 20000, C.byref(srv)) == -1
+    box = (C.c_float * 6)()
+    cnt = C.c_uint32()
+    assert L.fkd_box_server_search(None, box, None, 0, C.byref(cnt)) == -1
+    assert L.fkd_box_server_stats(None, None, None, None) == -1
+    L.fkd_box_server_destroy(None)
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
