@@ -56,6 +56,9 @@ def parse():
                     help="host thread counts of the per-box search pattern (empty or 0 skips)")
     ap.add_argument("--events-only", action="store_true",
                     help="time only the batched events (build + search per batch) and exit; for profiling")
+    ap.add_argument("--per-box-input", default="",
+                    help="write the per-box driver's input (tests/cpp/fkdtree_perbox.cpp) for the workload "
+                         "to this path and exit; for profiling the driver")
     ap.add_argument("--calibrate", action="store_true",
                     help="run a plain copy kernel (counter calibration) three times and exit")
     ap.add_argument("--calib-mib", type=int, default=2048,
@@ -159,6 +162,10 @@ def main():
 
     search_ms, wall_ms, fill_ms = timed(two_pass, True)
     progress(f"search {search_ms:.3f} ms (fill {fill_ms:.3f} ms)")
+    if args.per_box_input:
+        write_per_box_input(args.per_box_input, pts, boxes, d_off.download(stream), d_out)
+        progress(f"per-box driver input written to {args.per_box_input}")
+        return 0
     per_box = None
     threads = [int(x) for x in str(args.per_box_threads).split(",") if x.strip() and int(x) > 0]
     if threads:
@@ -245,6 +252,24 @@ def main():
     return 0
 
 
+def write_per_box_input(path, pts, boxes, off, d_out, k: int = 1000) -> None:
+    """Input of tests/cpp/fkdtree_perbox.cpp: points, boxes, every box's
+    expected count and the first k boxes' sorted ids (from the batched
+    search's offsets `off` and ids `d_out`)."""
+    ids = d_out.download(count=int(off[k]))
+    sorted_ids = np.concatenate([np.sort(ids[int(off[i]):int(off[i + 1])]) for i in range(k)])
+    with open(path, "wb") as f:
+        f.write(np.array([pts.n, boxes.q], np.uint32).tobytes())
+        f.write(np.ascontiguousarray(pts.coords.T).tobytes())
+        f.write(pts.ids.tobytes())
+        f.write(np.ascontiguousarray(boxes.mins.T).tobytes())
+        f.write(np.ascontiguousarray(boxes.maxs.T).tobytes())
+        f.write(np.diff(off).astype(np.uint32).tobytes())
+        f.write(np.array([k], np.uint32).tobytes())
+        f.write(np.ascontiguousarray(off[:k + 1], dtype=np.uint64).tobytes())
+        f.write(sorted_ids.astype(np.uint32).tobytes())
+
+
 def bench_per_box(pts, boxes, off, d_out, thread_counts) -> list:
     """The reference driver's per-box pattern (main.cpp:351-359, :372): host
     threads call FKDTree_HIP::search_in_the_box once per box of the workload
@@ -257,21 +282,9 @@ def bench_per_box(pts, boxes, off, d_out, thread_counts) -> list:
     from fkdtree_amd import build_lib
     if not os.path.exists(build_lib.CPP_PERBOX_BIN):
         return [{"skipped": "tests/cpp/fkdtree_perbox.bin not built"}]
-    k = 1000
-    ids = d_out.download(count=int(off[k]))
-    sorted_ids = np.concatenate([np.sort(ids[int(off[i]):int(off[i + 1])]) for i in range(k)])
     with tempfile.TemporaryDirectory() as d:
         fin = os.path.join(d, "in.bin")
-        with open(fin, "wb") as f:
-            f.write(np.array([pts.n, boxes.q], np.uint32).tobytes())
-            f.write(np.ascontiguousarray(pts.coords.T).tobytes())
-            f.write(pts.ids.tobytes())
-            f.write(np.ascontiguousarray(boxes.mins.T).tobytes())
-            f.write(np.ascontiguousarray(boxes.maxs.T).tobytes())
-            f.write(np.diff(off).astype(np.uint32).tobytes())
-            f.write(np.array([k], np.uint32).tobytes())
-            f.write(np.ascontiguousarray(off[:k + 1], dtype=np.uint64).tobytes())
-            f.write(sorted_ids.astype(np.uint32).tobytes())
+        write_per_box_input(fin, pts, boxes, off, d_out)
         out = []
         for threads in thread_counts:
             r = subprocess.run([build_lib.CPP_PERBOX_BIN, fin, str(threads)], capture_output=True, text=True,
