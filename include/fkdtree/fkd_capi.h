@@ -248,7 +248,9 @@ int fkd_search_small(fkd_tree_t tree, uint32_t nb, const float* host_boxes, uint
  * no launch and no stream synchronisation, only PCIe round trips and a
  * breadth-first walk of one box by one wave (the top 12 depths, 11 at dim
  * 4, read from LDS).  The kernel leaves after `idle_us` microseconds without
- * a request (1..10^7); the next call starts it again.
+ * a request (1..10^7); the next call starts it again.  While it is live it
+ * holds one compute unit per slot (a one-wave workgroup with 144 KB of LDS),
+ * which other kernels on the GPU cannot use meanwhile.
  *   fkd_box_server_create   single built trees only; 1..FKD_SERVER_MAX_SLOTS
  *                           slots of `slot_ids` ids (1..2^20) each.  While a
  *                           server exists fkd_build on its tree fails
