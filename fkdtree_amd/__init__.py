@@ -433,9 +433,13 @@ class FKDTree(_TreeBase):
     def getIdVector(self) -> np.ndarray:
         return self.tree_arrays()[1]
 
-    def box_server(self, slots: int = 64, slot_ids: int = 4096, idle_us: int = 20000) -> "BoxServer":
+    def box_server(self, slots: Optional[int] = None, slot_ids: int = 4096, idle_us: int = 20000) -> "BoxServer":
         """A resident per-box search kernel on this built tree
-        (fkd_box_server_create); building again or closing the tree stops it."""
+        (fkd_box_server_create); building again or closing the tree stops it.
+        slots defaults to the host's hardware threads, 16 to 64 (each holds a
+        compute unit while the kernel is live)."""
+        if slots is None:
+            slots = min(64, max(16, os.cpu_count() or 16))
         return BoxServer(self, slots, slot_ids, idle_us)
 
     def search_in_the_box(self, min_point: Sequence[float], max_point: Sequence[float]) -> np.ndarray:
@@ -474,7 +478,7 @@ class BoxServer:
 
     OVERFLOW = 0xFFFFFFFF
 
-    def __init__(self, tree: "FKDTree", slots: int = 64, slot_ids: int = 4096, idle_us: int = 20000):
+    def __init__(self, tree: "FKDTree", slots: int = 16, slot_ids: int = 4096, idle_us: int = 20000):
         h = C.c_void_p()
         check(lib().fkd_box_server_create(tree._h, slots, slot_ids, idle_us, C.byref(h)))
         self._h = h.value

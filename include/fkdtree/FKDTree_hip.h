@@ -192,7 +192,7 @@ class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
     if (!built_) throw std::logic_error("fkdtree: search before build()");
     s = server_.load(std::memory_order_acquire);
     if (!s) {
-      check(fkd_box_server_create(tree_, kSlots, kSlotIds, FKDTREE_HIP_SERVER_IDLE_US, &s));
+      check(fkd_box_server_create(tree_, server_slots(), kSlotIds, FKDTREE_HIP_SERVER_IDLE_US, &s));
       server_.store(s, std::memory_order_release);
     }
     return s;
@@ -235,7 +235,13 @@ class FKDTree_HIP : public FKDTree<TYPE, numberOfDimensions> {
     mirror_ = true;
   }
 
-  static constexpr uint32_t kSlots = FKD_SERVER_MAX_SLOTS;  // concurrent search_in_the_box calls served
+  // Concurrent search_in_the_box calls served: one per host hardware thread
+  // (a waiting call spins on its core), 16 to 64; each slot holds a compute
+  // unit while the resident kernel is live.
+  static uint32_t server_slots() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::min<uint32_t>(FKD_SERVER_MAX_SLOTS, std::max<uint32_t>(16u, hw));
+  }
   static constexpr uint32_t kSlotIds = 4096;               // ids per call answered by the resident kernel
   fkd_tree_t tree_ = nullptr;
   fkd_stream_t stream_ = nullptr;
