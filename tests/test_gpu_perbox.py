@@ -267,3 +267,59 @@ def test_box_server_lifecycle(require_gpu):
     e.build()
     assert lib().fkd_box_server_create(e._h, 4, 64, 1000, C.byref(h)) == -1
     e.close()
+
+
+def test_box_server_beside_batched_work(require_gpu):
+    """Per-box calls from four threads on one tree while another tree is built
+    and searched in batches on its own stream: the resident kernel's compute
+    units and the build's look-back launches share the GPU, and every result
+    of both is the oracle's."""
+    import threading
+
+    from fkdtree_amd import Stream
+
+    pa, ba = gen.workload_mixed_1m(n=100_000, q=2_000)
+    pb, bb = gen.workload_mixed_1m(n=300_000, q=50_000)
+    oca, oia = oracle.build(pa.coords, pa.ids)
+    offa, oidsa = oracle.search(oca, oia, ba.mins, ba.maxs)
+    ocb, oib = oracle.build(pb.coords, pb.ids)
+    offb, oidsb = oracle.search(ocb, oib, bb.mins, bb.maxs)
+    ta = FKDTree(coords=pa.coords, ids=pa.ids)
+    ta.build()
+    srv = ta.box_server(slots=4)
+    st = Stream()
+    tb = FKDTree(coords=pb.coords, ids=pb.ids, stream=st)
+    got = [None] * ba.q
+    errors = []
+
+    def work(w):
+        try:
+            for q in range(w, ba.q, 4):
+                got[q] = srv.search(ba.mins[:, q], ba.maxs[:, q])
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    print("[server] beside batched builds and searches", flush=True)
+    th = [threading.Thread(target=work, args=(w,)) for w in range(4)]
+    for x in th:
+        x.start()
+    rounds = 0
+    while rounds < 2 or (any(x.is_alive() for x in th) and rounds < 20):
+        tb.build()
+        c, i = tb.tree_arrays()
+        assert np.array_equal(i, oib) and np.array_equal(c.view(np.uint32), ocb.view(np.uint32)), rounds
+        off, ids = tb.search_in_the_box_multiple_csr(bb.mins, bb.maxs)
+        assert np.array_equal(off, offb), rounds
+        rounds += 1
+    for x in th:
+        x.join(120)
+    assert not errors, errors[:3]
+    for q in range(ba.q):
+        assert got[q] is not None, q
+        assert np.array_equal(np.sort(got[q]), np.sort(oidsa[int(offa[q]):int(offa[q + 1])])), q
+    assert np.array_equal(sorted_sets(off, ids), sorted_sets(offb, oidsb))
+    print(f"[server] {rounds} batched rounds beside {ba.q} per-box calls", flush=True)
+    srv.close()
+    ta.close()
+    tb.close()
+    st.close()
