@@ -623,6 +623,30 @@ __device__ inline void wave_load_top(const float* recs, uint32_t n, float* top) 
     reinterpret_cast<float4*>(top)[i] = reinterpret_cast<const float4*>(recs)[i];
 }
 
+// Window ancestors of every lane, for each window size lw = 1..6 (64 >> lw
+// frontier nodes of 2^lw lanes each): the lanes holding the ancestors from
+// which the lane's node is a left son (x, y: low and high words) and a right
+// son (z, w).  Filled once per workgroup; lw = 0 has no ancestors.
+constexpr uint32_t kAncSizes = 7;
+__device__ inline void wave_fill_ancestors(uint4* anc) {
+  const uint32_t lane = lane_id();
+  for (uint32_t lw = 0; lw < kAncSizes; ++lw) {
+    uint64_t lm = 0, rm = 0;
+    if (lw) {
+      const uint32_t e = lane >> lw, j = lane & ((1u << lw) - 1u);
+      const uint32_t lvl = j ? 31u - (uint32_t)__builtin_clz(j) : 0u;
+      for (uint32_t a = 1; a <= lvl; ++a) {
+        const uint64_t bit = 1ull << ((e << lw) | (j >> a));
+        if ((j >> (a - 1u)) & 1u)
+          rm |= bit;
+        else
+          lm |= bit;
+      }
+    }
+    anc[lw * kWave + lane] = make_uint4((uint32_t)lm, (uint32_t)(lm >> 32), (uint32_t)rm, (uint32_t)(rm >> 32));
+  }
+}
+
 // One box walked by the calling wave (a workgroup of exactly one wave: the
 // rounds synchronise with __syncthreads).  The records of depths < TOP come
 // from `top` (wave_load_top), the rest from `src`.  Hits go to out[i] for
@@ -639,11 +663,13 @@ __device__ inline void wave_load_top(const float* recs, uint32_t n, float* top) 
 // are exactly the depth-by-depth walk's; only the window's bottom depth
 // writes the next frontier.  Wider frontiers take one depth per round, one
 // node per lane.  A box around one point (a frontier of 1-4 nodes down to
-// the last depths) needs about a third of the dependent rounds.
+// the last depths) needs about a third of the dependent rounds.  `anc` is
+// the wave_fill_ancestors table: a lane's ancestor test is two masked
+// compares of the push ballots.
 template <int DIM, bool BUF, uint32_t TOP>
 __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top, uint32_t n, float (&mn)[DIM],
-                                         float (&mx)[DIM], uint32_t (*fr)[kWaveFrontier], uint32_t* out,
-                                         uint32_t slot) {
+                                         float (&mx)[DIM], uint32_t (*fr)[kWaveFrontier], const uint4* anc,
+                                         uint32_t* out, uint32_t slot) {
   constexpr int RF = NodeRec<DIM>::kFloats;
   const uint32_t lane = lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
@@ -667,6 +693,8 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
     const uint32_t j = lw ? (lane & ((1u << lw) - 1u)) : 1u;  // 1-based heap index inside the window
     const uint32_t lvl = j ? 31u - (uint32_t)__builtin_clz(j) : 0u;
     const uint32_t ndepth = depth + lvl;
+    const uint4 am = anc[lw * kWave + lane];
+    const uint64_t anc_l = (uint64_t)am.x | ((uint64_t)am.y << 32), anc_r = (uint64_t)am.z | ((uint64_t)am.w << 32);
     // the box rotated like this lane's records: slot i is dimension (ndepth + i) % DIM
     const uint32_t dd = ndepth % (uint32_t)DIM;
     float lo[DIM], hi[DIM];
@@ -708,15 +736,7 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
       const bool go_r = valid && (lower || le) && node < tr;
       const uint64_t gl = __ballot(go_l), gr = __ballot(go_r);
       // visited: every window ancestor pushed towards this lane's node
-      bool reach = valid;
-#pragma unroll
-      for (uint32_t a = 1; a < 6u; ++a) {
-        if (a <= lvl) {
-          const uint32_t al = (e << lw) | (j >> a);
-          const uint64_t side = ((j >> (a - 1u)) & 1u) ? gr : gl;
-          reach = reach && ((side >> al) & 1ull);
-        }
-      }
+      const bool reach = valid && (gl & anc_l) == anc_l && (gr & anc_r) == anc_r;
       const bool hit = reach && in;
       const uint64_t hm = __ballot(hit);
       if (hit) {
@@ -757,10 +777,12 @@ __global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint
                                                          uint32_t* counts, uint32_t* ids) {
   constexpr int RF = NodeRec<DIM>::kFloats;
   __shared__ uint32_t fr[2][kWaveFrontier];
+  __shared__ uint4 anc[kAncSizes * kWave];
   __shared__ __align__(16) float top[kTopNodes * RF];
   const uint32_t b = blockIdx.x;
   if (b >= nb) return;
   wave_load_top<DIM, kTopLevels>(recs, n, top);
+  wave_fill_ancestors(anc);
   float mn[DIM], mx[DIM];
 #pragma unroll
   for (int m = 0; m < DIM; ++m) {
@@ -770,7 +792,8 @@ __global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint
   NodeSource src;
   src.flat = recs;
   if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)node_bytes, 0x00020000);
-  const uint32_t c = wave_box_walk<DIM, BUF, kTopLevels>(src, top, n, mn, mx, fr, ids + (uint64_t)b * slot, slot);
+  const uint32_t c =
+      wave_box_walk<DIM, BUF, kTopLevels>(src, top, n, mn, mx, fr, anc, ids + (uint64_t)b * slot, slot);
   if (lane_id() == 0) counts[b] = c;
 }
 
@@ -833,10 +856,12 @@ __global__ __launch_bounds__(kWave) void box_server_kernel(const float* recs, ui
   constexpr int RF = NodeRec<DIM>::kFloats;
   constexpr uint32_t TOP = ServerTop<DIM>::kLevels;
   __shared__ uint32_t fr[2][kWaveFrontier];
+  __shared__ uint4 anc[kAncSizes * kWave];
   __shared__ __align__(16) float top[((1u << TOP) - 1u) * RF];
   const uint32_t s = blockIdx.x;
   const uint32_t lane = lane_id();
   wave_load_top<DIM, TOP>(recs, n, top);
+  wave_fill_ancestors(anc);
   NodeSource src;
   src.flat = recs;
   if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)node_bytes, 0x00020000);
@@ -859,7 +884,7 @@ __global__ __launch_bounds__(kWave) void box_server_kernel(const float* recs, ui
         mx[m] = __uint_as_float((uint32_t)__shfl((int)w, 2 + DIM + m));
       }
       const uint32_t c =
-          wave_box_walk<DIM, BUF, TOP>(src, top, n, mn, mx, fr, ids + (uint64_t)s * slot_ids, slot_ids);
+          wave_box_walk<DIM, BUF, TOP>(src, top, n, mn, mx, fr, anc, ids + (uint64_t)s * slot_ids, slot_ids);
       const unsigned long long t1 = wall_clock64();
       BoxRep* r = rep + s;
       if (lane == 0) {
