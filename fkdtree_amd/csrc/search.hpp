@@ -669,7 +669,7 @@ __device__ inline void wave_fill_ancestors(uint4* anc) {
 template <int DIM, bool BUF, uint32_t TOP>
 __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top, uint32_t n, float (&mn)[DIM],
                                          float (&mx)[DIM], uint32_t (*fr)[kWaveFrontier], const uint4* anc,
-                                         uint32_t* out, uint32_t slot) {
+                                         float2* rbox, uint32_t* out, uint32_t slot) {
   constexpr int RF = NodeRec<DIM>::kFloats;
   const uint32_t lane = lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
@@ -679,6 +679,17 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
       mn[m] = -__builtin_huge_valf();
       mx[m] = __builtin_nanf("");
     }
+  // the box rotated like the records of a depth d with d % DIM = r:
+  // rbox[r * DIM + i] = (min, max) of dimension (r + i) % DIM
+  if (lane < (uint32_t)(DIM * DIM)) {
+    const uint32_t r = lane / (uint32_t)DIM, i = lane % (uint32_t)DIM;
+    const uint32_t m = (r + i) % (uint32_t)DIM;
+    float lo = mn[0], hi = mx[0];
+#pragma unroll
+    for (int k = 1; k < DIM; ++k)
+      if (m == (uint32_t)k) lo = mn[k], hi = mx[k];
+    rbox[lane] = make_float2(lo, hi);
+  }
   const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
   uint32_t nf = n ? 1u : 0u, cur = 0, hits = 0, depth = 0;
   bool overflow = false;
@@ -696,17 +707,13 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
     const uint4 am = anc[lw * kWave + lane];
     const uint64_t anc_l = (uint64_t)am.x | ((uint64_t)am.y << 32), anc_r = (uint64_t)am.z | ((uint64_t)am.w << 32);
     // the box rotated like this lane's records: slot i is dimension (ndepth + i) % DIM
-    const uint32_t dd = ndepth % (uint32_t)DIM;
+    const float2* rb = rbox + (ndepth % (uint32_t)DIM) * (uint32_t)DIM;
     float lo[DIM], hi[DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-      uint32_t m = dd + (uint32_t)i;
-      m = m >= (uint32_t)DIM ? m - (uint32_t)DIM : m;
-      lo[i] = mn[0];
-      hi[i] = mx[0];
-#pragma unroll
-      for (int k = 1; k < DIM; ++k)
-        if (m == (uint32_t)k) lo[i] = mn[k], hi[i] = mx[k];
+      const float2 t = rb[i];
+      lo[i] = t.x;
+      hi[i] = t.y;
     }
     uint32_t nn = 0;
     for (uint32_t base = 0; base < nf; base += per) {
@@ -778,6 +785,7 @@ __global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint
   constexpr int RF = NodeRec<DIM>::kFloats;
   __shared__ uint32_t fr[2][kWaveFrontier];
   __shared__ uint4 anc[kAncSizes * kWave];
+  __shared__ float2 rbox[DIM * DIM];
   __shared__ __align__(16) float top[kTopNodes * RF];
   const uint32_t b = blockIdx.x;
   if (b >= nb) return;
@@ -793,7 +801,7 @@ __global__ __launch_bounds__(kWave) void box_wave_kernel(const float* recs, uint
   src.flat = recs;
   if (BUF) src.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)recs, (short)0, (int)node_bytes, 0x00020000);
   const uint32_t c =
-      wave_box_walk<DIM, BUF, kTopLevels>(src, top, n, mn, mx, fr, anc, ids + (uint64_t)b * slot, slot);
+      wave_box_walk<DIM, BUF, kTopLevels>(src, top, n, mn, mx, fr, anc, rbox, ids + (uint64_t)b * slot, slot);
   if (lane_id() == 0) counts[b] = c;
 }
 
@@ -857,6 +865,7 @@ __global__ __launch_bounds__(kWave) void box_server_kernel(const float* recs, ui
   constexpr uint32_t TOP = ServerTop<DIM>::kLevels;
   __shared__ uint32_t fr[2][kWaveFrontier];
   __shared__ uint4 anc[kAncSizes * kWave];
+  __shared__ float2 rbox[DIM * DIM];
   __shared__ __align__(16) float top[((1u << TOP) - 1u) * RF];
   const uint32_t s = blockIdx.x;
   const uint32_t lane = lane_id();
@@ -883,8 +892,8 @@ __global__ __launch_bounds__(kWave) void box_server_kernel(const float* recs, ui
         mn[m] = __uint_as_float((uint32_t)__shfl((int)w, 2 + m));
         mx[m] = __uint_as_float((uint32_t)__shfl((int)w, 2 + DIM + m));
       }
-      const uint32_t c =
-          wave_box_walk<DIM, BUF, TOP>(src, top, n, mn, mx, fr, anc, ids + (uint64_t)s * slot_ids, slot_ids);
+      const uint32_t c = wave_box_walk<DIM, BUF, TOP>(src, top, n, mn, mx, fr, anc, rbox,
+                                                      ids + (uint64_t)s * slot_ids, slot_ids);
       const unsigned long long t1 = wall_clock64();
       BoxRep* r = rep + s;
       if (lane == 0) {
