@@ -681,14 +681,13 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
     }
   // the box rotated like the records of a depth d with d % DIM = r:
   // rbox[r * DIM + i] = (min, max) of dimension (r + i) % DIM
-  if (lane < (uint32_t)(DIM * DIM)) {
-    const uint32_t r = lane / (uint32_t)DIM, i = lane % (uint32_t)DIM;
-    const uint32_t m = (r + i) % (uint32_t)DIM;
-    float lo = mn[0], hi = mx[0];
+  // (compile-time indices only: a per-lane index into mn / mx would put
+  // them in private memory)
+  if (lane == 0) {
 #pragma unroll
-    for (int k = 1; k < DIM; ++k)
-      if (m == (uint32_t)k) lo = mn[k], hi = mx[k];
-    rbox[lane] = make_float2(lo, hi);
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) rbox[r * DIM + i] = make_float2(mn[(r + i) % DIM], mx[(r + i) % DIM]);
   }
   const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
   uint32_t nf = n ? 1u : 0u, cur = 0, hits = 0, depth = 0;
