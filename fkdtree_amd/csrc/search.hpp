@@ -623,6 +623,11 @@ __device__ inline void wave_load_top(const float* recs, uint32_t n, float* top) 
     reinterpret_cast<float4*>(top)[i] = reinterpret_cast<const float4*>(recs)[i];
 }
 
+// Set bits of a wave mask in the lanes below the calling lane.
+__device__ inline uint32_t below_count(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Window ancestors of every lane, for each window size lw = 1..6 (64 >> lw
 // frontier nodes of 2^lw lanes each): the lanes holding the ancestors from
 // which the lane's node is a left son (x, y: low and high words) and a right
@@ -672,7 +677,6 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
                                          float2* rbox, uint32_t* out, uint32_t slot) {
   constexpr int RF = NodeRec<DIM>::kFloats;
   const uint32_t lane = lane_id();
-  const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
   for (int m = 0; m < DIM; ++m)
     if (mn[m] != mn[m]) {
@@ -690,6 +694,7 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
       for (int i = 0; i < DIM; ++i) rbox[r * DIM + i] = make_float2(mn[(r + i) % DIM], mx[(r + i) % DIM]);
   }
   const uint32_t tl = n >> 1, tr = (n - 1u) >> 1;
+  const bool narrow = n <= (1u << 26);
   uint32_t nf = n ? 1u : 0u, cur = 0, hits = 0, depth = 0;
   bool overflow = false;
   if (lane == 0) fr[0][0] = 0;
@@ -718,10 +723,19 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
     for (uint32_t base = 0; base < nf; base += per) {
       const bool slot_ok = j != 0u && base + e < nf;
       const uint32_t root = slot_ok ? fr[cur][base + e] : 0u;
-      // 64-bit: a window below a deep root can pass 2^32 (such nodes are past n)
-      const uint64_t node64 = (((uint64_t)root + 1ull) << lvl) + (uint64_t)(j - (1u << lvl)) - 1ull;
-      const bool valid = slot_ok && node64 < (uint64_t)n;
-      const uint32_t node = valid ? (uint32_t)node64 : 0u;
+      // a window node below a deep root can pass 2^32 (such nodes are past
+      // n); with n <= 2^26 none can (roots < 2^26, windows of 5 depths below)
+      bool valid;
+      uint32_t node;
+      if (narrow) {
+        node = ((root + 1u) << lvl) + j - (1u << lvl) - 1u;
+        valid = slot_ok && node < n;
+      } else {
+        const uint64_t node64 = (((uint64_t)root + 1ull) << lvl) + (uint64_t)(j - (1u << lvl)) - 1ull;
+        valid = slot_ok && node64 < (uint64_t)n;
+        node = (uint32_t)node64;
+      }
+      node = valid ? node : 0u;
       float x[DIM];
       uint32_t id = 0;
 #pragma unroll
@@ -746,7 +760,7 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
       const bool hit = reach && in;
       const uint64_t hm = __ballot(hit);
       if (hit) {
-        const uint32_t pos = hits + (uint32_t)__popcll(hm & below);
+        const uint32_t pos = hits + below_count(hm);
         if (pos < slot) out[pos] = id;
       }
       hits += (uint32_t)__popcll(hm);
@@ -754,7 +768,7 @@ __device__ inline uint32_t wave_box_walk(const NodeSource& src, const float* top
       const bool bottom = reach && lvl + 1u == K;
       const bool pl = bottom && go_l, pr = bottom && go_r;
       const uint64_t lm = __ballot(pl), rm = __ballot(pr);
-      const uint32_t at = nn + (uint32_t)__popcll(lm & below) + (uint32_t)__popcll(rm & below);
+      const uint32_t at = nn + below_count(lm) + below_count(rm);
       const uint32_t nxt = cur ^ 1u;
       if (pl && at < kWaveFrontier) fr[nxt][at] = 2u * node + 1u;
       const uint32_t atr = at + (pl ? 1u : 0u);
